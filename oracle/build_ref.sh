@@ -1,0 +1,80 @@
+#!/usr/bin/env bash
+# Build the compiled-reference checker into oracle/_ref/ (test infrastructure
+# only).  The reference's C++ sources are compiled where they lie under
+# $REF; nothing is copied into the repository.  The reference configures
+# nghosts and maxlevs as compile-time constants in defs.hpp (defs.hpp:27-28),
+# so each variant gets a constants-only patched defs.hpp in a temporary
+# directory outside the repository; `-I<tmp> -I-` makes the reference's
+# `#include "defs.hpp"` resolve to it.  The temporary headers are deleted at
+# the end, so no reference source ever sits in the tree or travels with it.
+#
+# Variants: ng<G>_lv<L>      serial, -O3 -ffp-contract=off (the oracle build)
+#           ng<G>_lv<L>_omp  -fopenmp -DOPENMP, generic x86-64 (CPU baseline)
+# Output: oracle/_ref/<variant>/refrun  (harness + reference objects)
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+REF="${REF:-/root/reference}"
+OUT="$HERE/_ref"
+JOBS="${JOBS:-8}"
+VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp}"
+
+if [ ! -f "$REF/defs.hpp" ]; then
+  echo "build_ref: reference not present at $REF; skipping" >&2
+  exit 0
+fi
+
+TMP="$(mktemp -d /tmp/amr2d_refinc.XXXXXX)"
+trap 'rm -rf "$TMP"' EXIT
+
+# renames that route main.cpp's phase calls into the harness wrappers
+rename_flags() {
+  case "$1" in
+    main.cpp)          echo "-Dmain=ref_main" ;;
+    boundary.cpp)      echo "-Dboundary=ref_boundary" ;;
+    boundaryfc.cpp)    echo "-Dboundaryfc=ref_boundaryfc" ;;
+    boundaryec.cpp)    echo "-Dboundaryec=ref_boundaryec" ;;
+    CT2D.cpp)          echo "-DCT2D=ref_CT2D" ;;
+    MUSCL2D.cpp)       echo "-DMUSCL2D=ref_MUSCL2D" ;;
+    admesh.cpp)        echo "-Dadmesh=ref_admesh" ;;
+    locate_bounds.cpp) echo "-Dlocate_bounds=ref_locate_bounds" ;;
+    savedata.cpp)      echo "-Dsave4py=ref_save4py" ;;
+    criteria.cpp)      echo "-Dcriteria=ref_criteria" ;;
+    *) echo "" ;;
+  esac
+}
+
+build_variant() {
+  local v="$1"
+  local ng lv omp
+  ng="$(echo "$v" | sed -E 's/^ng([0-9]+)_.*/\1/')"
+  lv="$(echo "$v" | sed -E 's/^ng[0-9]+_lv([0-9]+).*/\1/')"
+  omp=0; [[ "$v" == *_omp ]] && omp=1
+  local inc="$TMP/$v" obj="$OUT/$v/obj" bin="$OUT/$v/refrun"
+  mkdir -p "$inc" "$obj"
+  sed -e "s/^const int nghosts=2;/const int nghosts=${ng};/" \
+      -e "s/^const int maxlevs=5;/const int maxlevs=${lv};/" "$REF/defs.hpp" > "$inc/defs.hpp"
+  grep -q "^const int nghosts=${ng};" "$inc/defs.hpp" || { echo "patch nghosts failed" >&2; return 1; }
+  grep -q "^const int maxlevs=${lv};" "$inc/defs.hpp" || { echo "patch maxlevs failed" >&2; return 1; }
+  local flags="-O3 -std=c++17 -ffp-contract=off -DDOUBLE_PREC -w"
+  if [ "$omp" = 1 ]; then flags="-O3 -std=c++17 -funroll-loops -ffp-contract=off -DDOUBLE_PREC -DOPENMP -fopenmp -w"; fi
+  # skip when up to date
+  if [ -x "$bin" ] && [ "$bin" -nt "$HERE/refharness/harness.cpp" ] && [ "$bin" -nt "$HERE/build_ref.sh" ] \
+     && [ -z "$(find "$REF" -newer "$bin" -name '*.[ch]pp' -print -quit)" ]; then
+    return 0
+  fi
+  local objs=()
+  for src in "$REF"/*.cpp; do
+    local b; b="$(basename "$src" .cpp)"
+    # shellcheck disable=SC2046
+    g++ $flags $(rename_flags "$b.cpp") -I"$inc" -I- -I"$REF" -c "$src" -o "$obj/$b.o" 2>"$obj/$b.err" || { cat "$obj/$b.err" >&2; return 1; }
+    objs+=("$obj/$b.o")
+  done
+  g++ $flags -I"$inc" -I- -I"$REF" -c "$HERE/refharness/harness.cpp" -o "$obj/harness.o" 2>"$obj/harness.err" || { cat "$obj/harness.err" >&2; return 1; }
+  g++ $flags "${objs[@]}" "$obj/harness.o" -Wl,--wrap=system -o "$bin"
+  echo "built $bin"
+}
+
+export -f build_variant rename_flags
+export REF OUT TMP HERE
+mkdir -p "$OUT"
+printf '%s\n' $VARIANTS | xargs -P "$JOBS" -I{} bash -c 'build_variant {} || exit 255'
