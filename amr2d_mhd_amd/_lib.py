@@ -1,0 +1,108 @@
+"""ctypes binding of libamr2d_mhd.so (the C ABI in include/amr2d_mhd.h).
+
+The library is built in-tree (``make -C amr2d_mhd_amd/csrc``); importing this
+module fails loudly when it is missing -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libamr2d_mhd.so")
+
+FIELDS = {"U": 0, "Us": 1, "Bi": 2, "Bis": 3, "ff": 4, "gg": 5, "EMF": 6}
+FIELD_VARS = {"U": 8, "Us": 8, "Bi": 2, "Bis": 2, "ff": 8, "gg": 8, "EMF": 1}
+FILL_KIND = {"cell": 0, "face": 1, "edge": 2}
+SOLVERS = {"RUSA": 0, "HLLE": 1, "HLLC": 2, "HLLD": 3}
+ERRORS = {0: "ok", 1: "non-positive pressure", 2: "NaN velocity", 3: "NaN flux", 4: "CT face mismatch",
+          5: "NaN edge state", 6: "HLLC wave-speed failure", 7: "invalid dt", 8: "maxblocks exceeded",
+          9: "NaN in refined face field", 10: "bad configuration", 11: "HIP runtime error",
+          12: "slot arena exhausted"}
+
+
+class SimConfig(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("nghosts", C.c_int32), ("maxlevs", C.c_int32), ("pad", C.c_int32),
+                ("lenx", C.c_double), ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16),
+                ("flux", C.c_char * 8), ("ic_params", C.c_double * 8)]
+
+
+class InstInfo(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("dt_next", C.c_double), ("gamma", C.c_double),
+                ("t_end", C.c_double), ("last_active", C.c_int32), ("nleaf", C.c_int32), ("count", C.c_int32),
+                ("done", C.c_int32), ("status", C.c_int32), ("maxblocks", C.c_int32)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("code", C.c_int32), ("inst", C.c_int32), ("nb", C.c_int32), ("i", C.c_int32),
+                ("j", C.c_int32), ("pad", C.c_int32)]
+
+
+class AmrError(RuntimeError):
+    def __init__(self, code, diag=None):
+        self.code = code
+        self.diag = diag
+        msg = f"amr2d_mhd error {code} ({ERRORS.get(code, '?')})"
+        if diag is not None:
+            msg += f" at instance {diag.inst} block {diag.nb} cell ({diag.i},{diag.j})"
+        super().__init__(msg)
+
+
+_lib = None
+
+_SIGS = {
+    "amr_create": (C.c_int, [C.c_int, C.POINTER(SimConfig), C.c_int64, C.POINTER(C.c_void_p)]),
+    "amr_destroy": (C.c_int, [C.c_void_p]),
+    "amr_init": (C.c_int, [C.c_void_p]),
+    "amr_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "amr_run": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "amr_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(InstInfo)]),
+    "amr_last_error": (C.c_int, [C.c_void_p, C.POINTER(Diag)]),
+    "amr_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "amr_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "amr_get_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_set_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_set_time": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "amr_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "amr_stage": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_ct": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_cfl": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "amr_regrid": (C.c_int, [C.c_void_p]),
+    "amr_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    "amr_riemann_batch": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    "amr_recon_batch": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "amr_debug_host_fill": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "amr_debug_coarsen_maps": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+}
+
+EXPORTED = sorted(_SIGS)
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libamr2d_mhd.so not built at {LIB_PATH}; run `make -C amr2d_mhd_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def make_config(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0,
+                ic_params=None):
+    cfg = SimConfig()
+    cfg.nx, cfg.nghosts, cfg.maxlevs, cfg.lenx, cfg.cfl = nx, nghosts, maxlevs, lenx, cfl
+    cfg.ic, cfg.limiter, cfg.flux = ic.encode(), limiter.encode(), flux.encode()
+    if ic_params is not None:
+        for q, v in enumerate(ic_params):
+            cfg.ic_params[q] = float(v)
+    return cfg

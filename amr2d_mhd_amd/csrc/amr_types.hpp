@@ -1,0 +1,88 @@
+// Shared host/device types of the AMR2D-MHD engine.
+#pragma once
+#include <cstdint>
+
+namespace amr {
+
+// limiter ids (slopelimiter.cpp names; WENO3/WENO5 select WENO2D_primvar.cpp)
+enum {
+  kLim1st = 0,
+  kLimNoSL,
+  kLimMinmod,
+  kLimVanLeer,
+  kLimSuperbee,
+  kLimMC,
+  kLimKoren,
+  kLimCada,
+  kLimWENO3,
+  kLimWENO5,
+  kLimCount
+};
+// Riemann solver ids (riemannS.cpp names)
+enum { kFluxRusanov = 0, kFluxHLLE, kFluxHLLC, kFluxHLLD, kFluxCount };
+
+// slot fields held in HBM for every slot of every instance, [slot][var][j][i]
+enum { kU = 0, kUs, kBi, kBis, kFF, kGG, kEMF, kNumFields };
+constexpr int kFieldVars[kNumFields] = {8, 8, 2, 2, 8, 8, 1};
+
+// ghost-fill combine kinds (boundary.cpp / boundaryfc.cpp / boundaryec.cpp)
+enum { kFillCell = 0, kFillFace = 1, kFillEdge = 2 };
+
+// error codes reported by kernels and by the host driver
+enum {
+  kOk = 0,
+  kErrPressure = 1,   // meshblock.cpp:136-142, 164-170
+  kErrNanVel = 2,     // meshblock.cpp:171-176
+  kErrNanFlux = 3,    // MUSCL2D.cpp:79-85, 103-108
+  kErrCtFace = 4,     // MUSCL2D.cpp:87-91, 110-114
+  kErrNanEdge = 5,    // celledges.cpp:17-21, 44-47
+  kErrHLLC = 6,       // riemannS.cpp:269-276
+  kErrDt = 7,         // main.cpp:235-238
+  kErrOverflow = 8,   // admesh.cpp:42-45
+  kErrRefineNan = 9,  // refineBi_CT.cpp:47
+  kErrConfig = 10,
+  kErrHip = 11,
+  kErrCapacity = 12
+};
+
+// one leaf block in a device work list
+struct LeafItem {
+  int32_t gs;    // global slot in the arena
+  int32_t inst;  // instance index
+  int32_t nb;    // reference block id within the instance (diagnostics)
+  int32_t lev;   // refinement level
+};
+
+// per-instance parameters read by kernels
+struct InstParams {
+  double gamma;
+  double dt;          // dt applied in the current step (meshblock::dt)
+  double dx[16];      // cell size per level (dx == dy)
+  int32_t limiter;
+  int32_t solver;
+  int32_t maxlevs;
+  int32_t pad;
+};
+
+// A ghost-fill reference: a pre-fill element (global slot, cell) or, with
+// kNodeBit set, the value of a nested restriction node evaluated earlier.
+constexpr int64_t kNodeBit = int64_t(1) << 62;
+inline __host__ __device__ int64_t src_ref(int64_t gs, int cell) { return (gs << 16) | cell; }
+
+// One destination of a ghost fill: either a copy of ref[0] or a fine->coarse
+// restriction whose four operands are the cells (i1,j1),(i1,j1+1),(i1+1,j1),
+// (i1+1,j1+1) of the fine block.  The combine rule (cell 2x2 mean, face pair
+// mean, edge injection) is chosen by the field being filled.
+struct FillOp {
+  int64_t ref[4];
+  int32_t dst_gs;
+  int16_t dst_cell;
+  int16_t op;  // 0 copy, 1 restriction
+};
+
+// device error record (first error wins)
+struct DevError {
+  int32_t code, inst, nb, i, j, pad[3];
+};
+
+}  // namespace amr

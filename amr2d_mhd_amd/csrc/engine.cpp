@@ -1,0 +1,1588 @@
+// Host engine of the AMR2D-MHD hot path: quadtree bookkeeping mirrored into
+// device work lists, the ghost-fill program compiler, the step driver and the
+// C ABI declared in include/amr2d_mhd.h.
+//
+// The solution state of every slot of every instance stays in HBM for the
+// whole run; per step only the per-leaf CFL bounds and refinement indicators
+// come back to the host, and work lists / fill programs go down when a tree
+// changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/amr2d_mhd.h"
+#include "amr_types.hpp"
+#include "kernels.hpp"
+#include "physics.hpp"
+
+using namespace amr;
+
+namespace {
+
+constexpr int kMaxLev = 16;
+constexpr int kNbRoots = 4;  // defs.hpp:29
+constexpr double kPiRef = 3.14159265359;  // defs.hpp:23
+
+int lim_code(const char* s) {
+  static const char* names[] = {"1st", "noSL", "minmod", "VanLeer", "Superbee", "MC", "Koren", "Cada3rd", "WENO3", "WENO5"};
+  if (!std::strcmp(s, "WENO2D")) return kLimNoSL;
+  for (int q = 0; q < kLimCount; q++)
+    if (!std::strcmp(s, names[q])) return q;
+  return -1;
+}
+int flux_code(const char* s) {
+  static const char* names[] = {"RUSA", "HLLE", "HLLC", "HLLD"};
+  for (int q = 0; q < kFluxCount; q++)
+    if (!std::strcmp(s, names[q])) return q;
+  return -1;
+}
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_COUNT };
+int ic_code(const char* s) {
+  static const char* names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p"};
+  for (int q = 0; q < IC_COUNT; q++)
+    if (!std::strcmp(s, names[q])) return q;
+  return -1;
+}
+
+// ------------------------------------------------------------- instance
+struct Instance {
+  amr_sim_config cfg;
+  int lim = 0, solver = 0, ic = 0, maxlevs = 1, maxblocks = 4;
+  double gamma = 2.0, tEnd = 0.2, cfl = 0.4;
+  double dx[kMaxLev] = {0};
+  // quadtree (defs.hpp:52-58): lp = level, father, first son, sibling id,
+  // left, right, bottom, top neighbour
+  std::vector<std::array<int32_t, 8>> lp;
+  std::vector<std::array<int32_t, 2>> icoord;
+  std::vector<uint8_t> leafs, iref, icoarse;
+  int lastActive = 0;
+  std::vector<int32_t> gs;  // nb -> global arena slot
+  std::vector<std::array<int32_t, 3>> sched;
+  double t = 0, dt_main = 0, dt_dom = 0;
+  int count = 0, status = 0;
+  bool done = false;
+};
+
+// --------------------------------------------------------------- context
+struct Profile {
+  double ms = 0;
+  int64_t launches = 0;
+};
+enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_COUNT };
+
+}  // namespace
+
+struct amr_ctx {
+  int NT = 20, ng = 2, nxi = 16;
+  std::vector<Instance> inst;
+  FieldPtrs F{};
+  int64_t cap_slots = 0, used_slots = 0;
+  hipStream_t stream = nullptr;
+  // device work lists and buffers (capacities in elements)
+  LeafItem* d_leaves = nullptr;
+  int nleaf = 0, cap_leaves = 0;
+  std::vector<LeafItem> h_leaves;
+  std::vector<int> leaf_inst_begin;  // per active instance, first leaf index
+  // host copies kept alive until the stream has consumed them
+  std::vector<FillOp> h_ops, h_nodes;
+  std::vector<RegridOp> h_refs, h_coars;
+  FillOp* d_ops = nullptr;
+  int nops = 0, cap_ops = 0;
+  FillOp* d_nodes = nullptr;
+  int nnodes = 0, cap_nodes = 0;
+  std::vector<int32_t> node_levels{0};
+  double* d_fillbuf = nullptr;
+  size_t cap_fillbuf = 0;
+  double* d_nodebuf = nullptr;
+  size_t cap_nodebuf = 0;
+  InstParams* d_ip = nullptr;
+  std::vector<InstParams> h_ip;
+  double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
+  int cap_red = 0;
+  DevError* d_err = nullptr;
+  DevError* h_err = nullptr;
+  RegridOp* d_rops = nullptr;
+  int cap_rops = 0;
+  int32_t *d_cmapU = nullptr, *d_cmapB = nullptr;
+  double* d_W = nullptr;  // initial-condition primitives for the first regrid
+  bool topo_dirty = true;
+  amr_diag diag{};
+  bool prof_on = false;
+  Profile prof[PF_COUNT];
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      std::fprintf(stderr, "amr2d_mhd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return AMR_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+int set_error(amr_ctx* c, int code, int inst, int nb, int i, int j) {
+  if (c->diag.code == 0) c->diag = amr_diag{code, inst, nb, i, j, 0};
+  return code;
+}
+
+template <class T>
+int ensure(T*& p, int& cap, int n) {
+  if (n <= cap) return 0;
+  if (p) hipFree(p);
+  int nc = std::max(n, cap * 2);
+  if (nc < 64) nc = 64;
+  if (hipMalloc((void**)&p, sizeof(T) * (size_t)nc) != hipSuccess) { p = nullptr; cap = 0; return AMR_ERR_HIP; }
+  cap = nc;
+  return 0;
+}
+
+int ensure_bytes(double*& p, size_t& cap, size_t n) {
+  if (n <= cap) return 0;
+  if (p) hipFree(p);
+  size_t nc = std::max(n, cap * 2);
+  if (hipMalloc((void**)&p, sizeof(double) * nc) != hipSuccess) { p = nullptr; cap = 0; return AMR_ERR_HIP; }
+  cap = nc;
+  return 0;
+}
+
+// ----------------------------------------------------- timing families
+struct Timed {
+  amr_ctx* c;
+  int fam;
+  Timed(amr_ctx* c_, int f) : c(c_), fam(f) {
+    if (c->prof_on) hipEventRecord(c->ev0, c->stream);
+  }
+  ~Timed() {
+    if (!c->prof_on) return;
+    hipEventRecord(c->ev1, c->stream);
+    hipEventSynchronize(c->ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->prof[fam].ms += ms;
+    c->prof[fam].launches += 1;
+  }
+};
+
+// -------------------------------------------------------- slot arena
+int slot_of(amr_ctx* c, Instance& I, int nb) {
+  while ((int)I.gs.size() <= nb) {
+    if (c->used_slots >= c->cap_slots) return -1;
+    I.gs.push_back((int32_t)c->used_slots++);
+  }
+  return I.gs[nb];
+}
+
+// ---------------------------------------------------- initial condition
+// IC2Dtype.cpp:6-190 for one root block: primitives of the whole tile, then
+// face B from cell averages (index 0 copies index 1).  rotor_p generalises
+// the reference rotor with (rho_in, v0, p, Bx, r0, r1).
+void initial_tile(const Instance& I, int nb, int NT, int ng, std::vector<double>& W, std::vector<double>& B) {
+  const int N2 = NT * NT;
+  W.assign(8 * N2, 0.0);
+  B.assign(2 * N2, 0.0);
+  double r4[4] = {0}, p4[4] = {0}, bx4[4] = {0}, by4[4] = {0};
+  auto set4 = [](double* a, double x0, double x1, double x2, double x3) { a[0] = x0; a[1] = x1; a[2] = x2; a[3] = x3; };
+  switch (I.ic) {
+    case IC_SST: set4(p4, 0.1, 1.0, 1.0, 0.1); set4(r4, 0.125, 1.0, 1.0, 0.125); break;
+    case IC_SST1: set4(p4, 1.0, 0.1, 0.1, 1.0); set4(r4, 1.0, 0.125, 0.125, 1.0); break;
+    case IC_SSTY: set4(p4, 0.1, 0.1, 1.0, 1.0); set4(r4, 0.125, 0.125, 1.0, 1.0); break;
+    case IC_BWX: set4(p4, 0.1, 1.0, 1.0, 0.1); set4(r4, 0.125, 1.0, 1.0, 0.125);
+      set4(bx4, 0.75, 0.75, 0.75, 0.75); set4(by4, -1.0, 1.0, 1.0, -1.0); break;
+    case IC_BWX1: set4(p4, 1.0, 0.1, 0.1, 1.0); set4(r4, 1.0, 0.125, 0.125, 1.0);
+      set4(bx4, 0.75, 0.75, 0.75, 0.75); set4(by4, 1.0, -1.0, -1.0, 1.0); break;
+    case IC_BWY: set4(p4, 0.1, 0.1, 1.0, 1.0); set4(r4, 0.125, 0.125, 1.0, 1.0);
+      set4(by4, 0.75, 0.75, 0.75, 0.75); set4(bx4, -1.0, -1.0, 1.0, 1.0); break;
+    case IC_BWY1: set4(p4, 1.0, 1.0, 0.1, 0.1); set4(r4, 1.0, 1.0, 0.125, 0.125);
+      set4(by4, 0.75, 0.75, 0.75, 0.75); set4(bx4, 1.0, 1.0, -1.0, -1.0); break;
+    default: break;
+  }
+  const int lev = I.lp[nb][0];
+  const double dxl = I.dx[lev], dx0 = I.dx[0];
+  const double* rp = I.cfg.ic_params;
+  for (int i = 0; i < NT; i++) {
+    const double x = ((i + I.icoord[nb][0] - ng) + 0.5) * dxl - dx0;
+    for (int j = 0; j < NT; j++) {
+      const double y = ((j + I.icoord[nb][1] - ng) + 0.5) * dxl - dx0;
+      double w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (I.ic == IC_ROTOR) {
+        const double rad = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
+        const double fr = (23. - 200. * rad) / 3.;
+        w[4] = 0.5;
+        w[5] = 2.5 / (sqrt(4 * kPiRef));
+        if (rad <= 0.1) {
+          w[0] = 10.;
+          w[1] = -(y - 0.5) / 0.1;
+          w[2] = (x - 0.5) / 0.1;
+        } else if (rad > 0.1 && rad < 0.115) {
+          w[0] = (1 + 9. * fr);
+          w[1] = (-(y - 0.5) * fr / 0.1);
+          w[2] = (x - 0.5) * fr / 0.1;
+        } else {
+          w[0] = 1.;
+        }
+      } else if (I.ic == IC_ROTOR_P) {
+        const double rho_in = rp[0], v0 = rp[1], r0 = rp[4], r1 = rp[5];
+        const double rad = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
+        const double fr = (r1 - rad) / (r1 - r0);
+        w[4] = rp[2];
+        w[5] = rp[3];
+        if (rad <= r0) {
+          w[0] = rho_in;
+          w[1] = -v0 * (y - 0.5) / r0;
+          w[2] = v0 * (x - 0.5) / r0;
+        } else if (rad > r0 && rad < r1) {
+          w[0] = 1. + (rho_in - 1.) * fr;
+          w[1] = -v0 * (y - 0.5) * fr / r0;
+          w[2] = v0 * (x - 0.5) * fr / r0;
+        } else {
+          w[0] = 1.;
+        }
+      } else {
+        int q;
+        if (x >= 0.5 && y >= 0.5) q = 0;
+        else if (x < 0.5 && y >= 0.5) q = 1;
+        else if (x < 0.5 && y < 0.5) q = 2;
+        else q = 3;
+        w[0] = r4[q];
+        w[4] = p4[q];
+        w[5] = bx4[q];
+        w[6] = by4[q];
+      }
+      for (int k = 0; k < 8; k++) W[k * N2 + j * NT + i] = w[k];
+    }
+  }
+  for (int i = 0; i < NT; i++)
+    for (int j = 0; j < NT; j++) {
+      if (i < NT - 1) B[j * NT + i + 1] = 0.5 * (W[5 * N2 + j * NT + i] + W[5 * N2 + j * NT + i + 1]);
+      else B[j * NT + 0] = B[j * NT + 1];
+      if (j < NT - 1) B[N2 + (j + 1) * NT + i] = 0.5 * (W[6 * N2 + j * NT + i] + W[6 * N2 + (j + 1) * NT + i]);
+      else B[N2 + 0 * NT + i] = B[N2 + 1 * NT + i];
+    }
+}
+
+// meshblock.cpp:181-199
+void prim_to_cons_tile(const std::vector<double>& W, std::vector<double>& U, int N2, double g) {
+  U.assign(8 * N2, 0.0);
+  for (int c = 0; c < N2; c++) {
+    const double w0 = W[c], w1 = W[N2 + c], w2 = W[2 * N2 + c], w3 = W[3 * N2 + c];
+    const double b5 = W[5 * N2 + c], b6 = W[6 * N2 + c], b7 = W[7 * N2 + c];
+    U[c] = w0;
+    U[N2 + c] = w1 * w0;
+    U[2 * N2 + c] = w2 * w0;
+    U[3 * N2 + c] = w3 * w0;
+    const double e = (W[4 * N2 + c] / (g - 1)) + 0.5 * w0 * (w1 * w1 + w2 * w2 + w3 * w3);
+    U[4 * N2 + c] = e + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
+    U[5 * N2 + c] = b5;
+    U[6 * N2 + c] = b6;
+    U[7 * N2 + c] = b7;
+  }
+}
+
+// ------------------------------------------------------ fill schedule
+// locate_bounds.cpp:35-208: ordered (block, neighbour, code) list; every
+// shared side of two leaves appears once, from the first block in id order
+// (or from the coarser block for a level jump).
+void build_schedule(Instance& I) {
+  I.sched.clear();
+  std::vector<uint8_t> seen((size_t)I.lastActive * 4, 0);
+  // per side: neighbour slot in lp, opposite side, codes, son offset of the
+  // finer neighbour's first adjacent son and the stride to the second
+  struct SideInfo { int opp, box, same, finer, son, stride; };
+  static const SideInfo S[4] = {{1, -1, 1, 2, 1, 2}, {0, -2, 4, 5, 0, 2}, {3, -3, 7, 8, 2, 1}, {2, -4, 10, 11, 0, 1}};
+  for (int nb = 0; nb < I.lastActive; nb++) {
+    if (!I.leafs[nb]) continue;
+    for (int s = 0; s < 4; s++) {
+      if (seen[nb * 4 + s]) continue;
+      const int n = I.lp[nb][4 + s];
+      if (n == -1) {
+        seen[nb * 4 + s] = 1;
+        I.sched.push_back({nb, -1, S[s].box});
+      } else if (I.lp[n][0] >= I.lp[nb][0]) {
+        seen[nb * 4 + s] = 1;
+        if (I.leafs[n]) {
+          I.sched.push_back({nb, n, S[s].same});
+          seen[n * 4 + S[s].opp] = 1;
+        } else {
+          const int f1 = I.lp[n][2] + S[s].son, f2 = f1 + S[s].stride;
+          I.sched.push_back({nb, f1, S[s].finer});
+          seen[f1 * 4 + S[s].opp] = 1;
+          seen[f2 * 4 + S[s].opp] = 1;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------- fill compiler
+// Symbolic execution of the reference's sequential ghost fill
+// (boundary.cpp / boundaryfc.cpp / boundaryec.cpp, extboundary.cpp): each
+// element holds a symbolic value -- its own pre-fill value, a pre-fill value
+// of another element (copies collapse) or a restriction node over four
+// elements.  The final symbolic value of every written element becomes one
+// FillOp, evaluated on the device as gather (from the pre-fill state) and
+// scatter, which reproduces the sequential last-writer semantics without
+// races.  One program serves all fields: only the restriction arithmetic
+// depends on the field kind.
+struct FillCompiler {
+  const Instance& I;
+  int NT, ng, N2;
+  std::vector<int64_t> cur;       // per (nb, cell): symbolic value, -1 = untouched
+  std::vector<int32_t> touched;   // elements written, in first-write order
+  std::vector<FillOp>& nodes;     // restriction nodes (refs may point to nodes)
+
+  FillCompiler(const Instance& inst, int NT_, int ng_, std::vector<FillOp>& nodes_)
+      : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), nodes(nodes_) {
+    cur.assign((size_t)I.lastActive * N2, -1);
+  }
+  int64_t read(int nb, int i, int j) const {
+    const int64_t v = cur[(size_t)nb * N2 + j * NT + i];
+    return v >= 0 ? v : src_ref(I.gs[nb], j * NT + i);
+  }
+  void write(int nb, int i, int j, int64_t v) {
+    const size_t a = (size_t)nb * N2 + j * NT + i;
+    if (cur[a] < 0) touched.push_back((int32_t)a);
+    cur[a] = v;
+  }
+  void copy(int dn, int di, int dj, int sn, int si, int sj) { write(dn, di, dj, read(sn, si, sj)); }
+  void restrict_(int dn, int di, int dj, int fn, int i1, int j1) {
+    FillOp op{};
+    op.ref[0] = read(fn, i1, j1);
+    op.ref[1] = read(fn, i1, j1 + 1);
+    op.ref[2] = read(fn, i1 + 1, j1);
+    op.ref[3] = read(fn, i1 + 1, j1 + 1);
+    op.op = 1;
+    nodes.push_back(op);
+    write(dn, di, dj, kNodeBit | (int64_t)(nodes.size() - 1));
+  }
+
+  // corner of an entry: the diagonal block `cb` found through the neighbour
+  // table is copied when it is a leaf (with same-level indices even if it is
+  // coarser), else its designated son is restricted when that is a leaf.
+  template <class Dst, class SameIdx, class FineIdx>
+  void corner(int nbs, int cb, int son_off, int ilo, int ihi, int jlo, int jhi, Dst dst, SameIdx same, FineIdx fine) {
+    if (cb == -1) return;
+    if (I.leafs[cb]) {
+      for (int i = ilo; i < ihi; i++)
+        for (int j = jlo; j < jhi; j++) {
+          int di, dj, si, sj;
+          dst(i, j, di, dj);
+          same(i, j, si, sj);
+          copy(nbs, di, dj, cb, si, sj);
+        }
+      return;
+    }
+    const int fb = I.lp[cb][2] + son_off;
+    if (!I.leafs[fb]) return;
+    for (int i = ilo; i < ihi; i++)
+      for (int j = jlo; j < jhi; j++) {
+        int di, dj, i1, j1;
+        dst(i, j, di, dj);
+        fine(i, j, i1, j1);
+        restrict_(nbs, di, dj, fb, i1, j1);
+      }
+  }
+
+  void run() {
+    const int lo = ng, hi = NT - 1 - ng, lob = lo - 1, hib = hi + 1, h2 = (NT - 1) / 2, top = hib;
+    auto id = [](int i, int j, int& a, int& b) { a = i; b = j; };
+    for (const auto& e : I.sched) {
+      const int nbs = e[0], n1 = e[1];
+      switch (e[2]) {
+        case 1:   // left neighbour, same level (boundary.cpp:17-35)
+        case 4:   // right neighbour, same level (boundary.cpp:109-127)
+        {
+          const bool left = e[2] == 1;
+          for (int j = 0; j < NT; j++)
+            for (int g = 0; g < ng; g++) {
+              if (left) {
+                copy(nbs, g, j, n1, hi - (ng - 1) + g, j);
+                copy(n1, top + g, j, nbs, lo + g, j);
+              } else {
+                copy(nbs, top + g, j, n1, lo + g, j);
+                copy(n1, g, j, nbs, hi - (ng - 1) + g, j);
+              }
+            }
+          break;
+        }
+        case 7:   // bottom neighbour, same level (boundary.cpp:199-217)
+        case 10:  // top neighbour, same level (boundary.cpp:289-307)
+        {
+          const bool bottom = e[2] == 7;
+          for (int i = 0; i < NT; i++)
+            for (int g = 0; g < ng; g++) {
+              if (bottom) {
+                copy(nbs, i, g, n1, i, hi - (ng - 1) + g);
+                copy(n1, i, top + g, nbs, i, lo + g);
+              } else {
+                copy(nbs, i, top + g, n1, i, lo + g);
+                copy(n1, i, g, nbs, i, hi - (ng - 1) + g);
+              }
+            }
+          break;
+        }
+        case 2: {  // finer neighbours on the left (boundary.cpp:36-107)
+          const int n2 = n1 + 2;
+          for (int j = lo; j <= h2; j++)
+            for (int i = 0; i < ng; i++) {
+              const int i1 = 2 * (i - ng / 2) - ng / 2 + hi, j1 = 2 * j - lo;
+              restrict_(nbs, i, j, n1, i1, j1);
+              restrict_(nbs, i, h2 + j - 1 - (ng - 2), n2, i1, j1);
+            }
+          corner(nbs, I.lp[I.lp[n1][1]][6], 3, 0, ng, 0, ng, id,
+                 [&](int i, int j, int& a, int& b) { a = hi - ng + i + 1; b = hi - ng + j + 1; },
+                 [&](int i, int j, int& a, int& b) { a = hi - 2 * ng + 2 * i + 1; b = hi - 2 * ng + 2 * j + 1; });
+          corner(nbs, I.lp[I.lp[n2][1]][7], 1, 0, ng, 0, ng,
+                 [&](int i, int j, int& a, int& b) { a = i; b = NT - 1 - j; },
+                 [&](int i, int j, int& a, int& b) { a = hi - ng + i + 1; b = lo + ng - (j + 1); },
+                 [&](int i, int j, int& a, int& b) { a = hi - 2 * ng + 2 * i + 1; b = lo + 2 * ng - 2 * (j + 1); });
+          for (int i = 1; i <= ng; i++)
+            for (int j = 0; j < NT; j++) {
+              copy(n1, i + hi, j, nbs, (i + 2 * ng - 1) / 2, (j + ng) / 2);
+              copy(n2, i + hi, j, nbs, (i + 2 * ng - 1) / 2, (j - ng + 2) / 2 + h2);
+            }
+          break;
+        }
+        case 5: {  // finer neighbours on the right (boundary.cpp:128-197)
+          const int n2 = n1 + 2;
+          for (int i = top; i < NT; i++)
+            for (int j = lo; j <= h2; j++) {
+              const int i1 = 2 * (i - hi + (ng / 2 - 1)), j1 = 2 * j - lo;
+              restrict_(nbs, i, j, n1, i1, j1);
+              restrict_(nbs, i, h2 + j - 1 - (ng - 2), n2, i1, j1);
+            }
+          corner(nbs, I.lp[I.lp[n1][1]][6], 2, hib, NT, 0, ng, id,
+                 [&](int i, int j, int& a, int& b) { a = lob + (i - hi); b = hi - ng + j + 1; },
+                 [&](int i, int j, int& a, int& b) { a = lob + 2 * (i - hi) - 1; b = hi - 2 * ng + 2 * j + 1; });
+          corner(nbs, I.lp[I.lp[n2][1]][7], 0, hib, NT, 0, ng,
+                 [&](int i, int j, int& a, int& b) { a = i; b = NT - 1 - j; },
+                 [&](int i, int j, int& a, int& b) { a = lob + (i - hi); b = lo + ng - (j + 1); },
+                 [&](int i, int j, int& a, int& b) { a = lob + 2 * (i - hi) - 1; b = lo + 2 * ng - 2 * (j + 1); });
+          for (int i = 1; i <= ng; i++)
+            for (int j = 0; j < NT; j++) {
+              copy(n1, i - 1, j, nbs, (i + 1) / 2 + hi - ng / 2, (j + ng) / 2);
+              copy(n2, i - 1, j, nbs, (i + 1) / 2 + hi - ng / 2, (j - ng + 2) / 2 + h2);
+            }
+          break;
+        }
+        case 8: {  // finer neighbours below (boundary.cpp:218-287)
+          const int n2 = n1 + 1;
+          for (int i = lo; i <= h2; i++)
+            for (int j = 0; j < ng; j++) {
+              const int i1 = 2 * i - lo, j1 = 2 * (j - ng / 2) - ng / 2 + hi;
+              restrict_(nbs, i, j, n1, i1, j1);
+              restrict_(nbs, h2 + i - 1 - (ng - 2), j, n2, i1, j1);
+            }
+          corner(nbs, I.lp[I.lp[n1][1]][4], 3, 0, ng, 0, ng, id,
+                 [&](int i, int j, int& a, int& b) { a = hi - ng + i + 1; b = hi - ng + j + 1; },
+                 [&](int i, int j, int& a, int& b) { a = hi - 2 * ng + 2 * i + 1; b = hi - 2 * ng + 2 * j + 1; });
+          corner(nbs, I.lp[I.lp[n2][1]][5], 2, 0, ng, 0, ng,
+                 [&](int i, int j, int& a, int& b) { a = NT - 1 - i; b = j; },
+                 [&](int i, int j, int& a, int& b) { a = lo + ng - (i + 1); b = hi - ng + j + 1; },
+                 [&](int i, int j, int& a, int& b) { a = lo + 2 * ng - 2 * (i + 1); b = hi - 2 * ng + 2 * j + 1; });
+          for (int j = 1; j <= ng; j++)
+            for (int i = 0; i < NT; i++) {
+              copy(n1, i, j + hi, nbs, (i + ng) / 2, (j + 2 * ng - 1) / 2);
+              copy(n2, i, j + hi, nbs, (i - ng + 2) / 2 + h2, (j + 2 * ng - 1) / 2);
+            }
+          break;
+        }
+        case 11: {  // finer neighbours above (boundary.cpp:308-377)
+          const int n2 = n1 + 1;
+          for (int i = lo; i <= h2; i++)
+            for (int j = top; j < NT; j++) {
+              const int i1 = 2 * i - lo, j1 = 2 * (j - hi + (ng / 2 - 1));
+              restrict_(nbs, i, j, n1, i1, j1);
+              restrict_(nbs, h2 + i - 1 - (ng - 2), j, n2, i1, j1);
+            }
+          corner(nbs, I.lp[I.lp[n1][1]][4], 1, 0, ng, hib, NT, id,
+                 [&](int i, int j, int& a, int& b) { a = hi - ng + i + 1; b = lob + (j - hi); },
+                 [&](int i, int j, int& a, int& b) { a = hi - 2 * ng + 2 * i + 1; b = lob + 2 * (j - hi) - 1; });
+          {  // right corner: the same-level copy mirrors i, the finer restriction does not
+            const int cb = I.lp[I.lp[n2][1]][5];
+            if (cb != -1) {
+              if (I.leafs[cb]) {
+                for (int i = 0; i < ng; i++)
+                  for (int j = hib; j < NT; j++) copy(nbs, NT - 1 - i, j, cb, lo + ng - (i + 1), lob + (j - hi));
+              } else {
+                const int fb = I.lp[cb][2];
+                if (I.leafs[fb])
+                  for (int i = hib; i < NT; i++)
+                    for (int j = hib; j < NT; j++)
+                      restrict_(nbs, i, j, fb, lob + 2 * (i - hi) - 1, lob + 2 * (j - hi) - 1);
+              }
+            }
+          }
+          for (int j = 1; j <= ng; j++)
+            for (int i = 0; i < NT; i++) {
+              copy(n1, i, j - 1, nbs, (i + ng) / 2, (j + 1) / 2 + hi - ng / 2);
+              copy(n2, i, j - 1, nbs, (i - ng + 2) / 2 + h2, (j + 1) / 2 + hi - ng / 2);
+            }
+          break;
+        }
+        default: break;
+      }
+    }
+    // extboundary.cpp:5-61: mirror of the first ng interior rows/columns,
+    // applied to every field kind alike, after all internal entries
+    for (const auto& e : I.sched) {
+      const int nbs = e[0];
+      switch (e[2]) {
+        case -1:
+          for (int g = 0; g < ng; g++)
+            for (int j = 0; j < NT; j++) copy(nbs, g, j, nbs, lo + ng - 1 - g, j);
+          break;
+        case -2:
+          for (int g = 0; g < ng; g++)
+            for (int j = 0; j < NT; j++) copy(nbs, top + g, j, nbs, hi - g, j);
+          break;
+        case -3:
+          for (int g = 0; g < ng; g++)
+            for (int i = 0; i < NT; i++) copy(nbs, i, g, nbs, i, lo + ng - 1 - g);
+          break;
+        case -4:
+          for (int g = 0; g < ng; g++)
+            for (int i = 0; i < NT; i++) copy(nbs, i, top + g, nbs, i, hi - g);
+          break;
+        default: break;
+      }
+    }
+  }
+};
+
+// Flatten nested restriction nodes into levels (a node whose operands are
+// all pre-fill elements has level 1).  Destination ops referencing a node
+// inline it; only nodes referenced from inside other ops are materialised.
+struct ProgramBuilder {
+  std::vector<FillOp> ops, nested;
+  std::vector<int32_t> level_start{0};
+};
+
+int compile_programs(amr_ctx* c, ProgramBuilder& pb) {
+  std::vector<FillOp> all_nodes;
+  std::vector<FillOp> dest_ops;
+  std::vector<int32_t> node_base_of_inst;
+  for (auto& I : c->inst) {
+    if (I.done) continue;
+    const size_t base = all_nodes.size();
+    std::vector<FillOp> nodes;
+    FillCompiler fc(I, c->NT, c->ng, nodes);
+    fc.run();
+    auto rebase = [&](int64_t r) { return (r & kNodeBit) ? (kNodeBit | ((r & ~kNodeBit) + (int64_t)base)) : r; };
+    for (auto& n : nodes)
+      for (auto& r : n.ref) r = rebase(r);
+    all_nodes.insert(all_nodes.end(), nodes.begin(), nodes.end());
+    const int N2 = c->NT * c->NT;
+    for (int32_t a : fc.touched) {
+      const int nb = a / N2, cell = a % N2;
+      const int64_t v = rebase(fc.cur[a]);
+      FillOp op{};
+      op.dst_gs = I.gs[nb];
+      op.dst_cell = (int16_t)cell;
+      if (v & kNodeBit) {
+        op = all_nodes[v & ~kNodeBit];
+        op.dst_gs = I.gs[nb];
+        op.dst_cell = (int16_t)cell;
+      } else {
+        if (v == src_ref(I.gs[nb], cell)) continue;  // value unchanged
+        op.op = 0;
+        op.ref[0] = v;
+      }
+      dest_ops.push_back(op);
+    }
+  }
+  // levels of nodes referenced from destination ops (recursively)
+  std::vector<int> level(all_nodes.size(), 0);
+  std::vector<int> stack;
+  std::function<int(int64_t)> lvl = [&](int64_t r) -> int {
+    if (!(r & kNodeBit)) return 0;
+    const int64_t n = r & ~kNodeBit;
+    if (level[n] > 0) return level[n];
+    int m = 0;
+    for (auto rr : all_nodes[n].ref) m = std::max(m, lvl(rr));
+    level[n] = m + 1;
+    return level[n];
+  };
+  std::vector<int64_t> needed;
+  for (auto& op : dest_ops)
+    for (int q = 0; q < (op.op ? 4 : 1); q++)
+      if (op.ref[q] & kNodeBit) needed.push_back(op.ref[q] & ~kNodeBit);
+  std::vector<uint8_t> mark(all_nodes.size(), 0);
+  std::vector<int64_t> order;
+  while (!needed.empty()) {
+    int64_t n = needed.back();
+    needed.pop_back();
+    if (mark[n]) continue;
+    mark[n] = 1;
+    order.push_back(n);
+    for (auto r : all_nodes[n].ref)
+      if (r & kNodeBit) needed.push_back(r & ~kNodeBit);
+  }
+  int maxl = 0;
+  for (auto n : order) maxl = std::max(maxl, lvl(kNodeBit | n));
+  std::vector<int64_t> newid(all_nodes.size(), -1);
+  pb.nested.clear();
+  pb.level_start.assign(1, 0);
+  for (int l = 1; l <= maxl; l++) {
+    for (auto n : order)
+      if (level[n] == l) {
+        newid[n] = (int64_t)pb.nested.size();
+        pb.nested.push_back(all_nodes[n]);
+      }
+    pb.level_start.push_back((int32_t)pb.nested.size());
+  }
+  auto remap = [&](int64_t r) { return (r & kNodeBit) ? (kNodeBit | newid[r & ~kNodeBit]) : r; };
+  for (auto& n : pb.nested)
+    for (auto& r : n.ref) r = remap(r);
+  for (auto& op : dest_ops)
+    for (auto& r : op.ref) r = remap(r);
+  pb.ops.swap(dest_ops);
+  return 0;
+}
+
+// ------------------------------------------------- tree bookkeeping
+// updatelpup.cpp:4-118: link four new sons of nb into the neighbour table
+void link_sons(Instance& I, int nb, const int s[4]) {
+  auto& lp = I.lp;
+  const int lev = lp[nb][0] + 1;
+  // adjacent son of a refined neighbour, or the neighbour itself
+  auto outer = [&](int son, int side, int nsr_off, int back_side) {
+    const int n = lp[nb][side];
+    if (n == -1 || I.leafs[n]) {
+      lp[son][side] = n;
+    } else {
+      const int nsr = lp[n][2] + nsr_off;
+      lp[son][side] = nsr;
+      lp[nsr][back_side] = son;
+    }
+  };
+  for (int q = 0; q < 4; q++) {
+    lp[s[q]][0] = lev;
+    lp[s[q]][1] = nb;
+    lp[s[q]][3] = q;
+  }
+  lp[s[0]][5] = s[1]; lp[s[0]][7] = s[2];
+  outer(s[0], 4, 1, 5);
+  outer(s[0], 6, 2, 7);
+  lp[s[1]][4] = s[0]; lp[s[1]][7] = s[3];
+  outer(s[1], 5, 0, 4);
+  outer(s[1], 6, 3, 7);
+  lp[s[2]][5] = s[3]; lp[s[2]][6] = s[0];
+  outer(s[2], 4, 3, 5);
+  outer(s[2], 7, 0, 6);
+  lp[s[3]][4] = s[2]; lp[s[3]][6] = s[1];
+  outer(s[3], 5, 2, 4);
+  outer(s[3], 7, 1, 6);
+  I.leafs[nb] = 0;
+  for (int q = 0; q < 4; q++) {
+    I.leafs[s[q]] = 1;
+    I.icoarse[s[q]] = 0;
+  }
+  I.iref[nb] = 0;
+  I.icoarse[nb] = 0;
+}
+
+// updatelpdown.cpp:5-54
+void unlink_sons(Instance& I, int dad, const int s[4]) {
+  auto& lp = I.lp;
+  I.leafs[dad] = 1;
+  for (int q = 0; q < 4; q++) I.leafs[s[q]] = 0;
+  // side, first son id offset to reach the adjacent column/row, pointer to fix, step to the second son
+  int n = lp[dad][4];
+  if (n != -1 && !I.leafs[n]) { n = lp[lp[n][2]][5]; lp[n][5] = dad; lp[lp[n][7]][5] = dad; }
+  n = lp[dad][5];
+  if (n != -1 && !I.leafs[n]) { n = lp[n][2]; lp[n][4] = dad; lp[lp[n][7]][4] = dad; }
+  n = lp[dad][6];
+  if (n != -1 && !I.leafs[n]) { n = lp[lp[n][2]][7]; lp[n][7] = dad; lp[lp[n][5]][7] = dad; }
+  n = lp[dad][7];
+  if (n != -1 && !I.leafs[n]) { n = lp[n][2]; lp[n][6] = dad; lp[lp[n][5]][6] = dad; }
+}
+
+void grow_tree(Instance& I, int n) {
+  if ((int)I.lp.size() < n) {
+    I.lp.resize(n, std::array<int32_t, 8>{0, 0, 0, 0, 0, 0, 0, 0});
+    I.icoord.resize(n, std::array<int32_t, 2>{0, 0});
+    I.leafs.resize(n, 0);
+    I.iref.resize(n, 0);
+    I.icoarse.resize(n, 0);
+  }
+}
+
+// admesh.cpp:17-102 given the per-leaf indicator (criteria.cpp:151-163),
+// with critneighup.cpp / critneighdown.cpp.  Emits the data-transfer ops.
+int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std::vector<RegridOp>& refs,
+                  std::vector<RegridOp>& coars) {
+  Instance& I = c->inst[ii];
+  auto& lp = I.lp;
+  for (int nb = 0; nb < I.lastActive; nb++) { I.iref[nb] = 0; I.icoarse[nb] = 0; }
+  for (int nb = 0; nb < I.lastActive; nb++) {
+    if (!(I.leafs[nb] && lp[nb][0] >= 0)) continue;
+    if (grad_of_nb[nb] >= 0.05) {
+      if (lp[nb][0] < I.maxlevs - 1) { I.iref[nb] = 1; I.icoarse[nb] = 0; }
+    } else {
+      I.icoarse[nb] = 1;
+    }
+  }
+  for (int nl = I.maxlevs - 1; nl >= 0; nl--)
+    for (int nb = 0; nb < I.lastActive; nb++)
+      if (I.iref[nb] && lp[nb][0] == nl)
+        for (int s = 4; s < 8; s++) {
+          const int n1 = lp[nb][s];
+          if (n1 != -1 && lp[n1][0] < lp[nb][0]) I.iref[n1] = 1;
+        }
+  const int side = c->NT - 2 * c->ng;
+  for (int nl = 0; nl < I.maxlevs - 1; nl++)
+    for (int nb = 0; nb < I.lastActive; nb++) {
+      if (!(lp[nb][0] == nl && I.iref[nb])) continue;
+      int s[4];
+      if (lp[nb][2] == 0) {
+        s[0] = I.lastActive;
+        s[1] = s[0] + 1; s[2] = s[0] + 2; s[3] = s[0] + 3;
+        lp[nb][2] = s[0];
+        I.lastActive += 4;
+        if (I.lastActive - 1 >= I.maxblocks) return set_error(c, AMR_ERR_OVERFLOW, ii, nb, 0, 0);
+        grow_tree(I, I.lastActive);
+      } else {
+        s[0] = lp[nb][2]; s[1] = lp[s[0]][5]; s[2] = lp[s[0]][7]; s[3] = lp[s[2]][5];
+      }
+      RegridOp op{};
+      op.parent_gs = slot_of(c, I, nb);
+      for (int q = 0; q < 4; q++) {
+        op.son_gs[q] = slot_of(c, I, s[q]);
+        if (op.son_gs[q] < 0) return set_error(c, AMR_ERR_CAPACITY, ii, s[q], 0, 0);
+      }
+      op.inst = ii;
+      op.nb = nb;
+      refs.push_back(op);
+      // refine.cpp:61-84
+      const int x0 = I.icoord[nb][0] * 2, y0 = I.icoord[nb][1] * 2;
+      for (int q = 0; q < 4; q++) {
+        I.icoord[s[q]][0] = x0 + ((q & 1) ? side : 0);
+        I.icoord[s[q]][1] = y0 + ((q & 2) ? side : 0);
+      }
+      link_sons(I, nb, s);
+    }
+  for (int nb = 0; nb < I.lastActive; nb++) {
+    if (!I.icoarse[nb]) continue;
+    for (int s = 4; s < 8; s++) {
+      const int n1 = lp[nb][s];
+      if (n1 != -1 && !I.leafs[n1]) I.icoarse[nb] = 0;
+    }
+  }
+  for (int nl = I.maxlevs - 1; nl >= 1; nl--)
+    for (int nb = 0; nb < I.lastActive; nb++) {
+      if (!(I.icoarse[nb] && lp[nb][3] == 0 && lp[nb][0] == nl)) continue;
+      const int dad = lp[nb][1];
+      int s[4] = {nb, lp[nb][5], lp[nb][7], 0};
+      s[3] = lp[s[1]][7];
+      if (!(I.icoarse[s[1]] && I.icoarse[s[2]] && I.icoarse[s[3]])) continue;
+      RegridOp op{};
+      op.parent_gs = slot_of(c, I, dad);
+      for (int q = 0; q < 4; q++) op.son_gs[q] = slot_of(c, I, s[q]);
+      op.inst = ii;
+      op.nb = dad;
+      coars.push_back(op);
+      unlink_sons(I, dad, s);
+    }
+  return 0;
+}
+
+// coarsen.cpp:14-148 / coarsenBi_CT.cpp:14-306 replayed once per tile size:
+// for each dad cell the last write's son and lower-left source cell.
+void coarsen_maps(int NT, int ng, std::vector<int32_t>& mU, std::vector<int32_t>& mB) {
+  const int N2 = NT * NT, lo = ng, lob = lo - 1, hib = NT - ng, h2 = (NT - 1) / 2;
+  mU.assign(N2, -1);
+  mB.assign(2 * N2, -1);
+  auto pk = [](int s, int i1, int j1) { return s | (i1 << 8) | (j1 << 16); };
+  auto wU = [&](int i, int j, int s, int i1, int j1) { mU[j * NT + i] = pk(s, i1, j1); };
+  auto wB = [&](int k, int i, int j, int s, int i1, int j1) { mB[k * N2 + j * NT + i] = pk(s, i1, j1); };
+  // interior (both fields)
+  for (int i = lo; i <= h2; i++)
+    for (int j = lo; j <= h2; j++) {
+      const int i1 = 2 * i - lo, j1 = 2 * j - lo, i2 = h2 + i - 1 - (ng - 2), j2 = h2 + j - 1 - (ng - 2);
+      wU(i, j, 0, i1, j1); wU(i2, j, 1, i1, j1); wU(i, j2, 2, i1, j1); wU(i2, j2, 3, i1, j1);
+      for (int k = 0; k < 2; k++) {
+        wB(k, i, j, 0, i1, j1); wB(k, i2, j, 1, i1, j1); wB(k, i, j2, 2, i1, j1); wB(k, i2, j2, 3, i1, j1);
+      }
+    }
+  // cell field ghost ring (coarsen.cpp:33-148)
+  for (int i = 0; i < ng / 2; i++)
+    for (int j = 0; j < ng / 2; j++) {
+      wU(lob - i, lob - j, 0, lob - 2 * i - 1, lob - 2 * j - 1);
+      wU(lob - i, hib + j, 2, lob - 2 * i - 1, hib + 2 * j);
+    }
+  for (int i = 0; i <= lob; i++)
+    for (int j = lo; j <= h2; j++) {
+      int i1 = lob - 2 * i - 1;
+      if (i1 < 0) i1 = 0;
+      wU(lob - i, j, 0, i1, 2 * j - lo);
+      wU(lob - i, j + h2 - 1 - (ng - 2), 2, i1, 2 * j - lo);
+    }
+  for (int i = 0; i < ng / 2; i++)
+    for (int j = 0; j < ng / 2; j++) {
+      wU(hib + i, lob - j, 1, hib + 2 * i, lob - 2 * j - 1);
+      wU(hib + i, hib + j, 3, hib + 2 * i, hib + 2 * j);
+    }
+  for (int i = 0; i < ng; i++)
+    for (int j = lo; j <= h2; j++) {
+      int i1 = hib + 2 * i;
+      if (i1 + 1 > NT - 1) i1 = NT - 2;
+      wU(hib + i, j, 1, i1, 2 * j - lo);
+      wU(hib + i, j + h2 - 1 - (ng - 2), 3, i1, 2 * j - lo);
+    }
+  for (int j = 0; j < ng / 2; j++)
+    for (int i = 0; i < ng / 2; i++) {
+      wU(lob - i, lob - j, 0, lob - 2 * i - 1, lob - 2 * j - 1);
+      wU(hib + i, lob - j, 1, hib + 2 * i, lob - 2 * j - 1);
+    }
+  for (int i = lo; i <= h2; i++)
+    for (int j = 0; j <= lob; j++) {
+      int j1 = lob - 2 * j - 1;
+      if (j1 < 0) j1 = 0;
+      wU(i, lob - j, 0, 2 * i - lo, j1);
+      wU(i + h2 - 1 - (ng - 2), lob - j, 1, 2 * i - lo, j1);
+    }
+  for (int j = 0; j < ng / 2; j++)
+    for (int i = 0; i < ng / 2; i++) {
+      wU(lob - i, hib + j, 2, lob - 2 * i - 1, hib + 2 * j);
+      wU(hib + i, hib + j, 3, hib + 2 * i, hib + 2 * j);
+    }
+  for (int i = lo; i <= h2; i++)
+    for (int j = 0; j < ng; j++) {
+      int j1 = hib + 2 * j;
+      if (j1 + 1 > NT - 1) j1 = NT - 2;
+      wU(i, hib + j, 2, 2 * i - lo, j1);
+      wU(i + h2 - 1 - (ng - 2), hib + j, 3, 2 * i - lo, j1);
+    }
+  // face field ghost ring (coarsenBi_CT.cpp:189-306): same cells and clamps;
+  // the pairing direction is applied on the device by component
+  for (int k = 0; k < 2; k++) {
+    for (int i = 0; i < ng / 2; i++)
+      for (int j = 0; j < ng / 2; j++) {
+        wB(k, lob - i, lob - j, 0, lob - 2 * i - 1, lob - 2 * j - 1);
+        wB(k, lob - i, hib + j, 2, lob - 2 * i - 1, hib + 2 * j);
+      }
+    for (int i = 0; i <= lob; i++)
+      for (int j = lo; j <= h2; j++) {
+        int i1 = lob - 2 * i - 1;
+        if (i1 < 0) i1 = 0;
+        wB(k, lob - i, j, 0, i1, 2 * j - lo);
+        wB(k, lob - i, j + h2 - 1 - (ng - 2), 2, i1, 2 * j - lo);
+      }
+    for (int i = 0; i < ng / 2; i++)
+      for (int j = 0; j < ng / 2; j++) {
+        wB(k, hib + i, lob - j, 1, hib + 2 * i, lob - 2 * j - 1);
+        wB(k, hib + i, hib + j, 3, hib + 2 * i, hib + 2 * j);
+      }
+    for (int i = 0; i < ng; i++)
+      for (int j = lo; j <= h2; j++) {
+        int i1 = hib + 2 * i;
+        if (i1 + 1 > NT - 1) i1 = NT - 2;
+        wB(k, hib + i, j, 1, i1, 2 * j - lo);
+        wB(k, hib + i, j + h2 - 1 - (ng - 2), 3, i1, 2 * j - lo);
+      }
+    for (int j = 0; j < ng / 2; j++)
+      for (int i = 0; i < ng / 2; i++) {
+        wB(k, lob - i, lob - j, 0, lob - 2 * i - 1, lob - 2 * j - 1);
+        wB(k, hib + i, lob - j, 1, hib + 2 * i, lob - 2 * j - 1);
+      }
+    for (int i = lo; i <= h2; i++)
+      for (int j = 0; j <= lob; j++) {
+        int j1 = lob - 2 * j - 1;
+        if (j1 < 0) j1 = 0;
+        wB(k, i, lob - j, 0, 2 * i - lo, j1);
+        wB(k, i + h2 - 1 - (ng - 2), lob - j, 1, 2 * i - lo, j1);
+      }
+    for (int j = 0; j < ng / 2; j++)
+      for (int i = 0; i < ng / 2; i++) {
+        wB(k, lob - i, hib + j, 2, lob - 2 * i - 1, hib + 2 * j);
+        wB(k, hib + i, hib + j, 3, hib + 2 * i, hib + 2 * j);
+      }
+    for (int i = lo; i <= h2; i++)
+      for (int j = 0; j < ng; j++) {
+        int j1 = hib + 2 * j;
+        if (j1 + 1 > NT - 1) j1 = NT - 2;
+        wB(k, i, hib + j, 2, 2 * i - lo, j1);
+        wB(k, i + h2 - 1 - (ng - 2), hib + j, 3, 2 * i - lo, j1);
+      }
+  }
+}
+
+}  // namespace
+
+// ====================================================================
+// context operations
+// ====================================================================
+namespace {
+
+void fill_params(amr_ctx* c) {
+  c->h_ip.resize(c->inst.size());
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    const Instance& I = c->inst[q];
+    InstParams& P = c->h_ip[q];
+    P.gamma = I.gamma;
+    P.dt = I.dt_dom;
+    for (int l = 0; l < kMaxLev; l++) P.dx[l] = I.dx[l];
+    P.limiter = I.lim;
+    P.solver = I.solver;
+    P.maxlevs = I.maxlevs;
+  }
+}
+
+int upload_params(amr_ctx* c) {
+  fill_params(c);
+  HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->h_ip.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  return 0;
+}
+
+// leaf work list and fill program of the running instances
+int rebuild_topology(amr_ctx* c) {
+  c->h_leaves.clear();
+  c->leaf_inst_begin.assign(c->inst.size() + 1, 0);
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    Instance& I = c->inst[q];
+    c->leaf_inst_begin[q] = (int)c->h_leaves.size();
+    if (I.done) continue;
+    for (int nb = 0; nb < I.lastActive; nb++)
+      if (I.leafs[nb]) c->h_leaves.push_back(LeafItem{I.gs[nb], (int32_t)q, nb, I.lp[nb][0]});
+    build_schedule(I);
+  }
+  c->leaf_inst_begin[c->inst.size()] = (int)c->h_leaves.size();
+  c->nleaf = (int)c->h_leaves.size();
+  if (int e = ensure(c->d_leaves, c->cap_leaves, std::max(1, c->nleaf))) return e;
+  if (c->nleaf)
+    HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
+                          c->stream));
+  ProgramBuilder pb;
+  compile_programs(c, pb);
+  c->h_ops.swap(pb.ops);
+  c->h_nodes.swap(pb.nested);
+  pb.ops = c->h_ops;
+  pb.nested = c->h_nodes;
+  c->nops = (int)pb.ops.size();
+  c->nnodes = (int)pb.nested.size();
+  c->node_levels = pb.level_start;
+  if (int e = ensure(c->d_ops, c->cap_ops, std::max(1, c->nops))) return e;
+  if (int e = ensure(c->d_nodes, c->cap_nodes, std::max(1, c->nnodes))) return e;
+  if (c->nops)
+    HIPCHK(hipMemcpyAsync(c->d_ops, c->h_ops.data(), sizeof(FillOp) * c->nops, hipMemcpyHostToDevice, c->stream));
+  if (c->nnodes)
+    HIPCHK(hipMemcpyAsync(c->d_nodes, c->h_nodes.data(), sizeof(FillOp) * c->nnodes, hipMemcpyHostToDevice,
+                          c->stream));
+  if (int e = ensure_bytes(c->d_fillbuf, c->cap_fillbuf, (size_t)4 * 8 * std::max(1, c->nops))) return e;
+  if (int e = ensure_bytes(c->d_nodebuf, c->cap_nodebuf, (size_t)4 * 8 * std::max(1, c->nnodes))) return e;
+  const int old_cap = c->cap_red;
+  if (int e = ensure(c->d_lam, c->cap_red, std::max(1, c->nleaf))) return e;
+  if (c->cap_red != old_cap || !c->d_grad) {
+    // reduction outputs (device + pinned host) follow the leaf capacity
+    if (c->d_grad) hipFree(c->d_grad);
+    if (c->h_lam) hipHostFree(c->h_lam);
+    if (c->h_grad) hipHostFree(c->h_grad);
+    HIPCHK(hipMalloc((void**)&c->d_grad, sizeof(double) * c->cap_red));
+    HIPCHK(hipHostMalloc((void**)&c->h_lam, sizeof(double) * c->cap_red));
+    HIPCHK(hipHostMalloc((void**)&c->h_grad, sizeof(double) * c->cap_red));
+  }
+  c->topo_dirty = false;
+  return 0;
+}
+
+FillSet make_set(amr_ctx* c, std::initializer_list<std::pair<int, int>> fk) {
+  FillSet s{};
+  for (auto& p : fk) {
+    s.base[s.n] = c->F.f[p.first];
+    s.nv[s.n] = kFieldVars[p.first];
+    s.kind[s.n] = p.second;
+    s.n++;
+  }
+  return s;
+}
+
+void run_fill(amr_ctx* c, const FillSet& s) {
+  Timed tm(c, PF_FILL);
+  launch_fill(c->NT, c->d_ops, c->nops, c->d_nodes, c->node_levels.data(), (int)c->node_levels.size() - 1,
+              c->nnodes, s, c->d_fillbuf, c->d_nodebuf, c->stream);
+}
+
+void run_stage(amr_ctx* c, int step) {
+  Timed tm(c, PF_STAGE);
+  launch_stage(c->NT, c->ng, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream);
+}
+
+// CT2D.cpp:11-124
+void run_ct(amr_ctx* c, int step) {
+  {
+    Timed tm(c, PF_CT);
+    launch_emf(c->NT, c->F, c->d_leaves, c->nleaf, c->stream);
+  }
+  run_fill(c, make_set(c, {{kEMF, kFillEdge}}));
+  {
+    Timed tm(c, PF_CT);
+    launch_bupdate(c->NT, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->stream);
+  }
+  run_fill(c, make_set(c, {{step == 1 ? kBis : kBi, kFillFace}}));
+  {
+    Timed tm(c, PF_CT);
+    launch_bsync(c->NT, c->F, c->d_leaves, c->nleaf, step, c->stream);
+  }
+}
+
+int sync_errors(amr_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->h_err->code) {
+    const DevError& e = *c->h_err;
+    return set_error(c, e.code, e.inst, e.nb, e.i, e.j);
+  }
+  return 0;
+}
+
+// regrid data transfer in reference order: all refinements, then all
+// coarsenings; ops that touch a slot an earlier op wrote run in a later batch
+int apply_regrid_ops(amr_ctx* c, const std::vector<RegridOp>& ops, bool refine) {
+  size_t b = 0;
+  while (b < ops.size()) {
+    std::vector<int32_t> written, read;
+    size_t e = b;
+    for (; e < ops.size(); e++) {
+      const RegridOp& o = ops[e];
+      std::vector<int32_t> r, w;
+      if (refine) { r = {o.parent_gs}; w = {o.son_gs[0], o.son_gs[1], o.son_gs[2], o.son_gs[3]}; }
+      else { r = {o.son_gs[0], o.son_gs[1], o.son_gs[2], o.son_gs[3], o.parent_gs}; w = {o.parent_gs}; }
+      bool clash = false;
+      for (auto x : r) clash |= std::find(written.begin(), written.end(), x) != written.end();
+      for (auto x : w) clash |= std::find(written.begin(), written.end(), x) != written.end() ||
+                               std::find(read.begin(), read.end(), x) != read.end();
+      if (clash && e > b) break;
+      written.insert(written.end(), w.begin(), w.end());
+      read.insert(read.end(), r.begin(), r.end());
+    }
+    const int n = (int)(e - b);
+    if (int er = ensure(c->d_rops, c->cap_rops, n)) return er;
+    HIPCHK(hipMemcpyAsync(c->d_rops, ops.data() + b, sizeof(RegridOp) * n, hipMemcpyHostToDevice, c->stream));
+    {
+      Timed tm(c, PF_REGRID);
+      if (refine) launch_refine(c->NT, c->ng, c->F, c->d_rops, n, c->d_ip, c->d_err, c->stream);
+      else launch_coarsen(c->NT, c->ng, c->F, c->d_rops, n, c->d_cmapU, c->d_cmapB, c->d_ip, c->d_err, c->stream);
+    }
+    b = e;
+  }
+  return 0;
+}
+
+// per-leaf CFL bounds and indicators of the running instances -> host
+int cfl_and_indicators(amr_ctx* c, const double* dW) {
+  {
+    Timed tm(c, PF_CFL);
+    launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
+                        c->stream);
+  }
+  if (c->nleaf) {
+    HIPCHK(hipMemcpyAsync(c->h_lam, c->d_lam, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
+  }
+  return sync_errors(c);
+}
+
+// admesh for every running instance from the downloaded indicators
+int regrid_all(amr_ctx* c) {
+  std::vector<RegridOp>& refs = c->h_refs;
+  std::vector<RegridOp>& coars = c->h_coars;
+  HIPCHK(hipStreamSynchronize(c->stream));  // previous ops consumed
+  refs.clear();
+  coars.clear();
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    Instance& I = c->inst[q];
+    if (I.done || I.maxlevs <= 1) continue;
+    std::vector<double> grad(I.lastActive, 0.0);
+    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) grad[c->h_leaves[l].nb] = c->h_grad[l];
+    const int before = I.lastActive;
+    const size_t nr = refs.size(), nc = coars.size();
+    if (int e = regrid_decide(c, (int)q, grad, refs, coars)) return e;
+    if (refs.size() != nr || coars.size() != nc || I.lastActive != before) c->topo_dirty = true;
+  }
+  if (int e = apply_regrid_ops(c, refs, true)) return e;
+  if (int e = apply_regrid_ops(c, coars, false)) return e;
+  return 0;
+}
+
+// main.cpp:128-247 for all running instances
+int step_all(amr_ctx* c, int* n_active) {
+  if (c->diag.code) return c->diag.code;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  run_fill(c, make_set(c, {{kU, kFillCell}, {kBi, kFillFace}}));
+  run_stage(c, 1);
+  run_fill(c, make_set(c, {{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}}));
+  run_ct(c, 1);
+  run_fill(c, make_set(c, {{kUs, kFillCell}}));
+  run_stage(c, 2);
+  run_fill(c, make_set(c, {{kFF, kFillFace}, {kGG, kFillFace}}));
+  run_ct(c, 2);
+  run_fill(c, make_set(c, {{kU, kFillCell}}));
+  if (int e = cfl_and_indicators(c, nullptr)) return e;
+  int active = 0;
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    Instance& I = c->inst[q];
+    if (I.done) continue;
+    double lambda = 1.E6;
+    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) lambda = std_min(lambda, c->h_lam[l]);
+    I.t = I.t + I.dt_main;
+    I.dt_dom = I.dt_main;
+    I.count = I.count + 1;
+    I.dt_main = I.cfl * lambda;
+    if (I.dt_main < 0 || std::isnan(I.dt_main)) return set_error(c, AMR_ERR_DT, (int)q, -1, 0, 0);
+  }
+  if (int e = regrid_all(c)) return e;
+  for (auto& I : c->inst) {
+    if (!I.done && !(I.t <= I.tEnd)) {
+      I.done = true;
+      c->topo_dirty = true;
+    }
+    active += I.done ? 0 : 1;
+  }
+  if (n_active) *n_active = active;
+  return 0;
+}
+
+int check_inst(amr_ctx* c, int inst) { return (c && inst >= 0 && inst < (int)c->inst.size()) ? 0 : AMR_ERR_CONFIG; }
+
+}  // namespace
+
+// ====================================================================
+// C ABI
+// ====================================================================
+extern "C" {
+
+int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, amr_ctx** out) {
+  if (!out || n_inst <= 0 || !cfgs) return AMR_ERR_CONFIG;
+  *out = nullptr;
+  amr_ctx* c = new amr_ctx();
+  c->nxi = cfgs[0].nx;
+  c->ng = cfgs[0].nghosts;
+  c->NT = c->nxi + 2 * c->ng;
+  if (!tile_supported(c->NT, c->ng)) { delete c; return AMR_ERR_CONFIG; }
+  int64_t total_mb = 0;
+  c->inst.resize(n_inst);
+  for (int q = 0; q < n_inst; q++) {
+    const amr_sim_config& g = cfgs[q];
+    Instance& I = c->inst[q];
+    I.cfg = g;
+    I.lim = lim_code(g.limiter);
+    I.solver = flux_code(g.flux);
+    I.ic = ic_code(g.ic);
+    I.maxlevs = g.maxlevs;
+    I.cfl = g.cfl;
+    if (g.nx != c->nxi || g.nghosts != c->ng || I.lim < 0 || I.solver < 0 || I.ic < 0 || g.maxlevs < 1 ||
+        g.maxlevs > 8 || ((I.lim == kLimWENO5) && c->ng < 3)) {
+      delete c;
+      return AMR_ERR_CONFIG;
+    }
+    // defs.hpp:30
+    const double p = std::pow(kNbRoots, g.maxlevs - 1);
+    I.maxblocks = kNbRoots > p ? kNbRoots : (int)(std::pow(4, g.maxlevs) * 2);
+    total_mb += I.maxblocks;
+    // meshblock.cpp:90-96
+    I.dx[0] = g.lenx / (std::sqrt((double)kNbRoots) * (c->NT - 2 * c->ng - 1));
+    for (int l = 1; l < kMaxLev; l++) I.dx[l] = I.dx[l - 1] / 2.;
+    // IC2Dtype.cpp:18-80 (gamma, tEnd per problem)
+    if (I.ic == IC_ROTOR) { I.gamma = 1.4; I.tEnd = 0.295; }
+    else if (I.ic == IC_ROTOR_P) { I.gamma = g.ic_params[6]; I.tEnd = g.ic_params[7]; }
+    else { I.gamma = 2.0; I.tEnd = 0.2; }
+  }
+  c->cap_slots = slot_capacity > 0 ? slot_capacity : total_mb;
+  const size_t N2 = (size_t)c->NT * c->NT;
+  for (int f = 0; f < kNumFields; f++) {
+    const size_t bytes = (size_t)c->cap_slots * kFieldVars[f] * N2 * sizeof(double);
+    if (hipMalloc((void**)&c->F.f[f], bytes) != hipSuccess || hipMemset(c->F.f[f], 0, bytes) != hipSuccess) {
+      amr_destroy(c);
+      return AMR_ERR_HIP;
+    }
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
+      hipMalloc((void**)&c->d_err, sizeof(DevError)) != hipSuccess ||
+      hipMemset(c->d_err, 0, sizeof(DevError)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_err, sizeof(DevError)) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    amr_destroy(c);
+    return AMR_ERR_HIP;
+  }
+  std::vector<int32_t> mU, mB;
+  coarsen_maps(c->NT, c->ng, mU, mB);
+  if (hipMalloc((void**)&c->d_cmapU, sizeof(int32_t) * mU.size()) != hipSuccess ||
+      hipMalloc((void**)&c->d_cmapB, sizeof(int32_t) * mB.size()) != hipSuccess ||
+      hipMemcpy(c->d_cmapU, mU.data(), sizeof(int32_t) * mU.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_cmapB, mB.data(), sizeof(int32_t) * mB.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    amr_destroy(c);
+    return AMR_ERR_HIP;
+  }
+  *out = c;
+  return AMR_OK;
+}
+
+int amr_destroy(amr_ctx* c) {
+  if (!c) return AMR_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (int f = 0; f < kNumFields; f++)
+    if (c->F.f[f]) hipFree(c->F.f[f]);
+  hipFree(c->d_leaves); hipFree(c->d_ops); hipFree(c->d_nodes); hipFree(c->d_fillbuf); hipFree(c->d_nodebuf);
+  hipFree(c->d_ip); hipFree(c->d_lam); hipFree(c->d_grad); hipFree(c->d_err); hipFree(c->d_rops);
+  hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
+  if (c->h_lam) hipHostFree(c->h_lam);
+  if (c->h_grad) hipHostFree(c->h_grad);
+  if (c->h_err) hipHostFree(c->h_err);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return AMR_OK;
+}
+
+// basegrid.cpp:4-89 + IC2Dtype.cpp + main.cpp:97-110
+int amr_init(amr_ctx* c) {
+  if (!c) return AMR_ERR_CONFIG;
+  static const int32_t root_lp[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
+                                        {0, 0, 0, 0, -1, 3, 0, -1}, {0, 0, 0, 0, 2, -1, 1, -1}};
+  const int NT = c->NT, N2 = NT * NT, side = NT - 2 * c->ng;
+  std::vector<double> Wall;
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    Instance& I = c->inst[q];
+    grow_tree(I, kNbRoots);
+    for (int r = 0; r < kNbRoots; r++) {
+      for (int s = 0; s < 8; s++) I.lp[r][s] = root_lp[r][s];
+      I.icoord[r] = {(r & 1) ? side : 0, (r & 2) ? side : 0};
+      I.leafs[r] = 1;
+      I.iref[r] = 1;
+      if (slot_of(c, I, r) < 0) return set_error(c, AMR_ERR_CAPACITY, (int)q, r, 0, 0);
+    }
+    I.lastActive = kNbRoots;
+    I.t = 0; I.dt_main = 0; I.dt_dom = 0; I.count = 0; I.done = false;
+    for (int r = 0; r < kNbRoots; r++) {
+      std::vector<double> W, B, U;
+      initial_tile(I, r, NT, c->ng, W, B);
+      prim_to_cons_tile(W, U, N2, I.gamma);
+      HIPCHK(hipMemcpy(c->F.f[kU] + (size_t)I.gs[r] * 8 * N2, U.data(), sizeof(double) * 8 * N2, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->F.f[kBi] + (size_t)I.gs[r] * 2 * N2, B.data(), sizeof(double) * 2 * N2, hipMemcpyHostToDevice));
+      Wall.insert(Wall.end(), W.begin(), W.end());  // leaf order = instance, then nb
+    }
+  }
+  c->topo_dirty = true;
+  if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  // the initial regrid reads the IC primitives directly (main.cpp:110)
+  if (c->d_W) hipFree(c->d_W);
+  HIPCHK(hipMalloc((void**)&c->d_W, sizeof(double) * Wall.size()));
+  HIPCHK(hipMemcpy(c->d_W, Wall.data(), sizeof(double) * Wall.size(), hipMemcpyHostToDevice));
+  if (int e = cfl_and_indicators(c, c->d_W)) return e;
+  hipFree(c->d_W);
+  c->d_W = nullptr;
+  if (int e = regrid_all(c)) return e;
+  c->topo_dirty = true;
+  return sync_errors(c);
+}
+
+int amr_step(amr_ctx* c, int* n_active) {
+  if (!c) return AMR_ERR_CONFIG;
+  return step_all(c, n_active);
+}
+
+int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
+  if (!c) return AMR_ERR_CONFIG;
+  int64_t n = 0, zu = 0;
+  const int64_t cells = (int64_t)c->nxi * c->nxi;
+  int active = 0;
+  for (auto& I : c->inst) active += I.done ? 0 : 1;
+  while (active > 0 && (max_steps < 0 || n < max_steps)) {
+    int64_t leaves = 0;
+    for (auto& I : c->inst)
+      if (!I.done)
+        for (int nb = 0; nb < I.lastActive; nb++) leaves += I.leafs[nb];
+    if (int e = step_all(c, &active)) {
+      if (steps) *steps = n;
+      if (zone_updates) *zone_updates = zu;
+      return e;
+    }
+    zu += leaves * cells;
+    n++;
+  }
+  if (steps) *steps = n;
+  if (zone_updates) *zone_updates = zu;
+  return AMR_OK;
+}
+
+int amr_info(amr_ctx* c, int inst, amr_inst_info* o) {
+  if (check_inst(c, inst) || !o) return AMR_ERR_CONFIG;
+  const Instance& I = c->inst[inst];
+  int nl = 0;
+  for (int nb = 0; nb < I.lastActive; nb++) nl += I.leafs[nb];
+  *o = amr_inst_info{I.t, I.dt_dom, I.dt_main, I.gamma, I.tEnd, I.lastActive, nl, I.count, I.done ? 1 : 0,
+                     I.status, I.maxblocks};
+  return AMR_OK;
+}
+
+int amr_last_error(amr_ctx* c, amr_diag* d) {
+  if (!c || !d) return AMR_ERR_CONFIG;
+  *d = c->diag;
+  return AMR_OK;
+}
+
+int amr_download(amr_ctx* c, int inst, int field, int nslots, double* host) {
+  if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int nb = 0; nb < nslots; nb++) {
+    const int gs = slot_of(c, I, nb);
+    if (gs < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
+    HIPCHK(hipMemcpy(host + per * nb, c->F.f[field] + per * gs, per * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return AMR_OK;
+}
+
+int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) {
+  if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int nb = 0; nb < nslots; nb++) {
+    const int gs = slot_of(c, I, nb);
+    if (gs < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
+    HIPCHK(hipMemcpy(c->F.f[field] + per * gs, host + per * nb, per * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return AMR_OK;
+}
+
+int amr_get_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* ic, int32_t* lf) {
+  if (check_inst(c, inst)) return AMR_ERR_CONFIG;
+  const Instance& I = c->inst[inst];
+  if (n > (int)I.lp.size()) return AMR_ERR_CONFIG;
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) lp[8 * nb + s] = I.lp[nb][s];
+    ic[2 * nb] = I.icoord[nb][0];
+    ic[2 * nb + 1] = I.icoord[nb][1];
+    lf[nb] = I.leafs[nb];
+  }
+  return AMR_OK;
+}
+
+int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* ic, const int32_t* lf) {
+  if (check_inst(c, inst) || n > c->inst[inst].maxblocks) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.icoord[nb] = {ic[2 * nb], ic[2 * nb + 1]};
+    I.leafs[nb] = (uint8_t)lf[nb];
+    if (slot_of(c, I, nb) < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
+  }
+  I.lastActive = n;
+  I.done = false;
+  c->topo_dirty = true;
+  return AMR_OK;
+}
+
+int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int32_t count) {
+  if (check_inst(c, inst)) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  I.t = t;
+  I.dt_dom = dt;
+  I.dt_main = dt_next;
+  I.count = count;
+  return AMR_OK;
+}
+
+int amr_fill(amr_ctx* c, int field, int kind) {
+  if (!c || field < 0 || field >= kNumFields || kind < 0 || kind > 2) return AMR_ERR_CONFIG;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  run_fill(c, make_set(c, {{field, kind}}));
+  return sync_errors(c);
+}
+
+int amr_stage(amr_ctx* c, int step) {
+  if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  run_stage(c, step);
+  return sync_errors(c);
+}
+
+int amr_ct(amr_ctx* c, int step) {
+  if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  run_ct(c, step);
+  return sync_errors(c);
+}
+
+int amr_cfl(amr_ctx* c, int inst, double* lambda) {
+  if (check_inst(c, inst) || !lambda) return AMR_ERR_CONFIG;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  if (int e = cfl_and_indicators(c, nullptr)) return e;
+  double lam = 1.E6;
+  for (int l = c->leaf_inst_begin[inst]; l < c->leaf_inst_begin[inst + 1]; l++) lam = std_min(lam, c->h_lam[l]);
+  *lambda = lam;
+  return AMR_OK;
+}
+
+int amr_regrid(amr_ctx* c) {
+  if (!c) return AMR_ERR_CONFIG;
+  if (c->topo_dirty)
+    if (int e = rebuild_topology(c)) return e;
+  if (int e = upload_params(c)) return e;
+  if (int e = cfl_and_indicators(c, nullptr)) return e;
+  if (int e = regrid_all(c)) return e;
+  return sync_errors(c);
+}
+
+int amr_schedule(amr_ctx* c, int inst, int cap, int32_t* ib3, int* n) {
+  if (check_inst(c, inst) || !n) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  build_schedule(I);
+  *n = (int)I.sched.size();
+  for (int e = 0; e < *n && e < cap; e++)
+    for (int q = 0; q < 3; q++) ib3[3 * e + q] = I.sched[e][q];
+  return AMR_OK;
+}
+
+int amr_riemann_batch(int solver, int dir, double gamma, int n, const double* wL, const double* wR, double* flux,
+                      int32_t* status, void* stream) {
+  if (solver < 0 || solver >= kFluxCount || (dir != 0 && dir != 1) || n < 0) return AMR_ERR_CONFIG;
+  launch_riemann_batch(solver, dir, gamma, n, wL, wR, flux, status, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return AMR_OK;
+}
+
+int amr_recon_batch(int NT, int nghosts, const char* limiter, double dx, int nt, const double* W, double* out,
+                    void* stream) {
+  const int lim = limiter ? lim_code(limiter) : -1;
+  if (lim < 0 || NT < 2 * nghosts + 4) return AMR_ERR_CONFIG;
+  launch_recon_batch(NT, nghosts, lim, dx, nt, W, out, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return AMR_OK;
+}
+
+int amr_profile_enable(amr_ctx* c, int on) {
+  if (!c) return AMR_ERR_CONFIG;
+  c->prof_on = on != 0;
+  if (on)
+    for (auto& p : c->prof) p = Profile{};
+  return AMR_OK;
+}
+
+int amr_profile_read(amr_ctx* c, int family, double* ms, int64_t* launches) {
+  if (!c || family < 0 || family >= PF_COUNT) return AMR_ERR_CONFIG;
+  if (ms) *ms = c->prof[family].ms;
+  if (launches) *launches = c->prof[family].launches;
+  return AMR_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================
+// host-only test hooks (no device work): run the compiled ghost-fill
+// program and the coarsen maps on host arrays, so the program compiler and
+// the index tables can be checked on a machine without a GPU.
+// ====================================================================
+namespace {
+
+double host_fill_load(const double* Q, int nv, int N2, const std::vector<double>& nodeval, int64_t ref, int k) {
+  if (ref & kNodeBit) return nodeval[(ref & ~kNodeBit) * 8 + k];
+  return Q[((size_t)(ref >> 16) * nv + k) * N2 + (ref & 0xffff)];
+}
+
+double host_fill_value(const double* Q, int nv, int N2, int kind, const std::vector<double>& nodeval,
+                       const FillOp& op, int k) {
+  if (op.op == 0) return host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
+  double v = 0.;
+  if (kind == kFillCell) {
+    for (int q = 0; q < 4; q++) v += host_fill_load(Q, nv, N2, nodeval, op.ref[q], k);
+    return 0.25 * v;
+  }
+  if (kind == kFillFace) {
+    v += host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
+    v += host_fill_load(Q, nv, N2, nodeval, op.ref[k == 0 ? 1 : 2], k);
+    return 0.5 * v;
+  }
+  return host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Applies the fill program compiled for the given tree to Q[n][nv][NT][NT]
+// in place; returns the number of destination ops (negative on error) and the
+// number of nested restriction nodes in *nnodes.
+int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
+                        double* Q, int nv, int kind, int* nnodes) {
+  amr_ctx c;
+  c.NT = NT;
+  c.ng = ng;
+  c.nxi = NT - 2 * ng;
+  c.inst.resize(1);
+  Instance& I = c.inst[0];
+  I.lastActive = n;
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.icoord[nb] = {icoord[2 * nb], icoord[2 * nb + 1]};
+    I.leafs[nb] = (uint8_t)leafs[nb];
+    I.gs.push_back(nb);
+  }
+  build_schedule(I);
+  ProgramBuilder pb;
+  compile_programs(&c, pb);
+  const int N2 = NT * NT;
+  std::vector<double> nodeval(pb.nested.size() * 8, 0.0), buf(pb.ops.size() * 8, 0.0);
+  for (size_t l = 0; l + 1 < pb.level_start.size(); l++)
+    for (int e = pb.level_start[l]; e < pb.level_start[l + 1]; e++)
+      for (int k = 0; k < nv; k++) nodeval[e * 8 + k] = host_fill_value(Q, nv, N2, kind, nodeval, pb.nested[e], k);
+  for (size_t e = 0; e < pb.ops.size(); e++)
+    for (int k = 0; k < nv; k++) buf[e * 8 + k] = host_fill_value(Q, nv, N2, kind, nodeval, pb.ops[e], k);
+  for (size_t e = 0; e < pb.ops.size(); e++)
+    for (int k = 0; k < nv; k++) Q[((size_t)pb.ops[e].dst_gs * nv + k) * N2 + pb.ops[e].dst_cell] = buf[e * 8 + k];
+  if (nnodes) *nnodes = (int)pb.nested.size();
+  return (int)pb.ops.size();
+}
+
+// coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]
+int amr_debug_coarsen_maps(int NT, int ng, int32_t* mU, int32_t* mB) {
+  std::vector<int32_t> u, b;
+  coarsen_maps(NT, ng, u, b);
+  std::memcpy(mU, u.data(), u.size() * sizeof(int32_t));
+  std::memcpy(mB, b.data(), b.size() * sizeof(int32_t));
+  return 0;
+}
+
+// the locate_bounds schedule of a tree (no device work)
+int amr_debug_schedule(int n, const int32_t* lp, const int32_t* leafs, int cap, int32_t* ib3) {
+  Instance I;
+  I.lastActive = n;
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.leafs[nb] = (uint8_t)leafs[nb];
+  }
+  build_schedule(I);
+  for (int e = 0; e < (int)I.sched.size() && e < cap; e++)
+    for (int q = 0; q < 3; q++) ib3[3 * e + q] = I.sched[e][q];
+  return (int)I.sched.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,654 @@
+// FP64 HIP kernels of the AMR2D-MHD step for gfx950.
+//
+// Layout: every slot field is [slot][var][j][i] with NT x NT tiles
+// (NT = 16 + 2*nghosts), i fastest, so a wavefront reads 64 consecutive
+// doubles of one row.  One workgroup handles one leaf block; the block's
+// primitive tile and its face fluxes live in LDS for the whole stage.
+// No MFMA: nothing on this path is a dense contraction.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "physics.hpp"
+
+namespace amr {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline void report(DevError* err, int code, int inst, int nb, int i, int j) {
+  if (atomicCAS(&err->code, 0, code) == 0) {
+    err->inst = inst;
+    err->nb = nb;
+    err->i = i;
+    err->j = j;
+  }
+}
+
+// conserved -> primitive of one cell (meshblock.cpp:119-179); returns the
+// pressure so the caller can apply the reference's interior p<=0 check
+__device__ inline void cons_to_prim(const double* u, double g, double* w) {
+  w[0] = u[0];
+  w[1] = u[1] / u[0];
+  w[2] = u[2] / u[0];
+  w[3] = u[3] / u[0];
+  double p = (g - 1) * (u[4] - 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
+  w[5] = u[5];
+  w[6] = u[6];
+  w[7] = u[7];
+  w[4] = p - (g - 1) * (0.5 * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]));
+}
+
+// ------------------------------------------------------------------ stage
+// MUSCL2D.cpp:13-121 + the RK update of main.cpp:147-157 / 179-189 for one
+// leaf: primitives of the whole tile, edge states, normal-B override from the
+// faces, Riemann fluxes on [nxminb,nxmax]^2, then every cell of the tile is
+// advanced (the residual is zero outside [nxminb,nxmax]^2).
+template <int NT, int NG>
+__global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
+                                                         const InstParams* __restrict__ ip, int step,
+                                                         DevError* err) {
+  constexpr int N2 = NT * NT;
+  constexpr int LO = NG - 1, HI = NT - 1 - NG, NF = HI - LO + 1;
+  __shared__ double sW[8 * N2];
+  __shared__ double sF[8 * NF * (NF + 1)];  // ff[i][j], i in [LO,HI+1], j in [LO,HI]
+  __shared__ double sG[8 * NF * (NF + 1)];  // gg[i][j], i in [LO,HI], j in [LO,HI+1]
+  __shared__ int sNan;
+
+  const LeafItem L = leaves[blockIdx.x];
+  const InstParams& P = ip[L.inst];
+  const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
+  const int lim = P.limiter, solver = P.solver;
+  const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
+  const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
+  const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
+  double* __restrict__ ff = F.f[kFF] + t8;
+  double* __restrict__ gg = F.f[kGG] + t8;
+  const int tid = threadIdx.x;
+  if (tid == 0) sNan = 0;
+  __syncthreads();
+
+  for (int c = tid; c < N2; c += kThreads) {
+    double u[8], w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) u[k] = Qin[k * N2 + c];
+    cons_to_prim(u, g, w);
+#pragma unroll
+    for (int k = 0; k < 8; k++) sW[k * N2 + c] = w[k];
+    const int i = c % NT, j = c / NT;
+    if (w[4] <= 0 && i > LO && i < HI + 1 && j > LO && j < HI + 1) report(err, kErrPressure, L.inst, L.nb, i, j);
+    if (step == 2 && (isnan(w[1]) || isnan(w[2]) || isnan(w[3]))) report(err, kErrNanVel, L.inst, L.nb, i, j);
+    if (lim < kLimWENO3) {
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < 8; k++) bad |= isnan(w[k]);
+      if (bad) sNan = 1;
+    }
+  }
+  // stale edge fluxes ff[LO][j], gg[i][LO] enter the residual of ghost cells
+  for (int q = tid; q < 8 * NF; q += kThreads) {
+    const int k = q / NF, r = q % NF;
+    sF[k * NF * (NF + 1) + r * (NF + 1)] = ff[k * N2 + (LO + r) * NT + LO];
+    sG[k * NF * (NF + 1) + 0 * NF + r] = gg[k * N2 + LO * NT + LO + r];
+  }
+  __syncthreads();
+  if (sNan) {
+    if (tid == 0) report(err, kErrNanEdge, L.inst, L.nb, 0, 0);
+    return;
+  }
+
+  for (int q = tid; q < 2 * NF * NF; q += kThreads) {
+    const int ydir = q >= NF * NF;
+    const int r = ydir ? q - NF * NF : q;
+    const int i = LO + r % NF, j = LO + r / NF;
+    const int s = ydir ? NT : 1;  // stride along the face normal
+    const int c = j * NT + i;     // left cell; right cell is c + s
+    double wl[8], wr[8], fl[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double* Wk = sW + k * N2;
+      if (lim == kLimWENO5) {
+        wl[k] = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
+        wr[k] = weno5_edge(Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], Wk[c + 3 * s], false);
+      } else if (lim == kLimWENO3) {
+        wl[k] = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
+        wr[k] = weno3_edge(Wk[c], Wk[c + s], Wk[c + 2 * s], false);
+      } else {
+        const double a = Wk[c] - Wk[c - s], b = Wk[c + s] - Wk[c], d = Wk[c + 2 * s] - Wk[c + s];
+        wl[k] = muscl_plus(lim, Wk[c], a, b, dxl);
+        wr[k] = muscl_minus(lim, Wk[c + s], b, d, dxl);
+      }
+    }
+    int st;
+    if (!ydir) {
+      wl[5] = B[0 * N2 + c + 1];
+      wr[5] = B[0 * N2 + c + 1];
+      st = normal_flux(solver, wl, wr, g, fl);
+    } else {
+      wl[6] = B[1 * N2 + c + NT];
+      wr[6] = B[1 * N2 + c + NT];
+      swap_xy(wl);
+      swap_xy(wr);
+      st = normal_flux(solver, wl, wr, g, fl);
+      swap_xy(fl);
+    }
+    if (st) report(err, kErrHLLC, L.inst, L.nb, i, j);
+    bool nan = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) nan |= isnan(fl[k]);
+    if (nan) report(err, kErrNanFlux, L.inst, L.nb, i, j);
+    if (!ydir) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        ff[k * N2 + c + 1] = fl[k];
+        sF[k * NF * (NF + 1) + (j - LO) * (NF + 1) + (i + 1 - LO)] = fl[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        gg[k * N2 + c + NT] = fl[k];
+        sG[k * NF * (NF + 1) + (j + 1 - LO) * NF + (i - LO)] = fl[k];
+      }
+    }
+  }
+  __syncthreads();
+
+  const double* U = F.f[kU] + t8;
+  double* Us = F.f[kUs] + t8;
+  double* Uw = F.f[kU] + t8;
+  for (int c = tid; c < N2; c += kThreads) {
+    const int i = c % NT, j = c / NT;
+    const bool in = i >= LO && i <= HI && j >= LO && j <= HI;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      double res = 0.;
+      if (in) {
+        const double* Fk = sF + k * NF * (NF + 1);
+        const double* Gk = sG + k * NF * (NF + 1);
+        const int fi = (j - LO) * (NF + 1) + (i - LO);
+        const int gi = (j - LO) * NF + (i - LO);
+        res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+      }
+      const int e = k * N2 + c;
+      if (step == 1) Us[e] = U[e] - dt * res;
+      else Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
+    }
+  }
+}
+
+// --------------------------------------------------------------------- CT
+// CT2D.cpp:15-27: corner EMF from the face fluxes for i,j in [1,NT-1]
+template <int NT>
+__global__ __launch_bounds__(kThreads) void emf_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves) {
+  constexpr int N2 = NT * NT;
+  const LeafItem L = leaves[blockIdx.y];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  if (i < 1 || j < 1) return;
+  const double* gg5 = F.f[kGG] + (size_t)L.gs * 8 * N2 + 5 * N2;
+  const double* ff6 = F.f[kFF] + (size_t)L.gs * 8 * N2 + 6 * N2;
+  F.f[kEMF][(size_t)L.gs * N2 + c] = 0.25 * ((gg5[c - 1] + gg5[c]) - (ff6[c - NT] + ff6[c]));
+}
+
+// CT2D.cpp:78-92: face-B RK update from the edge EMF
+template <int NT>
+__global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
+                                                           const InstParams* __restrict__ ip, int step) {
+  constexpr int N2 = NT * NT;
+  const LeafItem L = leaves[blockIdx.y];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  const InstParams& P = ip[L.inst];
+  const double dt = P.dt, dxl = P.dx[L.lev];
+  const double* E = F.f[kEMF] + (size_t)L.gs * N2;
+  const double* B = F.f[step == 1 ? kBi : kBis] + (size_t)L.gs * 2 * N2;
+  double* Bn = F.f[step == 1 ? kBis : kBi] + (size_t)L.gs * 2 * N2;
+  const int cjp = j < NT - 1 ? c + NT : c;
+  const int cip = i < NT - 1 ? c + 1 : c;
+  const double rx = (1. / dxl) * (E[cjp] - E[c]);
+  const double ry = -(1. / dxl) * (E[cip] - E[c]);
+  if (step == 1) {
+    Bn[c] = B[c] - dt * rx;
+    Bn[N2 + c] = B[N2 + c] - dt * ry;
+  } else {
+    Bn[c] = 0.5 * (B[c] + Bn[c] - dt * rx);
+    Bn[N2 + c] = 0.5 * (B[N2 + c] + Bn[N2 + c] - dt * ry);
+  }
+}
+
+// CT2D.cpp:95-123: cell-centred B from the updated faces + energy correction
+template <int NT>
+__global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves, int step) {
+  constexpr int N2 = NT * NT;
+  const LeafItem L = leaves[blockIdx.y];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  const double* Bn = F.f[step == 1 ? kBis : kBi] + (size_t)L.gs * 2 * N2;
+  double* q = F.f[step == 1 ? kUs : kU] + (size_t)L.gs * 8 * N2;
+  const int ip = i < NT - 1 ? i + 1 : i, jp = j < NT - 1 ? j + 1 : j;
+  const double bx = 0.5 * (Bn[j * NT + ip] + Bn[j * NT + ip - 1]);
+  const double by = 0.5 * (Bn[N2 + jp * NT + i] + Bn[N2 + (jp - 1) * NT + i]);
+  const double q5 = q[5 * N2 + c], q6 = q[6 * N2 + c];
+  q[4 * N2 + c] = q[4 * N2 + c] + 0.5 * ((bx * bx + by * by) - ((q5 * q5) + (q6 * q6)));
+  q[5 * N2 + c] = bx;
+  q[6 * N2 + c] = by;
+}
+
+// -------------------------------------------------------------- ghost fill
+// Gather/scatter execution of a compiled fill program: every destination's
+// value is computed from the pre-fill state (plus nested restriction nodes,
+// evaluated level by level), then all destinations are written.
+struct FillCtx {
+  const double* base;
+  int nv, kind, N2;
+  const double* nodes;  // node values of this field, [node][8]
+};
+
+__device__ inline double fill_load(const FillCtx& f, int64_t ref, int k) {
+  if (ref & kNodeBit) return f.nodes[(ref & ~kNodeBit) * 8 + k];
+  const int64_t gs = ref >> 16;
+  const int cell = (int)(ref & 0xffff);
+  return f.base[((size_t)gs * f.nv + k) * f.N2 + cell];
+}
+
+// boundary.cpp (0.25 x 2x2 sum in misc.cpp:35-48 order), boundaryfc.cpp
+// (0.5 x face pair: along j for component 0, along i otherwise) and
+// boundaryec.cpp (injection)
+__device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
+  if (op.op == 0) return fill_load(f, op.ref[0], k);
+  double v = 0.;
+  if (f.kind == kFillCell) {
+    v += fill_load(f, op.ref[0], k);
+    v += fill_load(f, op.ref[1], k);
+    v += fill_load(f, op.ref[2], k);
+    v += fill_load(f, op.ref[3], k);
+    return 0.25 * v;
+  }
+  if (f.kind == kFillFace) {
+    v += fill_load(f, op.ref[0], k);
+    v += fill_load(f, op.ref[k == 0 ? 1 : 2], k);
+    return 0.5 * v;
+  }
+  return fill_load(f, op.ref[0], k);
+}
+
+__global__ __launch_bounds__(kThreads) void fill_node_kernel(const FillOp* __restrict__ nodes, int begin, int end,
+                                                             int nnodes, FillSet set, double* nodebuf, int N2) {
+  const int e = begin + blockIdx.x * kThreads + threadIdx.x;
+  if (e >= end) return;
+  const int fi = blockIdx.y;
+  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], N2, nodebuf + (size_t)fi * nnodes * 8};
+  const FillOp op = nodes[e];
+  for (int k = 0; k < f.nv; k++) nodebuf[((size_t)fi * nnodes + e) * 8 + k] = fill_value(f, op, k);
+}
+
+__global__ __launch_bounds__(kThreads) void fill_gather_kernel(const FillOp* __restrict__ ops, int nops,
+                                                               FillSet set, double* buf, const double* nodebuf,
+                                                               int nnodes, int N2) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= nops) return;
+  const int fi = blockIdx.y;
+  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], N2, nodebuf + (size_t)fi * nnodes * 8};
+  const FillOp op = ops[e];
+  for (int k = 0; k < f.nv; k++) buf[((size_t)fi * nops + e) * 8 + k] = fill_value(f, op, k);
+}
+
+__global__ __launch_bounds__(kThreads) void fill_scatter_kernel(const FillOp* __restrict__ ops, int nops,
+                                                                FillSet set, const double* buf, int N2) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= nops) return;
+  const int fi = blockIdx.y;
+  const FillOp op = ops[e];
+  const int nv = set.nv[fi];
+  double* base = set.base[fi];
+  for (int k = 0; k < nv; k++)
+    base[((size_t)op.dst_gs * nv + k) * N2 + op.dst_cell] = buf[((size_t)fi * nops + e) * 8 + k];
+}
+
+// ------------------------------------------------------ CFL and criteria
+// main.cpp:209-230 (per-leaf wave-speed bound, with the reference's
+// |rho|+c_fx and |u|+c_fy terms) fused with criteria.cpp:6-44 (maximum of
+// the max4diff/|w| ratio over p, rho, Bx, By on the interior).
+template <int NT, int NG>
+__global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, const double* __restrict__ Win,
+                                                                const LeafItem* __restrict__ leaves,
+                                                                const InstParams* __restrict__ ip, double* lam_out,
+                                                                double* grad_out, DevError* err) {
+  constexpr int N2 = NT * NT;
+  constexpr int LO = NG, HI = NT - 1 - NG;
+  __shared__ double sW[8 * N2];
+  __shared__ double sRed[2][kThreads];
+  const int leaf = blockIdx.x;
+  const LeafItem L = leaves[leaf];
+  const double g = ip[L.inst].gamma;
+  const int tid = threadIdx.x;
+  if (Win) {
+    const double* W = Win + (size_t)leaf * 8 * N2;
+    for (int q = tid; q < 8 * N2; q += kThreads) sW[q] = W[q];
+  } else {
+    const double* U = F.f[kU] + (size_t)L.gs * 8 * N2;
+    for (int c = tid; c < N2; c += kThreads) {
+      double u[8], w[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) u[k] = U[k * N2 + c];
+      cons_to_prim(u, g, w);
+#pragma unroll
+      for (int k = 0; k < 8; k++) sW[k * N2 + c] = w[k];
+      const int i = c % NT, j = c / NT;
+      if (w[4] <= 0 && i > LO - 1 && i < HI + 1 && j > LO - 1 && j < HI + 1)
+        report(err, kErrPressure, L.inst, L.nb, i, j);
+    }
+  }
+  __syncthreads();
+  double lam = 0., grad = 0.;
+  constexpr int NI = HI - LO + 1;
+  for (int q = tid; q < NI * NI; q += kThreads) {
+    const int i = LO + q % NI, j = LO + q / NI, c = j * NT + i;
+    double w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = sW[k * N2 + c];
+    if (!Win) {
+      lam = std_max(lam, fabs(w[0]) + fast_speed(w, g, 5));
+      lam = std_max(lam, fabs(w[1]) + fast_speed(w, g, 6));
+    }
+    const int vars[4] = {4, 0, 5, 6};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const double* Wk = sW + vars[v] * N2;
+      const double x = Wk[c];
+      double m = fabs(Wk[c + 1] - x);
+      m = mac_max(m, fabs(Wk[c - 1] - x));
+      m = mac_max(m, fabs(Wk[c + NT] - x));
+      m = mac_max(m, fabs(Wk[c - NT] - x));
+      grad = std_max(grad, m / mac_max(kEps, fabs(x)));
+    }
+  }
+  sRed[0][tid] = lam;
+  sRed[1][tid] = grad;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      sRed[0][tid] = std_max(sRed[0][tid], sRed[0][tid + w]);
+      sRed[1][tid] = std_max(sRed[1][tid], sRed[1][tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double dxl = ip[L.inst].dx[L.lev];
+    lam_out[leaf] = std_min(dxl, dxl) / sRed[0][0];
+    grad_out[leaf] = sRed[1][0];
+  }
+}
+
+// ------------------------------------------------------------------ regrid
+// refine.cpp:17-36: minmod-limited quarter-slope prolongation of one value
+__device__ inline double prolong(const double* Q, int NT, int i, int j, int sx, int sy) {
+  const int c = j * NT + i;
+  double v = Q[c];
+  double r = (Q[c] - Q[c - 1]) / (Q[c + 1] - Q[c] + kEps);
+  double phi = 0.25 * limiter(kLimMinmod, r, 0) * (Q[c + 1] - Q[c]);
+  v = sx == 0 ? v - phi : v + phi;
+  r = (Q[c] - Q[c - NT]) / (Q[c + NT] - Q[c] + kEps);
+  phi = 0.25 * limiter(kLimMinmod, r, 0) * (Q[c + NT] - Q[c]);
+  v = sy == 0 ? v - phi : v + phi;
+  return v;
+}
+
+// refineBi_CT.cpp:4-50: quadratic face reconstruction sampled at the son face
+__device__ inline double prolong_face(const double* B, int NT, int N2, int i, int j, int k, int signx, int signy) {
+  const double* Bx = B;
+  const double* By = B + N2;
+  const int c = j * NT + i;
+  const double x_jm = Bx[c - NT], x0 = Bx[c], x_jp = Bx[c + NT];
+  const double x_ip = Bx[c + 1], x_ipjm = Bx[c + 1 - NT], x_ipjp = Bx[c + 1 + NT];
+  const double y_im = By[c - 1], y0 = By[c], y_ip = By[c + 1];
+  const double y_imjp = By[c - 1 + NT], y_jp = By[c + NT], y_ipjp = By[c + 1 + NT];
+  const double DyBxm = limiter(kLimMinmod, (x0 - x_jm) / (x_jp - x0 + kEps), 0) * (x_jp - x0);
+  const double DyBxp = limiter(kLimMinmod, (x_ip - x_ipjm) / (x_ipjp - x_ip + kEps), 0) * (x_ipjp - x_ip);
+  const double DxBym = limiter(kLimMinmod, (y0 - y_im) / (y_ip - y0 + kEps), 0) * (y_ip - y0);
+  const double DxByp = limiter(kLimMinmod, (y_jp - y_imjp) / (y_ipjp - y_jp + kEps), 0) * (y_ipjp - y_jp);
+  const double ax = (x_ip - x0), by = -1. * ax;
+  const double ay = 0.5 * (DyBxp + DyBxm), bx = 0.5 * (DxByp + DxBym);
+  const double axy = (DyBxp - DyBxm), byy = axy / (-2.);
+  const double bxy = (DxByp - DxBym), axx = bxy / (-2.);
+  const double ayy = 0., bxx = 0.;
+  if (k == 0) {
+    const double a0 = 0.5 * (x_ip + x0) - 0.25 * axx;
+    const double sx = signx == 0 ? 0 : 0.5, sy = signy == 0 ? -0.25 : 0.25;
+    return a0 + ax * sx + ay * sy + axx * sx * sx + axy * sx * sy + ayy * sy * sy;
+  }
+  const double b0 = 0.5 * (y_jp + y0) - 0.25 * byy;
+  const double sx = signx == 0 ? -0.25 : 0.25, sy = signy == 0 ? 0 : 0.5;
+  return b0 + bx * sx + by * sy + bxx * sx * sx + bxy * sx * sy + byy * sy * sy;
+}
+
+// refine.cpp:38-59 + refineBi_CT.cpp:52-74 (+ the sons' U2W p<=0 check)
+template <int NT, int NG>
+__global__ __launch_bounds__(kThreads) void refine_kernel(FieldPtrs F, const RegridOp* __restrict__ ops,
+                                                          const InstParams* __restrict__ ip, DevError* err) {
+  constexpr int N2 = NT * NT, NX2 = (NT - 1) / 2;
+  const RegridOp op = ops[blockIdx.y];
+  const int q = blockIdx.z;  // son 0..3
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  const int sx = (i + NG) % 2, sy = (j + NG) % 2;
+  const int pi = (q & 1) ? (i - NG + 2) / 2 + NX2 : (i + NG) / 2;
+  const int pj = (q & 2) ? (j - NG + 2) / 2 + NX2 : (j + NG) / 2;
+  const double* Up = F.f[kU] + (size_t)op.parent_gs * 8 * N2;
+  double* Us = F.f[kU] + (size_t)op.son_gs[q] * 8 * N2;
+  double u[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    u[k] = prolong(Up + k * N2, NT, pi, pj, sx, sy);
+    Us[k * N2 + c] = u[k];
+  }
+  double w[8];
+  cons_to_prim(u, ip[op.inst].gamma, w);
+  if (w[4] <= 0 && i > NG - 1 && i < NT - NG && j > NG - 1 && j < NT - NG)
+    report(err, kErrPressure, op.inst, op.son_gs[q], i, j);
+  const double* Bp = F.f[kBi] + (size_t)op.parent_gs * 2 * N2;
+  double* Bs = F.f[kBi] + (size_t)op.son_gs[q] * 2 * N2;
+  for (int k = 0; k < 2; k++) {
+    const double v = prolong_face(Bp, NT, N2, pi, pj, k, sx, sy);
+    if (isnan(v)) report(err, kErrRefineNan, op.inst, op.nb, i, j);
+    Bs[k * N2 + c] = v;
+  }
+}
+
+// coarsen.cpp:7-157 + coarsenBi_CT.cpp:7-155 through precomputed per-cell
+// source maps (last write wins): map = son | i1 << 8 | j1 << 16, -1 = untouched
+template <int NT, int NG>
+__global__ __launch_bounds__(kThreads) void coarsen_kernel(FieldPtrs F, const RegridOp* __restrict__ ops,
+                                                           const int32_t* __restrict__ cmapU,
+                                                           const int32_t* __restrict__ cmapB,
+                                                           const InstParams* __restrict__ ip, DevError* err) {
+  constexpr int N2 = NT * NT;
+  const RegridOp op = ops[blockIdx.y];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  double* Ud = F.f[kU] + (size_t)op.parent_gs * 8 * N2;
+  const int m = cmapU[c];
+  if (m >= 0) {
+    const int s = m & 0xff, i1 = (m >> 8) & 0xff, j1 = (m >> 16) & 0xff;
+    const double* Uq = F.f[kU] + (size_t)op.son_gs[s] * 8 * N2;
+    const int a = j1 * NT + i1;
+    double u[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double* Q = Uq + k * N2;
+      double v = 0.;
+      v += Q[a];
+      v += Q[a + NT];
+      v += Q[a + 1];
+      v += Q[a + 1 + NT];
+      u[k] = 0.25 * v;
+      Ud[k * N2 + c] = u[k];
+    }
+    double w[8];
+    cons_to_prim(u, ip[op.inst].gamma, w);
+    if (w[4] <= 0 && i > NG - 1 && i < NT - NG && j > NG - 1 && j < NT - NG)
+      report(err, kErrPressure, op.inst, op.nb, i, j);
+  }
+  double* Bd = F.f[kBi] + (size_t)op.parent_gs * 2 * N2;
+  for (int k = 0; k < 2; k++) {
+    const int mb = cmapB[k * N2 + c];
+    if (mb < 0) continue;
+    const int s = mb & 0xff, i1 = (mb >> 8) & 0xff, j1 = (mb >> 16) & 0xff;
+    const double* Q = F.f[kBi] + (size_t)op.son_gs[s] * 2 * N2 + k * N2;
+    const int a = j1 * NT + i1;
+    double v = 0.;
+    v += Q[a];
+    v += Q[k == 0 ? a + NT : a + 1];
+    Bd[k * N2 + c] = 0.5 * v;
+  }
+}
+
+// ----------------------------------------------------- parity entry points
+__global__ void riemann_batch_kernel(int solver, int dir, double g, int n, const double* wL, const double* wR,
+                                     double* flux, int32_t* status) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double l[8], r[8], f[8];
+  for (int k = 0; k < 8; k++) {
+    l[k] = wL[8 * q + k];
+    r[k] = wR[8 * q + k];
+  }
+  if (dir == 1) {
+    swap_xy(l);
+    swap_xy(r);
+  }
+  status[q] = normal_flux(solver, l, r, g, f);
+  if (dir == 1) swap_xy(f);
+  for (int k = 0; k < 8; k++) flux[8 * q + k] = f[k];
+}
+
+// edge states of whole tiles: out = [tile][4: xL xR yL yR][8][NT][NT]
+// (celledges.cpp ranges for MUSCL, WENO2D_primvar.cpp ranges for WENO)
+__global__ void recon_batch_kernel(int NT, int ng, int lim, double dxl, const double* W, double* out) {
+  const int N2 = NT * NT;
+  const int t = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N2) return;
+  const int i = c % NT, j = c / NT;
+  const double* Wt = W + (size_t)t * 8 * N2;
+  double* o = out + (size_t)t * 4 * 8 * N2;
+  const bool weno = lim == kLimWENO3 || lim == kLimWENO5;
+  const int lo = weno ? ng - 1 : 0, hi = weno ? NT - ng : NT - 2;
+  if (i < lo || i > hi || j < lo || j > hi) return;
+  for (int k = 0; k < 8; k++) {
+    const double* Wk = Wt + k * N2;
+    for (int dir = 0; dir < 2; dir++) {
+      if ((dir == 0 && k == 5) || (dir == 1 && k == 6)) continue;
+      const int s = dir == 0 ? 1 : NT;
+      double L, R;
+      if (lim == kLimWENO5) {
+        L = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
+        R = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], false);
+      } else if (lim == kLimWENO3) {
+        L = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
+        R = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], false);
+      } else {
+        const int idx = dir == 0 ? i : j;
+        const int cm = idx > 0 ? c - s : c;  // backward difference, index 0 duplicates 1
+        const double d0 = idx > 0 ? Wk[c] - Wk[cm] : Wk[c + s] - Wk[c];
+        const double d1 = Wk[c + s] - Wk[c];
+        L = muscl_plus(lim, Wk[c], d0, d1, dxl);
+        R = muscl_minus(lim, Wk[c], d0, d1, dxl);
+      }
+      o[((dir * 2 + 0) * 8 + k) * N2 + c] = L;
+      o[((dir * 2 + 1) * 8 + k) * N2 + c] = R;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+bool tile_supported(int NT, int ng) { return (NT == 20 && ng == 2) || (NT == 24 && ng == 4) || (NT == 12 && ng == 2); }
+
+#define AMR_DISPATCH_TILE(NT, NG, CALL)                                    \
+  do {                                                                     \
+    if (NT == 20 && NG == 2) { constexpr int T = 20, G = 2; (void)G; CALL; }        \
+    else if (NT == 24 && NG == 4) { constexpr int T = 24, G = 4; (void)G; CALL; }   \
+    else if (NT == 12 && NG == 2) { constexpr int T = 12, G = 2; (void)G; CALL; }   \
+  } while (0)
+
+void launch_stage(int NT, int ng, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip,
+                  int step, DevError* err, hipStream_t s) {
+  if (nleaf <= 0) return;
+  AMR_DISPATCH_TILE(NT, ng, (stage_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, leaves, ip, step, err)));
+}
+
+void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s) {
+  if (nleaf <= 0) return;
+  dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (emf_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves)));
+}
+
+void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
+                    hipStream_t s) {
+  if (nleaf <= 0) return;
+  dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bupdate_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step)));
+}
+
+void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int step, hipStream_t s) {
+  if (nleaf <= 0) return;
+  dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, step)));
+}
+
+void launch_fill(int NT, const FillOp* ops, int nops, const FillOp* nodes, const int32_t* node_level_start,
+                 int nlevels, int nnodes, const FillSet& set, double* buf, double* nodebuf, hipStream_t s) {
+  if (nops <= 0 || set.n <= 0) return;
+  const int N2 = NT * NT;
+  for (int l = 0; l < nlevels; l++) {
+    const int b = node_level_start[l], e = node_level_start[l + 1];
+    if (e <= b) continue;
+    dim3 g((e - b + kThreads - 1) / kThreads, set.n);
+    fill_node_kernel<<<g, kThreads, 0, s>>>(nodes, b, e, nnodes, set, nodebuf, N2);
+  }
+  dim3 grid((nops + kThreads - 1) / kThreads, set.n);
+  fill_gather_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, nodebuf, nnodes, N2);
+  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, N2);
+}
+
+void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,
+                         const InstParams* ip, double* lam_out, double* grad_out, DevError* err, hipStream_t s) {
+  if (nleaf <= 0) return;
+  AMR_DISPATCH_TILE(NT, ng,
+                    (cfl_criteria_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, W, leaves, ip, lam_out, grad_out, err)));
+}
+
+void launch_refine(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int nops, const InstParams* ip,
+                   DevError* err, hipStream_t s) {
+  if (nops <= 0) return;
+  dim3 grid((NT * NT + kThreads - 1) / kThreads, nops, 4);
+  AMR_DISPATCH_TILE(NT, ng, (refine_kernel<T, G><<<grid, kThreads, 0, s>>>(F, ops, ip, err)));
+}
+
+void launch_coarsen(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int nops, const int32_t* cmapU,
+                    const int32_t* cmapB, const InstParams* ip, DevError* err, hipStream_t s) {
+  if (nops <= 0) return;
+  dim3 grid((NT * NT + kThreads - 1) / kThreads, nops);
+  AMR_DISPATCH_TILE(NT, ng, (coarsen_kernel<T, G><<<grid, kThreads, 0, s>>>(F, ops, cmapU, cmapB, ip, err)));
+}
+
+void launch_riemann_batch(int solver, int dir, double gamma, int n, const double* wL, const double* wR, double* flux,
+                          int32_t* status, hipStream_t s) {
+  if (n <= 0) return;
+  riemann_batch_kernel<<<(n + 127) / 128, 128, 0, s>>>(solver, dir, gamma, n, wL, wR, flux, status);
+}
+
+void launch_recon_batch(int NT, int ng, int lim, double dx, int nt, const double* W, double* out, hipStream_t s) {
+  if (nt <= 0) return;
+  dim3 grid((NT * NT + 127) / 128, nt);
+  recon_batch_kernel<<<grid, 128, 0, s>>>(NT, ng, lim, dx, W, out);
+}
+
+}  // namespace amr

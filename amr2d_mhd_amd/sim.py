@@ -1,0 +1,184 @@
+"""Python side of the AMR2D-MHD engine: a thin owner of an ``amr_ctx``.
+
+Mirrors the reference's driver vocabulary (main.cpp): a run is configured by
+the input.txt triple (IC, limiter, flux), CFL and the compile-time nghosts /
+maxlevs, advances with ``step()`` (one main-loop iteration) and exposes the
+meshblock arrays (U, Us, Bi, Bis, ff, gg, EMF) per slot for inspection.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import FIELD_VARS, FIELDS, FILL_KIND, SOLVERS, AmrError, Diag, InstInfo, lib, make_config
+
+
+def _check(ctx, rc):
+    if rc != 0:
+        d = Diag()
+        if ctx:
+            lib().amr_last_error(ctx, C.byref(d))
+        raise AmrError(rc, d if ctx else None)
+
+
+class Simulation:
+    """One or more independent AMR simulations advanced together on the GPU."""
+
+    def __init__(self, configs, slot_capacity=0):
+        if isinstance(configs, dict):
+            configs = [configs]
+        cfgs = (type(make_config()) * len(configs))(*[make_config(**c) for c in configs])
+        self.n = len(configs)
+        self.nx = configs[0].get("nx", 16)
+        self.ng = configs[0].get("nghosts", 2)
+        self.NT = self.nx + 2 * self.ng
+        h = C.c_void_p()
+        rc = lib().amr_create(self.n, cfgs, int(slot_capacity), C.byref(h))
+        if rc != 0:
+            raise AmrError(rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- driver
+    def init(self):
+        _check(self.h, lib().amr_init(self.h))
+
+    def step(self):
+        n = C.c_int(0)
+        _check(self.h, lib().amr_step(self.h, C.byref(n)))
+        return n.value
+
+    def run(self, max_steps=-1):
+        steps, zu = C.c_int64(0), C.c_int64(0)
+        _check(self.h, lib().amr_run(self.h, int(max_steps), C.byref(steps), C.byref(zu)))
+        return steps.value, zu.value
+
+    def info(self, inst=0):
+        o = InstInfo()
+        _check(self.h, lib().amr_info(self.h, inst, C.byref(o)))
+        return {f: getattr(o, f) for f, _ in InstInfo._fields_}
+
+    # ----------------------------------------------------------------- state
+    def download(self, field, inst=0, nslots=None):
+        nslots = self.info(inst)["last_active"] if nslots is None else nslots
+        out = np.zeros((nslots, FIELD_VARS[field], self.NT, self.NT), np.float64)
+        _check(self.h, lib().amr_download(self.h, inst, FIELDS[field], nslots, out.ctypes.data))
+        return out
+
+    def upload(self, field, arr, inst=0):
+        arr = np.ascontiguousarray(arr, np.float64)
+        _check(self.h, lib().amr_upload(self.h, inst, FIELDS[field], arr.shape[0], arr.ctypes.data))
+
+    def get_tree(self, inst=0, nslots=None):
+        nslots = self.info(inst)["last_active"] if nslots is None else nslots
+        lp = np.zeros((nslots, 8), np.int32)
+        ic = np.zeros((nslots, 2), np.int32)
+        lf = np.zeros(nslots, np.int32)
+        _check(self.h, lib().amr_get_tree(self.h, inst, nslots, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data))
+        return lp, ic, lf
+
+    def set_tree(self, lp, ic, lf, inst=0):
+        lp = np.ascontiguousarray(lp, np.int32)
+        ic = np.ascontiguousarray(ic, np.int32)
+        lf = np.ascontiguousarray(lf, np.int32)
+        _check(self.h, lib().amr_set_tree(self.h, inst, lp.shape[0], lp.ctypes.data, ic.ctypes.data, lf.ctypes.data))
+
+    def set_time(self, t, dt, dt_next=0.0, count=0, inst=0):
+        _check(self.h, lib().amr_set_time(self.h, inst, t, dt, dt_next, count))
+
+    def load_state(self, st, inst=0):
+        """Load a full state (tree + all slot fields) given as a dict of arrays."""
+        self.set_tree(st["lp"], st["icoord"], st["leafs"], inst)
+        for f in FIELDS:
+            self.upload(f, st[f], inst)
+        self.set_time(st["t"], st["dt"], 0.0, 0, inst)
+
+    def state(self, inst=0):
+        lp, ic, lf = self.get_tree(inst)
+        st = dict(lp=lp, icoord=ic, leafs=lf)
+        for f in FIELDS:
+            st[f] = self.download(f, inst)
+        return st
+
+    # ---------------------------------------------------------------- phases
+    def fill(self, field, kind):
+        _check(self.h, lib().amr_fill(self.h, FIELDS[field], FILL_KIND[kind]))
+
+    def stage(self, step):
+        _check(self.h, lib().amr_stage(self.h, step))
+
+    def ct(self, step):
+        _check(self.h, lib().amr_ct(self.h, step))
+
+    def cfl(self, inst=0):
+        lam = C.c_double(0)
+        _check(self.h, lib().amr_cfl(self.h, inst, C.byref(lam)))
+        return lam.value
+
+    def regrid(self):
+        _check(self.h, lib().amr_regrid(self.h))
+
+    def schedule(self, inst=0):
+        n = C.c_int(0)
+        _check(self.h, lib().amr_schedule(self.h, inst, 0, None, C.byref(n)))
+        ib = np.zeros((n.value, 3), np.int32)
+        _check(self.h, lib().amr_schedule(self.h, inst, n.value, ib.ctypes.data, C.byref(n)))
+        return ib
+
+    # --------------------------------------------------------------- timing
+    def profile(self, on=True):
+        _check(self.h, lib().amr_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        names = ["stage", "fill", "ct", "cfl", "regrid"]
+        out = {}
+        for q, nm in enumerate(names):
+            ms, n = C.c_double(0), C.c_int64(0)
+            _check(self.h, lib().amr_profile_read(self.h, q, C.byref(ms), C.byref(n)))
+            out[nm] = (ms.value, n.value)
+        return out
+
+
+def riemann_batch(method, direction, gamma, wL, wR):
+    """riemannS on the GPU for torch cuda float64 tensors wL, wR of shape [n, 8]."""
+    import torch
+
+    assert wL.is_cuda and wL.dtype == torch.float64 and wL.shape[-1] == 8
+    wL = wL.contiguous()
+    wR = wR.contiguous()
+    flux = torch.empty_like(wL)
+    status = torch.empty(wL.shape[0], dtype=torch.int32, device=wL.device)
+    stream = torch.cuda.current_stream(wL.device).cuda_stream
+    rc = lib().amr_riemann_batch(SOLVERS[method], 0 if direction == "x" else 1, float(gamma), wL.shape[0],
+                                 wL.data_ptr(), wR.data_ptr(), flux.data_ptr(), status.data_ptr(), stream)
+    if rc != 0:
+        raise AmrError(rc)
+    return flux, status
+
+
+def recon_batch(W, limiter, nghosts, dx=1.0):
+    """Edge states of primitive tiles W [nt, 8, NT, NT] (torch cuda float64) ->
+    [nt, 4, 8, NT, NT] (wxL, wxR, wyL, wyR); entries outside the reference's
+    computed range stay zero."""
+    import torch
+
+    W = W.contiguous()
+    nt, NT = W.shape[0], W.shape[-1]
+    out = torch.zeros((nt, 4, 8, NT, NT), dtype=torch.float64, device=W.device)
+    stream = torch.cuda.current_stream(W.device).cuda_stream
+    rc = lib().amr_recon_batch(NT, nghosts, limiter.encode(), float(dx), nt, W.data_ptr(), out.data_ptr(), stream)
+    if rc != 0:
+        raise AmrError(rc)
+    return out

@@ -1,0 +1,134 @@
+/* amr2d_mhd.h -- C ABI of the MI355X-native AMR2D-MHD hot path.
+ *
+ * The reference (spaceauk/AMR2D_MHD) has no library interface: its hot path is
+ * a set of free C++ functions over `meshblock&` called from main.cpp.  Each
+ * entry point below names the reference function(s) it replaces (file:line).
+ * All pointers are plain host pointers unless stated; the context owns all
+ * device memory.  Every call returns an int status (AMR_OK = 0); details of a
+ * failure are available from amr_last_error().  Calls on one context must
+ * come from one host thread.
+ */
+#ifndef AMR2D_MHD_H
+#define AMR2D_MHD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AMR_OK = 0,
+  AMR_ERR_PRESSURE = 1,   /* meshblock.cpp:136-142, 164-170: p <= 0 in the interior */
+  AMR_ERR_NAN_VEL = 2,    /* meshblock.cpp:171-176 */
+  AMR_ERR_NAN_FLUX = 3,   /* MUSCL2D.cpp:79-85, 103-108 */
+  AMR_ERR_CT_FACE = 4,    /* MUSCL2D.cpp:87-91, 110-114 */
+  AMR_ERR_NAN_EDGE = 5,   /* celledges.cpp:17-21, 44-47 */
+  AMR_ERR_HLLC = 6,       /* riemannS.cpp:269-276 */
+  AMR_ERR_DT = 7,         /* main.cpp:235-238 */
+  AMR_ERR_OVERFLOW = 8,   /* admesh.cpp:42-45: maxblocks exceeded */
+  AMR_ERR_REFINE_NAN = 9, /* refineBi_CT.cpp:47 */
+  AMR_ERR_CONFIG = 10,
+  AMR_ERR_HIP = 11,
+  AMR_ERR_CAPACITY = 12   /* slot arena of the context exhausted */
+};
+
+/* slot fields (meshblock arrays, defs.hpp:69-83); layout [slot][var][j][i] */
+enum { AMR_U = 0, AMR_US = 1, AMR_BI = 2, AMR_BIS = 3, AMR_FF = 4, AMR_GG = 5, AMR_EMF = 6 };
+
+/* One simulation: the reference's input.txt (nx ny nvar / lenx leny / CFL /
+ * IC limiter flux, read at main.cpp:43-69) plus its compile-time constants
+ * nghosts and maxlevs (defs.hpp:27-28). */
+typedef struct {
+  int32_t nx;          /* interior cells per block side: 16 (or 8) */
+  int32_t nghosts;     /* 2, or 4 for WENO5 */
+  int32_t maxlevs;     /* number of AMR levels, 1..7 */
+  int32_t pad;
+  double lenx;         /* domain length parameter (lenx = leny) */
+  double cfl;
+  char ic[16];         /* SST SST1 SSTy BWx BWx1 BWy BWy1 rotor | rotor_p (parameterised rotor) */
+  char limiter[16];    /* 1st noSL minmod VanLeer Superbee MC Koren Cada3rd WENO3 WENO5 */
+  char flux[8];        /* RUSA HLLE HLLC HLLD */
+  double ic_params[8]; /* rotor_p: rho_in, v0, p, Bx, r0, r1, gamma, t_end */
+} amr_sim_config;
+
+typedef struct {
+  double t, dt, dt_next, gamma, t_end;
+  int32_t last_active, nleaf, count, done, status, maxblocks;
+} amr_inst_info;
+
+typedef struct {
+  int32_t code, inst, nb, i, j, pad;
+} amr_diag;
+
+typedef struct amr_ctx amr_ctx;
+
+/* Create a context advancing n_inst independent simulations together (n_inst
+ * = 1 is the reference's single run).  slot_capacity bounds the device slot
+ * arena over all instances (0: the sum of the instances' maxblocks).  All
+ * instances must share nx and nghosts. */
+int amr_create(int n_inst, const amr_sim_config *cfgs, int64_t slot_capacity, amr_ctx **out);
+int amr_destroy(amr_ctx *ctx);
+
+/* Base grid, initial condition and the initial regrid of every instance:
+ * basegrid.cpp:4-89, IC2Dtype.cpp:6-190, main.cpp:97-110 (admesh). */
+int amr_init(amr_ctx *ctx);
+
+/* One iteration of the reference time loop (main.cpp:128-247) for every
+ * instance that is still running: ghost fills, the two RK2 stages (MUSCL2D,
+ * riemannS, RK2ndv1, CT2D), the CFL reduction and the regrid (admesh).
+ * *n_active receives the number of instances still running afterwards. */
+int amr_step(amr_ctx *ctx, int *n_active);
+
+/* Steps until every instance has reached its end time (main.cpp:128) or
+ * max_steps iterations were done (max_steps < 0: no limit).  Returns the
+ * number of steps and the leaf-cell updates performed in *zone_updates. */
+int amr_run(amr_ctx *ctx, int max_steps, int64_t *steps, int64_t *zone_updates);
+
+int amr_info(amr_ctx *ctx, int inst, amr_inst_info *info);
+int amr_last_error(amr_ctx *ctx, amr_diag *diag);
+
+/* Copy slots [0, nslots) of one field of one instance to/from host memory,
+ * [nb][var][j][i] float64. */
+int amr_download(amr_ctx *ctx, int inst, int field, int nslots, double *host);
+int amr_upload(amr_ctx *ctx, int inst, int field, int nslots, const double *host);
+/* Quadtree of one instance: lp[nb][8], icoord[nb][2], leafs[nb] (defs.hpp:54-58) */
+int amr_get_tree(amr_ctx *ctx, int inst, int nslots, int32_t *lp, int32_t *icoord, int32_t *leafs);
+int amr_set_tree(amr_ctx *ctx, int inst, int nslots, const int32_t *lp, const int32_t *icoord, const int32_t *leafs);
+int amr_set_time(amr_ctx *ctx, int inst, double t, double dt, double dt_next, int32_t count);
+
+/* Individual phases of one step, for phase-level parity checks.
+ *  amr_fill:  boundary.cpp:7 (kind 0), boundaryfc.cpp:8 (1), boundaryec.cpp:7 (2)
+ *  amr_stage: MUSCL2D.cpp:13 + RK2ndv1 (timeIntegral.cpp:3) over the leaves
+ *  amr_ct:    CT2D.cpp:11
+ *  amr_cfl:   main.cpp:209-230, *lambda = min over leaves
+ *  amr_regrid: admesh.cpp:17 */
+int amr_fill(amr_ctx *ctx, int field, int kind);
+int amr_stage(amr_ctx *ctx, int step);
+int amr_ct(amr_ctx *ctx, int step);
+int amr_cfl(amr_ctx *ctx, int inst, double *lambda);
+int amr_regrid(amr_ctx *ctx);
+/* locate_bounds.cpp:35 for one instance: writes up to cap (source, neighbour,
+ * direction) triples, returns the count in *n */
+int amr_schedule(amr_ctx *ctx, int inst, int cap, int32_t *ib3, int *n);
+
+/* Pointwise parity entry points on DEVICE pointers (stream may be NULL):
+ * riemannS.cpp:7-56 for n face pairs wL[n][8], wR[n][8] -> flux[n][8];
+ * solver 0 RUSA 1 HLLE 2 HLLC 3 HLLD, dir 0 x 1 y; status[n] nonzero where
+ * the reference throws. */
+int amr_riemann_batch(int solver, int dir, double gamma, int n, const double *wL, const double *wR, double *flux,
+                      int32_t *status, void *stream);
+/* celledges.cpp:5-66 / WENO2D_primvar.cpp:7-198 on nt primitive tiles
+ * W[nt][8][NT][NT] -> out[nt][4][8][NT][NT] (wxL wxR wyL wyR), DEVICE pointers */
+int amr_recon_batch(int NT, int nghosts, const char *limiter, double dx, int nt, const double *W, double *out,
+                    void *stream);
+
+/* Kernel timing: accumulate HIP-event time per kernel family on the
+ * context's stream while enabled; read back as milliseconds and counts. */
+int amr_profile_enable(amr_ctx *ctx, int on);
+int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bar: bitwise equality (FP64, same evaluation order, no FMA contraction) of
+every slot field of every slot, the quadtree and the time/dt sequence."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import FIELD_MAP, compare_states, gpu_state, lockstep, oracle_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _physical_states(rng, n, bscale=1.0):
+    w = np.zeros((n, 8))
+    w[:, 0] = rng.uniform(0.1, 3.0, n)
+    w[:, 1:4] = rng.normal(0, 1.0, (n, 3))
+    w[:, 4] = rng.uniform(0.05, 3.0, n)
+    w[:, 5:8] = rng.normal(0, bscale, (n, 3))
+    return w
+
+
+def _torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.mark.parametrize("method", ["RUSA", "HLLE", "HLLC", "HLLD"])
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_riemann_batch_bitwise(method, direction):
+    torch = _torch()
+    import amr2d_mhd_amd as A
+
+    rng = np.random.default_rng(1000 + len(method) + (direction == "y"))
+    n = 4096
+    wL, wR = _physical_states(rng, n), _physical_states(rng, n)
+    # degenerate edges: identical states, zero normal field, zero field, strong jumps
+    wR[:64] = wL[:64]
+    wL[64:128, 5] = 0.0
+    wR[64:128, 5] = 0.0
+    wL[128:192, 5:8] = 0.0
+    wR[128:192, 5:8] = 0.0
+    wR[192:256, 0] *= 100.0
+    wL[256:320, 1] = 5.0
+    wR[256:320, 1] = 5.0
+    wL[320:384, 1] = -5.0
+    wR[320:384, 1] = -5.0
+    for gamma in (2.0, 1.4, 5.0 / 3.0):
+        fo, so = O.riemann(method, direction, gamma, wL, wR)
+        fg, sg = A.riemann_batch(method, direction, gamma, torch.from_numpy(wL).cuda(),
+                                 torch.from_numpy(wR).cuda())
+        fg, sg = fg.cpu().numpy(), sg.cpu().numpy()
+        assert np.array_equal(so != 0, sg != 0)
+        ok = so == 0
+        same = (fg[ok] == fo[ok]) | (np.isnan(fg[ok]) & np.isnan(fo[ok]))
+        assert same.all(), f"{method} {direction} gamma={gamma}: {int((~same).sum())} flux components differ"
+
+
+@pytest.mark.parametrize("lim,ng", [("MC", 2), ("minmod", 2), ("VanLeer", 2), ("Superbee", 2), ("Koren", 2),
+                                    ("Cada3rd", 2), ("1st", 2), ("noSL", 2), ("WENO3", 2), ("WENO5", 4)])
+def test_recon_batch_bitwise(lim, ng):
+    torch = _torch()
+    import amr2d_mhd_amd as A
+
+    NT = 16 + 2 * ng
+    rng = np.random.default_rng(7 + ng)
+    nt = 6
+    W = rng.uniform(0.2, 2.0, (nt, 8, NT, NT))
+    W[1] = np.round(W[1], 1)  # ties and flat regions
+    W[2, :, :, :NT // 2] = 1.0
+    out = A.recon_batch(torch.from_numpy(W).cuda(), lim, ng, dx=1.0 / 30).cpu().numpy()
+    weno = lim.startswith("WENO")
+    lo, hi = (ng - 1, NT - ng) if weno else (0, NT - 2)
+    for t in range(nt):
+        (xl, xr, yl, yr), st = O.recon_tile(W[t], lim, ng, dx=1.0 / 30)
+        ref = np.stack([xl, xr, yl, yr])
+        sl = (slice(None), slice(None), slice(lo, hi + 1), slice(lo, hi + 1))
+        a, b = out[t][sl], ref[sl]
+        a = a.copy()
+        b = b.copy()
+        a[0:2, 5] = b[0:2, 5] = 0  # x edges skip Bx, y edges skip By in the reference
+        a[2:4, 6] = b[2:4, 6] = 0
+        assert np.array_equal(a, b), f"{lim}: tile {t} differs in {int((a != b).sum())} entries"
+
+
+def _pair(ic, lim, flux, cfl, maxlevs, nghosts=2):
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(dict(ic=ic, limiter=lim, flux=flux, cfl=cfl, nx=16, nghosts=nghosts, maxlevs=maxlevs))
+    dom = O.Domain(ic=ic, limiter=lim, flux=flux, cfl=cfl, nx=16, nghosts=nghosts, maxlevs=maxlevs)
+    assert dom.init() == 0
+    sim.init()
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "initial state differs: " + "; ".join(d[:4])
+    return sim, dom
+
+
+def test_brio_wu_lv4_full_run_bitwise():
+    """Config 1: BWx, 4 levels, MC + Rusanov, CT, CFL 0.4, to t=0.2 (453 steps)."""
+    sim, dom = _pair("BWx", "MC", "RUSA", 0.4, 4)
+    n = lockstep(sim, dom, 1000, full_every=25, label="BWx lv4")
+    assert n == 453 and dom.done() and sim.info()["done"]
+    assert sim.info()["t"] == dom.info()["t"]
+
+
+def test_rotor_lv5_hlld_bitwise():
+    """Rotor (reference IC), 5 levels, MC + HLLD, CFL 0.5: first 150 steps."""
+    sim, dom = _pair("rotor", "MC", "HLLD", 0.5, 5)
+    lockstep(sim, dom, 150, full_every=30, label="rotor lv5")
+
+
+@pytest.mark.parametrize("flux", ["HLLE", "HLLC"])
+def test_rotor_lv4_other_solvers_bitwise(flux):
+    sim, dom = _pair("rotor", "MC", flux, 0.5, 4)
+    lockstep(sim, dom, 60, full_every=20, label=f"rotor {flux}")
+
+
+@pytest.mark.parametrize("ic,lim", [("BWy", "minmod"), ("BWx1", "VanLeer"), ("SST", "Superbee")])
+def test_other_ics_limiters_bitwise(ic, lim):
+    sim, dom = _pair(ic, lim, "RUSA", 0.4, 3)
+    lockstep(sim, dom, 60, full_every=20, label=f"{ic} {lim}")
+
+
+def test_weno5_ng4_bitwise():
+    """WENO5 needs nghosts=4 (24x24 tiles)."""
+    sim, dom = _pair("rotor", "WENO5", "HLLD", 0.4, 3, nghosts=4)
+    lockstep(sim, dom, 40, full_every=10, label="WENO5 ng4")
+
+
+def test_phases_on_oracle_state():
+    """Each phase applied to the same mid-run rotor state on both sides."""
+    import amr2d_mhd_amd as A
+
+    dom = O.Domain(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
+    dom.init()
+    for _ in range(40):
+        assert dom.step() == 0
+    st = oracle_state(dom)
+    sim = A.Simulation(dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5))
+    sim.init()
+    sim.load_state(st)
+    dom.locate_bounds()
+    assert np.array_equal(sim.schedule(), dom.locate_bounds())
+    phases = [("fill U", lambda: sim.fill("U", "cell"), lambda: dom.fill(O.U, O.FILL_CC)),
+              ("fill Bi", lambda: sim.fill("Bi", "face"), lambda: dom.fill(O.BI, O.FILL_FC)),
+              ("stage 1", lambda: sim.stage(1), lambda: dom.stage(1)),
+              ("fill ff", lambda: sim.fill("ff", "face"), lambda: dom.fill(O.FF, O.FILL_FC)),
+              ("fill gg", lambda: sim.fill("gg", "face"), lambda: dom.fill(O.GG, O.FILL_FC)),
+              ("ct 1", lambda: sim.ct(1), lambda: dom.ct(1)),
+              ("fill Us", lambda: sim.fill("Us", "cell"), lambda: dom.fill(O.US, O.FILL_CC)),
+              ("stage 2", lambda: sim.stage(2), lambda: dom.stage(2)),
+              ("ct 2", lambda: sim.ct(2), lambda: dom.ct(2))]
+    for name, g, o in phases:
+        g()
+        assert o() == 0
+        d = compare_states(gpu_state(sim), oracle_state(dom))
+        assert not d, f"after {name}: " + "; ".join(d[:4])
+    lam_g = sim.cfl()
+    st_o, lam_o = dom.cfl()
+    assert st_o == 0 and lam_g == lam_o
+
+
+def test_pressure_error_reported():
+    """Config 1 at CFL 0.5 fails at step 2 with p <= 0 in the reference (Us2W)."""
+    import amr2d_mhd_amd as A
+
+    dom = O.Domain(ic="BWx", limiter="MC", flux="RUSA", cfl=0.5, nx=16, nghosts=2, maxlevs=4)
+    dom.init()
+    codes = [dom.step() for _ in range(3)]
+    sim = A.Simulation(dict(ic="BWx", limiter="MC", flux="RUSA", cfl=0.5, nx=16, nghosts=2, maxlevs=4))
+    sim.init()
+    sim.step()
+    sim.step()
+    with pytest.raises(A.AmrError) as ei:
+        sim.step()
+    assert codes[:2] == [0, 0] and codes[2] == 1 and ei.value.code == 1
