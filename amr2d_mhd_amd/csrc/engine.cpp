@@ -12,6 +12,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -75,7 +76,7 @@ struct Profile {
   double ms = 0;
   int64_t launches = 0;
 };
-enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_COUNT };
+enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_COUNT };
 
 }  // namespace
 
@@ -114,9 +115,13 @@ struct amr_ctx {
   double* d_W = nullptr;  // initial-condition primitives for the first regrid
   bool topo_dirty = true;
   amr_diag diag{};
-  bool prof_on = false;
+  // kernel timing: families selected by prof_mask are bracketed by events on
+  // the context stream; elapsed times are collected after the next sync
+  unsigned prof_mask = 0;
   Profile prof[PF_COUNT];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<std::array<hipEvent_t, 2>> ev_pool;
+  std::vector<std::pair<int, int>> ev_pending;  // (family, pool index)
 };
 
 namespace {
@@ -156,22 +161,56 @@ int ensure_bytes(double*& p, size_t& cap, size_t n) {
 }
 
 // ----------------------------------------------------- timing families
+// With `attached`, the caller hands the two events to the launch itself
+// (hipExtLaunchKernelGGL) so they time the kernel; otherwise they are
+// recorded on the stream around the launch.
 struct Timed {
   amr_ctx* c;
-  int fam;
-  Timed(amr_ctx* c_, int f) : c(c_), fam(f) {
-    if (c->prof_on) hipEventRecord(c->ev0, c->stream);
+  int fam, slot = -1;
+  bool attached;
+  Timed(amr_ctx* c_, int f, bool attached_ = false) : c(c_), fam(f), attached(attached_) {
+    if (!(c->prof_mask & (1u << fam))) return;
+    slot = (int)c->ev_pending.size();
+    if (slot >= (int)c->ev_pool.size()) {
+      std::array<hipEvent_t, 2> e{};
+      hipEventCreate(&e[0]);
+      hipEventCreate(&e[1]);
+      c->ev_pool.push_back(e);
+    }
+    if (!attached) hipEventRecord(c->ev_pool[slot][0], c->stream);
   }
+  hipEvent_t start() const { return slot >= 0 ? c->ev_pool[slot][0] : nullptr; }
+  hipEvent_t stop() const { return slot >= 0 ? c->ev_pool[slot][1] : nullptr; }
   ~Timed() {
-    if (!c->prof_on) return;
-    hipEventRecord(c->ev1, c->stream);
-    hipEventSynchronize(c->ev1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->prof[fam].ms += ms;
+    if (slot < 0) return;
+    if (!attached) hipEventRecord(c->ev_pool[slot][1], c->stream);
+    c->ev_pending.push_back({fam, slot});
+  }
+};
+
+// host wall time of a scope, accumulated into a profile family
+struct HostTimed {
+  amr_ctx* c;
+  int fam;
+  std::chrono::steady_clock::time_point t0;
+  HostTimed(amr_ctx* c_, int f) : c(c_), fam(f), t0(std::chrono::steady_clock::now()) {}
+  ~HostTimed() {
+    if (!(c->prof_mask & (1u << fam))) return;
+    c->prof[fam].ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->prof[fam].launches += 1;
   }
 };
+
+// collect the elapsed times of the recorded launches (stream already synced)
+void flush_profile(amr_ctx* c) {
+  for (auto& p : c->ev_pending) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_pool[p.second][0], c->ev_pool[p.second][1]);
+    c->prof[p.first].ms += ms;
+    c->prof[p.first].launches += 1;
+  }
+  c->ev_pending.clear();
+}
 
 // -------------------------------------------------------- slot arena
 int slot_of(amr_ctx* c, Instance& I, int nb) {
@@ -947,6 +986,7 @@ int upload_params(amr_ctx* c) {
 
 // leaf work list and fill program of the running instances
 int rebuild_topology(amr_ctx* c) {
+  HostTimed ht(c, PF_HOST_TOPO);
   c->h_leaves.clear();
   c->leaf_inst_begin.assign(c->inst.size() + 1, 0);
   for (size_t q = 0; q < c->inst.size(); q++) {
@@ -1014,8 +1054,8 @@ void run_fill(amr_ctx* c, const FillSet& s) {
 }
 
 void run_stage(amr_ctx* c, int step) {
-  Timed tm(c, PF_STAGE);
-  launch_stage(c->NT, c->ng, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream);
+  Timed tm(c, PF_STAGE, true);
+  launch_stage(c->NT, c->ng, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream, tm.start(), tm.stop());
 }
 
 // CT2D.cpp:11-124
@@ -1038,7 +1078,11 @@ void run_ct(amr_ctx* c, int step) {
 
 int sync_errors(amr_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  {
+    HostTimed ht(c, PF_HOST_WAIT);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  flush_profile(c);
   if (c->h_err->code) {
     const DevError& e = *c->h_err;
     return set_error(c, e.code, e.inst, e.nb, e.i, e.j);
@@ -1100,6 +1144,7 @@ int regrid_all(amr_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));  // previous ops consumed
   refs.clear();
   coars.clear();
+  HostTimed ht(c, PF_HOST_REGRID);
   for (size_t q = 0; q < c->inst.size(); q++) {
     Instance& I = c->inst[q];
     if (I.done || I.maxlevs <= 1) continue;
@@ -1244,6 +1289,10 @@ int amr_destroy(amr_ctx* c) {
   if (c->h_err) hipHostFree(c->h_err);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (auto& e : c->ev_pool) {
+    hipEventDestroy(e[0]);
+    hipEventDestroy(e[1]);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return AMR_OK;
@@ -1477,16 +1526,19 @@ int amr_recon_batch(int NT, int nghosts, const char* limiter, double dx, int nt,
   return AMR_OK;
 }
 
-int amr_profile_enable(amr_ctx* c, int on) {
+int amr_profile_enable(amr_ctx* c, int mask) {
   if (!c) return AMR_ERR_CONFIG;
-  c->prof_on = on != 0;
-  if (on)
-    for (auto& p : c->prof) p = Profile{};
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_profile(c);
+  c->prof_mask = (unsigned)mask;
+  for (auto& p : c->prof) p = Profile{};
   return AMR_OK;
 }
 
 int amr_profile_read(amr_ctx* c, int family, double* ms, int64_t* launches) {
   if (!c || family < 0 || family >= PF_COUNT) return AMR_ERR_CONFIG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_profile(c);
   if (ms) *ms = c->prof[family].ms;
   if (launches) *launches = c->prof[family].launches;
   return AMR_OK;

@@ -5,6 +5,7 @@
 // doubles of one row.  One workgroup handles one leaf block; the block's
 // primitive tile and its face fluxes live in LDS for the whole stage.
 // No MFMA: nothing on this path is a dense contraction.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -579,8 +580,14 @@ bool tile_supported(int NT, int ng) { return (NT == 20 && ng == 2) || (NT == 24 
   } while (0)
 
 void launch_stage(int NT, int ng, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip,
-                  int step, DevError* err, hipStream_t s) {
+                  int step, DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (nleaf <= 0) return;
+  if (ev_start && ev_stop) {
+    // events attached to the dispatch itself: they time the kernel, not the launch
+    AMR_DISPATCH_TILE(NT, ng, (hipExtLaunchKernelGGL((stage_kernel<T, G>), dim3(nleaf), dim3(kThreads), 0, s,
+                                                     ev_start, ev_stop, 0, F, leaves, ip, step, err)));
+    return;
+  }
   AMR_DISPATCH_TILE(NT, ng, (stage_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, leaves, ip, step, err)));
 }
 

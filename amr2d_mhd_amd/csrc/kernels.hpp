@@ -30,7 +30,8 @@ struct RegridOp {
 bool tile_supported(int NT, int ng);
 
 void launch_stage(int NT, int ng, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip,
-                  int step, DevError* err, hipStream_t s);
+                  int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
+                  hipEvent_t ev_stop = nullptr);
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s);
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
                     hipStream_t s);

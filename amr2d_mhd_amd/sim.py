@@ -138,13 +138,18 @@ class Simulation:
         return ib
 
     # --------------------------------------------------------------- timing
-    def profile(self, on=True):
-        _check(self.h, lib().amr_profile_enable(self.h, 1 if on else 0))
+    PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait")
+
+    def profile(self, families=PROFILE_FAMILIES):
+        """Start HIP-event timing of the given kernel families (resets totals)."""
+        mask = 0
+        for f in families:
+            mask |= 1 << self.PROFILE_FAMILIES.index(f)
+        _check(self.h, lib().amr_profile_enable(self.h, mask))
 
     def profile_read(self):
-        names = ["stage", "fill", "ct", "cfl", "regrid"]
         out = {}
-        for q, nm in enumerate(names):
+        for q, nm in enumerate(self.PROFILE_FAMILIES):
             ms, n = C.c_double(0), C.c_int64(0)
             _check(self.h, lib().amr_profile_read(self.h, q, C.byref(ms), C.byref(n)))
             out[nm] = (ms.value, n.value)

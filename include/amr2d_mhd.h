@@ -123,9 +123,16 @@ int amr_riemann_batch(int solver, int dir, double gamma, int n, const double *wL
 int amr_recon_batch(int NT, int nghosts, const char *limiter, double dx, int nt, const double *W, double *out,
                     void *stream);
 
-/* Kernel timing: accumulate HIP-event time per kernel family on the
- * context's stream while enabled; read back as milliseconds and counts. */
-int amr_profile_enable(amr_ctx *ctx, int on);
+/* Kernel timing (no counterpart in the reference, which only keeps OpenMP
+ * wall-clock accumulators, main.cpp:124-127): launches of the kernel families
+ * selected by `mask` (bit 0 stage, 1 ghost fill, 2 CT, 3 CFL/criteria,
+ * 4 regrid) are bracketed by HIP events on the context's stream without extra
+ * synchronisation (the stage kernel's events are attached to its dispatch);
+ * bits 5-7 accumulate host wall time of the topology rebuild (work lists and
+ * fill program), of the regrid decisions and of the per-step wait for the
+ * device.  amr_profile_read returns the summed milliseconds and the number of
+ * launches (or host scopes) of one family since the last amr_profile_enable. */
+int amr_profile_enable(amr_ctx *ctx, int mask);
 int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
 
 #ifdef __cplusplus
