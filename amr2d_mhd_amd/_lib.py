@@ -74,7 +74,7 @@ _SIGS = {
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "amr_debug_host_fill": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_int, C.c_int, C.POINTER(C.c_int)]),
+                                      C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "amr_debug_coarsen_maps": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
 }

@@ -64,20 +64,33 @@ struct InstParams {
   int32_t pad;
 };
 
-// A ghost-fill reference: a pre-fill element (global slot, cell) or, with
-// kNodeBit set, the value of a nested restriction node evaluated earlier.
+// Ghost-fill program.  A reference to a value is either a pre-fill element
+// (global slot << 16 | cell) or, with kNodeBit set, a nested restriction
+// node evaluated earlier in the same fill.
 constexpr int64_t kNodeBit = int64_t(1) << 62;
 inline __host__ __device__ int64_t src_ref(int64_t gs, int cell) { return (gs << 16) | cell; }
 
-// One destination of a ghost fill: either a copy of ref[0] or a fine->coarse
-// restriction whose four operands are the cells (i1,j1),(i1,j1+1),(i1+1,j1),
-// (i1+1,j1+1) of the fine block.  The combine rule (cell 2x2 mean, face pair
-// mean, edge injection) is chosen by the field being filled.
+enum { kFillCopy = 0, kFillRestrict = 1, kFillGeneral = 2 };
+
+// One destination element of a fill (16 bytes):
+//   kFillCopy      value = pre-fill element (src, src_cell)
+//   kFillRestrict  fine->coarse restriction of the 2x2 cells starting at
+//                  (src, src_cell): (i1,j1), (i1,j1+1), (i1+1,j1), (i1+1,j1+1)
+//   kFillGeneral   restriction whose operands are given by FillGen[src]
+// The combine rule (cell 2x2 mean, face-pair mean, edge injection) is chosen
+// by the field being filled.
 struct FillOp {
-  int64_t ref[4];
   int32_t dst_gs;
-  int16_t dst_cell;
-  int16_t op;  // 0 copy, 1 restriction
+  uint16_t dst_cell;
+  int16_t op;
+  int32_t src;
+  uint16_t src_cell;
+  uint16_t pad;
+};
+
+// explicit operands of a restriction, in the order of FillOp's 2x2 cells
+struct FillGen {
+  int64_t ref[4];
 };
 
 // device error record (first error wins)

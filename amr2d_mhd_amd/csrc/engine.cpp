@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <chrono>
+#include <cstdlib>
+#include <new>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -78,6 +80,9 @@ struct Profile {
 };
 enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_COUNT };
 
+struct ProgState;  // fill compiler scratch + compiled program (defined below)
+void destroy_prog(ProgState* p);
+
 }  // namespace
 
 struct amr_ctx {
@@ -92,11 +97,13 @@ struct amr_ctx {
   std::vector<LeafItem> h_leaves;
   std::vector<int> leaf_inst_begin;  // per active instance, first leaf index
   // host copies kept alive until the stream has consumed them
-  std::vector<FillOp> h_ops, h_nodes;
   std::vector<RegridOp> h_refs, h_coars;
+  ProgState* prog = nullptr;
   FillOp* d_ops = nullptr;
   int nops = 0, cap_ops = 0;
-  FillOp* d_nodes = nullptr;
+  FillGen* d_gen = nullptr;
+  int ngen = 0, cap_gen = 0;
+  FillGen* d_nodes = nullptr;
   int nnodes = 0, cap_nodes = 0;
   std::vector<int32_t> node_levels{0};
   double* d_fillbuf = nullptr;
@@ -366,44 +373,108 @@ void build_schedule(Instance& I) {
 // Symbolic execution of the reference's sequential ghost fill
 // (boundary.cpp / boundaryfc.cpp / boundaryec.cpp, extboundary.cpp): each
 // element holds a symbolic value -- its own pre-fill value, a pre-fill value
-// of another element (copies collapse) or a restriction node over four
-// elements.  The final symbolic value of every written element becomes one
-// FillOp, evaluated on the device as gather (from the pre-fill state) and
-// scatter, which reproduces the sequential last-writer semantics without
-// races.  One program serves all fields: only the restriction arithmetic
-// depends on the field kind.
+// of another element (copies collapse) or a restriction node.  The final
+// symbolic value of every written element becomes one FillOp, evaluated on
+// the device as gather (from the pre-fill state) and scatter, which
+// reproduces the sequential last-writer semantics without races.  One
+// program serves all fields: only the restriction arithmetic depends on the
+// field kind.
+
+// restriction node: `simple` when its four operand cells still hold their
+// pre-fill values (the common case), else explicit operand references
+struct RNode {
+  int32_t gs;
+  uint16_t cell;
+  bool simple;
+  int64_t ref[4];
+};
+
+// host scratch reused across compiles (cur[] is reset only where touched)
+struct FillScratch {
+  std::vector<int64_t> cur;
+  std::vector<int32_t> touched;
+  std::vector<RNode> nodes;
+  std::vector<int64_t> memo;
+  bool force_general = false;  // test hook: emit every restriction as kFillGeneral
+};
+
+// page-locked growable array, so uploads are true async DMA (falls back to
+// ordinary memory where no device is present, e.g. for the host test hooks)
+template <class T>
+struct PinnedVec {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  bool pinned = false;
+  ~PinnedVec() { release(p, pinned); }
+  static void release(T* q, bool pin) {
+    if (!q) return;
+    if (pin) hipHostFree(q);
+    else std::free(q);
+  }
+  void clear() { n = 0; }
+  void reserve(size_t m) {
+    if (m <= cap) return;
+    size_t nc = std::max(m, cap * 2 + 1024);
+    T* q = nullptr;
+    bool pin = hipHostMalloc((void**)&q, nc * sizeof(T)) == hipSuccess;
+    if (!pin) {
+      q = (T*)std::malloc(nc * sizeof(T));
+      if (!q) throw std::bad_alloc();
+    }
+    if (n) std::memcpy(q, p, n * sizeof(T));
+    release(p, pinned);
+    p = q;
+    pinned = pin;
+    cap = nc;
+  }
+  void push_back(const T& v) {
+    if (n == cap) reserve(n + 1);
+    p[n++] = v;
+  }
+  size_t size() const { return n; }
+  T* data() { return p; }
+};
+
 struct FillCompiler {
   const Instance& I;
   int NT, ng, N2;
-  std::vector<int64_t> cur;       // per (nb, cell): symbolic value, -1 = untouched
-  std::vector<int32_t> touched;   // elements written, in first-write order
-  std::vector<FillOp>& nodes;     // restriction nodes (refs may point to nodes)
+  FillScratch& S;
 
-  FillCompiler(const Instance& inst, int NT_, int ng_, std::vector<FillOp>& nodes_)
-      : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), nodes(nodes_) {
-    cur.assign((size_t)I.lastActive * N2, -1);
+  FillCompiler(const Instance& inst, int NT_, int ng_, FillScratch& s) : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), S(s) {
+    const size_t need = (size_t)I.lastActive * N2;
+    if (S.cur.size() < need) S.cur.resize(need, -1);
+    S.touched.clear();
+    S.nodes.clear();
   }
+  size_t addr(int nb, int i, int j) const { return (size_t)nb * N2 + j * NT + i; }
   int64_t read(int nb, int i, int j) const {
-    const int64_t v = cur[(size_t)nb * N2 + j * NT + i];
+    const int64_t v = S.cur[addr(nb, i, j)];
     return v >= 0 ? v : src_ref(I.gs[nb], j * NT + i);
   }
   void write(int nb, int i, int j, int64_t v) {
-    const size_t a = (size_t)nb * N2 + j * NT + i;
-    if (cur[a] < 0) touched.push_back((int32_t)a);
-    cur[a] = v;
+    const size_t a = addr(nb, i, j);
+    if (S.cur[a] < 0) S.touched.push_back((int32_t)a);
+    S.cur[a] = v;
   }
   void copy(int dn, int di, int dj, int sn, int si, int sj) { write(dn, di, dj, read(sn, si, sj)); }
   void restrict_(int dn, int di, int dj, int fn, int i1, int j1) {
-    FillOp op{};
-    op.ref[0] = read(fn, i1, j1);
-    op.ref[1] = read(fn, i1, j1 + 1);
-    op.ref[2] = read(fn, i1 + 1, j1);
-    op.ref[3] = read(fn, i1 + 1, j1 + 1);
-    op.op = 1;
-    nodes.push_back(op);
-    write(dn, di, dj, kNodeBit | (int64_t)(nodes.size() - 1));
+    RNode nd{};
+    nd.gs = I.gs[fn];
+    nd.cell = (uint16_t)(j1 * NT + i1);
+    nd.simple = !S.force_general && S.cur[addr(fn, i1, j1)] < 0 && S.cur[addr(fn, i1, j1 + 1)] < 0 &&
+                S.cur[addr(fn, i1 + 1, j1)] < 0 && S.cur[addr(fn, i1 + 1, j1 + 1)] < 0;
+    if (!nd.simple) {
+      nd.ref[0] = read(fn, i1, j1);
+      nd.ref[1] = read(fn, i1, j1 + 1);
+      nd.ref[2] = read(fn, i1 + 1, j1);
+      nd.ref[3] = read(fn, i1 + 1, j1 + 1);
+    }
+    S.nodes.push_back(nd);
+    write(dn, di, dj, kNodeBit | (int64_t)(S.nodes.size() - 1));
   }
-
+  void finish() {
+    for (int32_t a : S.touched) S.cur[a] = -1;
+  }
   // corner of an entry: the diagonal block `cb` found through the neighbour
   // table is copied when it is a leaf (with same-level indices even if it is
   // coarser), else its designated son is restricted when that is a leaf.
@@ -598,95 +669,114 @@ struct FillCompiler {
   }
 };
 
-// Flatten nested restriction nodes into levels (a node whose operands are
-// all pre-fill elements has level 1).  Destination ops referencing a node
-// inline it; only nodes referenced from inside other ops are materialised.
-struct ProgramBuilder {
-  std::vector<FillOp> ops, nested;
+// Device program of all running instances: 16-byte destination ops, the
+// explicit-operand table for non-simple restrictions and nested nodes (only
+// when a restriction reads an element written earlier in the same fill),
+// grouped by evaluation level.
+struct Program {
+  PinnedVec<FillOp> ops;
+  std::vector<FillGen> gen;
+  std::vector<std::vector<FillGen>> nested_lv;  // [level-1][i]
+  std::vector<FillGen> nested;
   std::vector<int32_t> level_start{0};
 };
 
-int compile_programs(amr_ctx* c, ProgramBuilder& pb) {
-  std::vector<FillOp> all_nodes;
-  std::vector<FillOp> dest_ops;
-  std::vector<int32_t> node_base_of_inst;
+constexpr int64_t kTmpLevelShift = 40;
+
+// value of a nested node as a temporary reference kNodeBit|level<<40|index
+int64_t materialize(Program& P, FillScratch& S, int NT, int64_t n) {
+  if (S.memo[n] >= 0) return S.memo[n];
+  const RNode& nd = S.nodes[n];
+  FillGen g{};
+  int lvl = 1;
+  const int off[4] = {0, NT, 1, NT + 1};
+  for (int q = 0; q < 4; q++) {
+    int64_t r = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
+    if (r & kNodeBit) {
+      r = materialize(P, S, NT, r & ~kNodeBit);
+      lvl = std::max(lvl, (int)((r & ~kNodeBit) >> kTmpLevelShift) + 1);
+    }
+    g.ref[q] = r;
+  }
+  if ((int)P.nested_lv.size() < lvl) P.nested_lv.resize(lvl);
+  P.nested_lv[lvl - 1].push_back(g);
+  const int64_t t = kNodeBit | ((int64_t)lvl << kTmpLevelShift) | (int64_t)(P.nested_lv[lvl - 1].size() - 1);
+  S.memo[n] = t;
+  return t;
+}
+
+void compile_programs(amr_ctx* c, Program& P, FillScratch& S) {
+  P.ops.clear();
+  P.gen.clear();
+  P.nested_lv.clear();
+  const int NT = c->NT, N2 = NT * NT;
+  const int off[4] = {0, NT, 1, NT + 1};
   for (auto& I : c->inst) {
     if (I.done) continue;
-    const size_t base = all_nodes.size();
-    std::vector<FillOp> nodes;
-    FillCompiler fc(I, c->NT, c->ng, nodes);
+    FillCompiler fc(I, NT, c->ng, S);
     fc.run();
-    auto rebase = [&](int64_t r) { return (r & kNodeBit) ? (kNodeBit | ((r & ~kNodeBit) + (int64_t)base)) : r; };
-    for (auto& n : nodes)
-      for (auto& r : n.ref) r = rebase(r);
-    all_nodes.insert(all_nodes.end(), nodes.begin(), nodes.end());
-    const int N2 = c->NT * c->NT;
-    for (int32_t a : fc.touched) {
+    S.memo.assign(S.nodes.size(), -1);
+    P.ops.reserve(P.ops.size() + S.touched.size());
+    for (int32_t a : S.touched) {
       const int nb = a / N2, cell = a % N2;
-      const int64_t v = rebase(fc.cur[a]);
+      const int64_t v = S.cur[a];
       FillOp op{};
       op.dst_gs = I.gs[nb];
-      op.dst_cell = (int16_t)cell;
-      if (v & kNodeBit) {
-        op = all_nodes[v & ~kNodeBit];
-        op.dst_gs = I.gs[nb];
-        op.dst_cell = (int16_t)cell;
-      } else {
+      op.dst_cell = (uint16_t)cell;
+      if (!(v & kNodeBit)) {
         if (v == src_ref(I.gs[nb], cell)) continue;  // value unchanged
-        op.op = 0;
-        op.ref[0] = v;
+        op.op = kFillCopy;
+        op.src = (int32_t)(v >> 16);
+        op.src_cell = (uint16_t)(v & 0xffff);
+      } else {
+        const RNode& nd = S.nodes[v & ~kNodeBit];
+        if (nd.simple && !S.force_general) {
+          op.op = kFillRestrict;
+          op.src = nd.gs;
+          op.src_cell = nd.cell;
+        } else {
+          FillGen g{};
+          for (int q = 0; q < 4; q++) {
+            int64_t r = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
+            if (r & kNodeBit) r = materialize(P, S, NT, r & ~kNodeBit);
+            g.ref[q] = r;
+          }
+          op.op = kFillGeneral;
+          op.src = (int32_t)P.gen.size();
+          P.gen.push_back(g);
+        }
       }
-      dest_ops.push_back(op);
+      P.ops.push_back(op);
     }
+    fc.finish();
   }
-  // levels of nodes referenced from destination ops (recursively)
-  std::vector<int> level(all_nodes.size(), 0);
-  std::vector<int> stack;
-  std::function<int(int64_t)> lvl = [&](int64_t r) -> int {
-    if (!(r & kNodeBit)) return 0;
-    const int64_t n = r & ~kNodeBit;
-    if (level[n] > 0) return level[n];
-    int m = 0;
-    for (auto rr : all_nodes[n].ref) m = std::max(m, lvl(rr));
-    level[n] = m + 1;
-    return level[n];
+  // final node numbering: levels in order
+  std::vector<int64_t> lv_off(P.nested_lv.size() + 1, 0);
+  for (size_t l = 0; l < P.nested_lv.size(); l++) lv_off[l + 1] = lv_off[l] + (int64_t)P.nested_lv[l].size();
+  auto remap = [&](int64_t r) {
+    if (!(r & kNodeBit)) return r;
+    const int64_t t = r & ~kNodeBit;
+    const int64_t lvl = t >> kTmpLevelShift, idx = t & ((int64_t(1) << kTmpLevelShift) - 1);
+    return kNodeBit | (lv_off[lvl - 1] + idx);
   };
-  std::vector<int64_t> needed;
-  for (auto& op : dest_ops)
-    for (int q = 0; q < (op.op ? 4 : 1); q++)
-      if (op.ref[q] & kNodeBit) needed.push_back(op.ref[q] & ~kNodeBit);
-  std::vector<uint8_t> mark(all_nodes.size(), 0);
-  std::vector<int64_t> order;
-  while (!needed.empty()) {
-    int64_t n = needed.back();
-    needed.pop_back();
-    if (mark[n]) continue;
-    mark[n] = 1;
-    order.push_back(n);
-    for (auto r : all_nodes[n].ref)
-      if (r & kNodeBit) needed.push_back(r & ~kNodeBit);
+  P.nested.clear();
+  P.level_start.assign(1, 0);
+  for (auto& lvv : P.nested_lv) {
+    for (auto g : lvv) {
+      for (auto& r : g.ref) r = remap(r);
+      P.nested.push_back(g);
+    }
+    P.level_start.push_back((int32_t)P.nested.size());
   }
-  int maxl = 0;
-  for (auto n : order) maxl = std::max(maxl, lvl(kNodeBit | n));
-  std::vector<int64_t> newid(all_nodes.size(), -1);
-  pb.nested.clear();
-  pb.level_start.assign(1, 0);
-  for (int l = 1; l <= maxl; l++) {
-    for (auto n : order)
-      if (level[n] == l) {
-        newid[n] = (int64_t)pb.nested.size();
-        pb.nested.push_back(all_nodes[n]);
-      }
-    pb.level_start.push_back((int32_t)pb.nested.size());
-  }
-  auto remap = [&](int64_t r) { return (r & kNodeBit) ? (kNodeBit | newid[r & ~kNodeBit]) : r; };
-  for (auto& n : pb.nested)
-    for (auto& r : n.ref) r = remap(r);
-  for (auto& op : dest_ops)
-    for (auto& r : op.ref) r = remap(r);
-  pb.ops.swap(dest_ops);
-  return 0;
+  for (auto& g : P.gen)
+    for (auto& r : g.ref) r = remap(r);
 }
+
+struct ProgState {
+  FillScratch scratch;
+  Program prog;
+};
+void destroy_prog(ProgState* p) { delete p; }
 
 // ------------------------------------------------- tree bookkeeping
 // updatelpup.cpp:4-118: link four new sons of nb into the neighbour table
@@ -1003,21 +1093,23 @@ int rebuild_topology(amr_ctx* c) {
   if (c->nleaf)
     HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
                           c->stream));
-  ProgramBuilder pb;
-  compile_programs(c, pb);
-  c->h_ops.swap(pb.ops);
-  c->h_nodes.swap(pb.nested);
-  pb.ops = c->h_ops;
-  pb.nested = c->h_nodes;
-  c->nops = (int)pb.ops.size();
-  c->nnodes = (int)pb.nested.size();
-  c->node_levels = pb.level_start;
+  if (!c->prog) c->prog = new ProgState();
+  HIPCHK(hipStreamSynchronize(c->stream));  // the previous program upload has been consumed
+  Program& P = c->prog->prog;
+  compile_programs(c, P, c->prog->scratch);
+  c->nops = (int)P.ops.size();
+  c->ngen = (int)P.gen.size();
+  c->nnodes = (int)P.nested.size();
+  c->node_levels = P.level_start;
   if (int e = ensure(c->d_ops, c->cap_ops, std::max(1, c->nops))) return e;
+  if (int e = ensure(c->d_gen, c->cap_gen, std::max(1, c->ngen))) return e;
   if (int e = ensure(c->d_nodes, c->cap_nodes, std::max(1, c->nnodes))) return e;
   if (c->nops)
-    HIPCHK(hipMemcpyAsync(c->d_ops, c->h_ops.data(), sizeof(FillOp) * c->nops, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_ops, P.ops.data(), sizeof(FillOp) * c->nops, hipMemcpyHostToDevice, c->stream));
+  if (c->ngen)
+    HIPCHK(hipMemcpyAsync(c->d_gen, P.gen.data(), sizeof(FillGen) * c->ngen, hipMemcpyHostToDevice, c->stream));
   if (c->nnodes)
-    HIPCHK(hipMemcpyAsync(c->d_nodes, c->h_nodes.data(), sizeof(FillOp) * c->nnodes, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(c->d_nodes, P.nested.data(), sizeof(FillGen) * c->nnodes, hipMemcpyHostToDevice,
                           c->stream));
   if (int e = ensure_bytes(c->d_fillbuf, c->cap_fillbuf, (size_t)4 * 8 * std::max(1, c->nops))) return e;
   if (int e = ensure_bytes(c->d_nodebuf, c->cap_nodebuf, (size_t)4 * 8 * std::max(1, c->nnodes))) return e;
@@ -1049,8 +1141,8 @@ FillSet make_set(amr_ctx* c, std::initializer_list<std::pair<int, int>> fk) {
 
 void run_fill(amr_ctx* c, const FillSet& s) {
   Timed tm(c, PF_FILL);
-  launch_fill(c->NT, c->d_ops, c->nops, c->d_nodes, c->node_levels.data(), (int)c->node_levels.size() - 1,
-              c->nnodes, s, c->d_fillbuf, c->d_nodebuf, c->stream);
+  launch_fill(c->NT, c->d_ops, c->nops, c->d_gen, c->d_nodes, c->node_levels.data(),
+              (int)c->node_levels.size() - 1, c->nnodes, s, c->d_fillbuf, c->d_nodebuf, c->stream);
 }
 
 void run_stage(amr_ctx* c, int step) {
@@ -1281,7 +1373,9 @@ int amr_destroy(amr_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < kNumFields; f++)
     if (c->F.f[f]) hipFree(c->F.f[f]);
-  hipFree(c->d_leaves); hipFree(c->d_ops); hipFree(c->d_nodes); hipFree(c->d_fillbuf); hipFree(c->d_nodebuf);
+  hipFree(c->d_leaves); hipFree(c->d_ops); hipFree(c->d_gen); hipFree(c->d_nodes); hipFree(c->d_fillbuf);
+  hipFree(c->d_nodebuf);
+  destroy_prog(c->prog);
   hipFree(c->d_ip); hipFree(c->d_lam); hipFree(c->d_grad); hipFree(c->d_err); hipFree(c->d_rops);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
   if (c->h_lam) hipHostFree(c->h_lam);
@@ -1553,36 +1647,44 @@ int amr_profile_read(amr_ctx* c, int family, double* ms, int64_t* launches) {
 // ====================================================================
 namespace {
 
-double host_fill_load(const double* Q, int nv, int N2, const std::vector<double>& nodeval, int64_t ref, int k) {
-  if (ref & kNodeBit) return nodeval[(ref & ~kNodeBit) * 8 + k];
-  return Q[((size_t)(ref >> 16) * nv + k) * N2 + (ref & 0xffff)];
-}
-
-double host_fill_value(const double* Q, int nv, int N2, int kind, const std::vector<double>& nodeval,
-                       const FillOp& op, int k) {
-  if (op.op == 0) return host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
-  double v = 0.;
-  if (kind == kFillCell) {
-    for (int q = 0; q < 4; q++) v += host_fill_load(Q, nv, N2, nodeval, op.ref[q], k);
-    return 0.25 * v;
+// host mirror of the device fill evaluation (fill_value in kernels.hip)
+struct HostFill {
+  const double* Q;
+  int nv, NT, N2, kind;
+  const std::vector<double>& nodeval;
+  const std::vector<FillGen>& gen;
+  double elem(int64_t gs, int cell, int k) const { return Q[((size_t)gs * nv + k) * N2 + cell]; }
+  double ref(int64_t r, int k) const {
+    if (r & kNodeBit) return nodeval[(r & ~kNodeBit) * 8 + k];
+    return elem(r >> 16, (int)(r & 0xffff), k);
   }
-  if (kind == kFillFace) {
-    v += host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
-    v += host_fill_load(Q, nv, N2, nodeval, op.ref[k == 0 ? 1 : 2], k);
-    return 0.5 * v;
+  double combine(int k, double a, double b, double c, double d) const {
+    double v = 0.;
+    if (kind == kFillCell) { v += a; v += b; v += c; v += d; return 0.25 * v; }
+    if (kind == kFillFace) { v += a; v += k == 0 ? b : c; return 0.5 * v; }
+    return a;
   }
-  return host_fill_load(Q, nv, N2, nodeval, op.ref[0], k);
-}
+  double general(const FillGen& g, int k) const { return combine(k, ref(g.ref[0], k), ref(g.ref[1], k), ref(g.ref[2], k), ref(g.ref[3], k)); }
+  double value(const FillOp& op, int k) const {
+    if (op.op == kFillCopy) return elem(op.src, op.src_cell, k);
+    if (op.op == kFillRestrict)
+      return combine(k, elem(op.src, op.src_cell, k), elem(op.src, op.src_cell + NT, k), elem(op.src, op.src_cell + 1, k),
+                     elem(op.src, op.src_cell + 1 + NT, k));
+    return general(gen[op.src], k);
+  }
+};
 
 }  // namespace
 
 extern "C" {
 
 // Applies the fill program compiled for the given tree to Q[n][nv][NT][NT]
-// in place; returns the number of destination ops (negative on error) and the
-// number of nested restriction nodes in *nnodes.
+// in place (emulating the device's gather/scatter on the host); returns the
+// number of destination ops (negative on error); *nnodes receives the number
+// of nested restriction nodes.  force_general != 0 emits every restriction
+// through the explicit-operand path.
 int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
-                        double* Q, int nv, int kind, int* nnodes) {
+                        double* Q, int nv, int kind, int* nnodes, int force_general) {
   amr_ctx c;
   c.NT = NT;
   c.ng = ng;
@@ -1598,19 +1700,23 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
     I.gs.push_back(nb);
   }
   build_schedule(I);
-  ProgramBuilder pb;
-  compile_programs(&c, pb);
+  ProgState ps;
+  ps.scratch.force_general = force_general != 0;
+  compile_programs(&c, ps.prog, ps.scratch);
+  Program& P = ps.prog;
   const int N2 = NT * NT;
-  std::vector<double> nodeval(pb.nested.size() * 8, 0.0), buf(pb.ops.size() * 8, 0.0);
-  for (size_t l = 0; l + 1 < pb.level_start.size(); l++)
-    for (int e = pb.level_start[l]; e < pb.level_start[l + 1]; e++)
-      for (int k = 0; k < nv; k++) nodeval[e * 8 + k] = host_fill_value(Q, nv, N2, kind, nodeval, pb.nested[e], k);
-  for (size_t e = 0; e < pb.ops.size(); e++)
-    for (int k = 0; k < nv; k++) buf[e * 8 + k] = host_fill_value(Q, nv, N2, kind, nodeval, pb.ops[e], k);
-  for (size_t e = 0; e < pb.ops.size(); e++)
-    for (int k = 0; k < nv; k++) Q[((size_t)pb.ops[e].dst_gs * nv + k) * N2 + pb.ops[e].dst_cell] = buf[e * 8 + k];
-  if (nnodes) *nnodes = (int)pb.nested.size();
-  return (int)pb.ops.size();
+  std::vector<double> nodeval(P.nested.size() * 8, 0.0), buf(P.ops.size() * 8, 0.0);
+  HostFill hf{Q, nv, NT, N2, kind, nodeval, P.gen};
+  for (size_t l = 0; l + 1 < P.level_start.size(); l++)
+    for (int e = P.level_start[l]; e < P.level_start[l + 1]; e++)
+      for (int k = 0; k < nv; k++) nodeval[e * 8 + k] = hf.general(P.nested[e], k);
+  FillOp* ops = P.ops.data();
+  for (size_t e = 0; e < P.ops.size(); e++)
+    for (int k = 0; k < nv; k++) buf[e * 8 + k] = hf.value(ops[e], k);
+  for (size_t e = 0; e < P.ops.size(); e++)
+    for (int k = 0; k < nv; k++) Q[((size_t)ops[e].dst_gs * nv + k) * N2 + ops[e].dst_cell] = buf[e * 8 + k];
+  if (nnodes) *nnodes = (int)P.nested.size();
+  return (int)P.ops.size();
 }
 
 // coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]

@@ -244,55 +244,73 @@ __global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const Leaf
 // evaluated level by level), then all destinations are written.
 struct FillCtx {
   const double* base;
-  int nv, kind, N2;
+  int nv, kind, N2, NT;
   const double* nodes;  // node values of this field, [node][8]
+  const FillGen* gen;
 };
 
-__device__ inline double fill_load(const FillCtx& f, int64_t ref, int k) {
-  if (ref & kNodeBit) return f.nodes[(ref & ~kNodeBit) * 8 + k];
-  const int64_t gs = ref >> 16;
-  const int cell = (int)(ref & 0xffff);
+__device__ inline double fill_elem(const FillCtx& f, int64_t gs, int cell, int k) {
   return f.base[((size_t)gs * f.nv + k) * f.N2 + cell];
+}
+
+__device__ inline double fill_ref(const FillCtx& f, int64_t ref, int k) {
+  if (ref & kNodeBit) return f.nodes[(ref & ~kNodeBit) * 8 + k];
+  return fill_elem(f, ref >> 16, (int)(ref & 0xffff), k);
 }
 
 // boundary.cpp (0.25 x 2x2 sum in misc.cpp:35-48 order), boundaryfc.cpp
 // (0.5 x face pair: along j for component 0, along i otherwise) and
-// boundaryec.cpp (injection)
-__device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
-  if (op.op == 0) return fill_load(f, op.ref[0], k);
+// boundaryec.cpp (injection); v[] holds (i1,j1),(i1,j1+1),(i1+1,j1),(i1+1,j1+1)
+__device__ inline double restrict_combine(int kind, int k, double v00, double v01, double v10, double v11) {
   double v = 0.;
-  if (f.kind == kFillCell) {
-    v += fill_load(f, op.ref[0], k);
-    v += fill_load(f, op.ref[1], k);
-    v += fill_load(f, op.ref[2], k);
-    v += fill_load(f, op.ref[3], k);
+  if (kind == kFillCell) {
+    v += v00;
+    v += v01;
+    v += v10;
+    v += v11;
     return 0.25 * v;
   }
-  if (f.kind == kFillFace) {
-    v += fill_load(f, op.ref[0], k);
-    v += fill_load(f, op.ref[k == 0 ? 1 : 2], k);
+  if (kind == kFillFace) {
+    v += v00;
+    v += k == 0 ? v01 : v10;
     return 0.5 * v;
   }
-  return fill_load(f, op.ref[0], k);
+  return v00;
 }
 
-__global__ __launch_bounds__(kThreads) void fill_node_kernel(const FillOp* __restrict__ nodes, int begin, int end,
-                                                             int nnodes, FillSet set, double* nodebuf, int N2) {
+__device__ inline double gen_value(const FillCtx& f, const FillGen& g, int k) {
+  return restrict_combine(f.kind, k, fill_ref(f, g.ref[0], k), fill_ref(f, g.ref[1], k), fill_ref(f, g.ref[2], k),
+                          fill_ref(f, g.ref[3], k));
+}
+
+__device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
+  if (op.op == kFillCopy) return fill_elem(f, op.src, op.src_cell, k);
+  if (op.op == kFillRestrict) {
+    const int a = op.src_cell;
+    return restrict_combine(f.kind, k, fill_elem(f, op.src, a, k), fill_elem(f, op.src, a + f.NT, k),
+                            fill_elem(f, op.src, a + 1, k), fill_elem(f, op.src, a + 1 + f.NT, k));
+  }
+  return gen_value(f, f.gen[op.src], k);
+}
+
+__global__ __launch_bounds__(kThreads) void fill_node_kernel(const FillGen* __restrict__ nodes, int begin, int end,
+                                                             int nnodes, FillSet set, double* nodebuf, int NT) {
   const int e = begin + blockIdx.x * kThreads + threadIdx.x;
   if (e >= end) return;
   const int fi = blockIdx.y;
-  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], N2, nodebuf + (size_t)fi * nnodes * 8};
-  const FillOp op = nodes[e];
-  for (int k = 0; k < f.nv; k++) nodebuf[((size_t)fi * nnodes + e) * 8 + k] = fill_value(f, op, k);
+  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, nodebuf + (size_t)fi * nnodes * 8, nullptr};
+  const FillGen g = nodes[e];
+  for (int k = 0; k < f.nv; k++) nodebuf[((size_t)fi * nnodes + e) * 8 + k] = gen_value(f, g, k);
 }
 
 __global__ __launch_bounds__(kThreads) void fill_gather_kernel(const FillOp* __restrict__ ops, int nops,
-                                                               FillSet set, double* buf, const double* nodebuf,
-                                                               int nnodes, int N2) {
+                                                               const FillGen* __restrict__ gen, FillSet set,
+                                                               double* buf, const double* nodebuf, int nnodes,
+                                                               int NT) {
   const int e = blockIdx.x * kThreads + threadIdx.x;
   if (e >= nops) return;
   const int fi = blockIdx.y;
-  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], N2, nodebuf + (size_t)fi * nnodes * 8};
+  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, nodebuf + (size_t)fi * nnodes * 8, gen};
   const FillOp op = ops[e];
   for (int k = 0; k < f.nv; k++) buf[((size_t)fi * nops + e) * 8 + k] = fill_value(f, op, k);
 }
@@ -610,19 +628,19 @@ void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
   AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, step)));
 }
 
-void launch_fill(int NT, const FillOp* ops, int nops, const FillOp* nodes, const int32_t* node_level_start,
-                 int nlevels, int nnodes, const FillSet& set, double* buf, double* nodebuf, hipStream_t s) {
+void launch_fill(int NT, const FillOp* ops, int nops, const FillGen* gen, const FillGen* nodes,
+                 const int32_t* node_level_start, int nlevels, int nnodes, const FillSet& set, double* buf,
+                 double* nodebuf, hipStream_t s) {
   if (nops <= 0 || set.n <= 0) return;
-  const int N2 = NT * NT;
   for (int l = 0; l < nlevels; l++) {
     const int b = node_level_start[l], e = node_level_start[l + 1];
     if (e <= b) continue;
     dim3 g((e - b + kThreads - 1) / kThreads, set.n);
-    fill_node_kernel<<<g, kThreads, 0, s>>>(nodes, b, e, nnodes, set, nodebuf, N2);
+    fill_node_kernel<<<g, kThreads, 0, s>>>(nodes, b, e, nnodes, set, nodebuf, NT);
   }
   dim3 grid((nops + kThreads - 1) / kThreads, set.n);
-  fill_gather_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, nodebuf, nnodes, N2);
-  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, N2);
+  fill_gather_kernel<<<grid, kThreads, 0, s>>>(ops, nops, gen, set, buf, nodebuf, nnodes, NT);
+  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, NT * NT);
 }
 
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,

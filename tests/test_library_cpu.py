@@ -53,9 +53,10 @@ need_ref = pytest.mark.skipif(not refrun.have_ref(2, 5), reason="compiled refere
 
 
 @need_ref
+@pytest.mark.parametrize("general", [0, 1])
 @pytest.mark.parametrize("ng,lv,seed,pr,pc,rounds", [(2, 4, 1, 0.5, 0.2, 5), (2, 6, 7, 0.3, 0.3, 8),
                                                      (4, 5, 3, 0.4, 0.25, 6), (2, 5, 99, 0.25, 0.5, 10)])
-def test_fill_program_compiler_matches_reference(tmp_path, ng, lv, seed, pr, pc, rounds):
+def test_fill_program_compiler_matches_reference(tmp_path, ng, lv, seed, pr, pc, rounds, general):
     """The symbolic gather program (host compiler + gather/scatter semantics,
     emulated on the host) equals the reference's sequential fills."""
     from amr2d_mhd_amd._lib import lib
@@ -69,7 +70,7 @@ def test_fill_program_compiler_matches_reference(tmp_path, ng, lv, seed, pr, pc,
         Q = np.ascontiguousarray(pre[fld]).copy()
         nn = C.c_int(0)
         nops = lib().amr_debug_host_fill(NT, ng, n, pre["lp"].ctypes.data, pre["icoord"].ctypes.data,
-                                         pre["leafs"].ctypes.data, Q.ctypes.data, nv, kind, C.byref(nn))
+                                         pre["leafs"].ctypes.data, Q.ctypes.data, nv, kind, C.byref(nn), general)
         assert nops > 0
         assert np.array_equal(Q, post[fld]), fld
 
