@@ -17,6 +17,7 @@
 #include <new>
 #include <cstring>
 #include <functional>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,73 @@ int ic_code(const char* s) {
   return -1;
 }
 
+// restriction node: `simple` when its four operand cells still hold their
+// pre-fill values (the common case), else explicit operand references
+struct RNode {
+  int32_t gs;
+  uint16_t cell;
+  bool simple;
+  int64_t ref[4];
+};
+
+// host scratch reused across compiles (cur[] is reset only where touched)
+struct FillScratch {
+  std::vector<int64_t> cur;
+  std::vector<int32_t> touched;
+  std::vector<RNode> nodes;
+  std::vector<int64_t> memo;
+  bool force_general = false;  // test hook: emit every restriction as kFillGeneral
+};
+
+// page-locked growable array, so uploads are true async DMA (falls back to
+// ordinary memory where no device is present, e.g. for the host test hooks)
+template <class T>
+struct PinnedVec {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  bool pinned = false;
+  PinnedVec() = default;
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  PinnedVec(PinnedVec&& o) noexcept : p(o.p), n(o.n), cap(o.cap), pinned(o.pinned) { o.p = nullptr; o.n = o.cap = 0; }
+  PinnedVec& operator=(PinnedVec&& o) noexcept {
+    if (this != &o) {
+      release(p, pinned);
+      p = o.p; n = o.n; cap = o.cap; pinned = o.pinned;
+      o.p = nullptr; o.n = o.cap = 0;
+    }
+    return *this;
+  }
+  ~PinnedVec() { release(p, pinned); }
+  static void release(T* q, bool pin) {
+    if (!q) return;
+    if (pin) hipHostFree(q);
+    else std::free(q);
+  }
+  void clear() { n = 0; }
+  void reserve(size_t m) {
+    if (m <= cap) return;
+    size_t nc = std::max(m, cap * 2 + 1024);
+    T* q = nullptr;
+    bool pin = hipHostMalloc((void**)&q, nc * sizeof(T)) == hipSuccess;
+    if (!pin) {
+      q = (T*)std::malloc(nc * sizeof(T));
+      if (!q) throw std::bad_alloc();
+    }
+    if (n) std::memcpy(q, p, n * sizeof(T));
+    release(p, pinned);
+    p = q;
+    pinned = pin;
+    cap = nc;
+  }
+  void push_back(const T& v) {
+    if (n == cap) reserve(n + 1);
+    p[n++] = v;
+  }
+  size_t size() const { return n; }
+  T* data() { return p; }
+};
+
 // ------------------------------------------------------------- instance
 struct Instance {
   amr_sim_config cfg;
@@ -70,7 +138,17 @@ struct Instance {
   std::vector<std::array<int32_t, 3>> sched;
   double t = 0, dt_main = 0, dt_dom = 0;
   int count = 0, status = 0;
+  amr_diag err{};
   bool done = false;
+  // ghost-fill program of this instance (global slot ids); recompiled only
+  // when its tree changed, kept in its own device segment
+  PinnedVec<FillOp> ops;
+  std::vector<FillGen> gen;
+  bool prog_dirty = true;
+  FillOp* d_ops = nullptr;
+  int cap_ops = 0;
+  FillGen* d_gen = nullptr;
+  int cap_gen = 0;
 };
 
 // --------------------------------------------------------------- context
@@ -80,8 +158,6 @@ struct Profile {
 };
 enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_COUNT };
 
-struct ProgState;  // fill compiler scratch + compiled program (defined below)
-void destroy_prog(ProgState* p);
 
 }  // namespace
 
@@ -98,24 +174,26 @@ struct amr_ctx {
   std::vector<int> leaf_inst_begin;  // per active instance, first leaf index
   // host copies kept alive until the stream has consumed them
   std::vector<RegridOp> h_refs, h_coars;
-  ProgState* prog = nullptr;
-  FillOp* d_ops = nullptr;
-  int nops = 0, cap_ops = 0;
-  FillGen* d_gen = nullptr;
-  int ngen = 0, cap_gen = 0;
-  FillGen* d_nodes = nullptr;
-  int nnodes = 0, cap_nodes = 0;
-  std::vector<int32_t> node_levels{0};
+  std::vector<FillScratch> scratch;  // one per compile thread
+  // fill program tables: per-instance segment pointers and op-count prefix
+  const FillOp** d_inst_ops = nullptr;
+  const FillGen** d_inst_gen = nullptr;
+  int32_t* d_prefix = nullptr;
+  int cap_tables = 0;
+  PinnedVec<const FillOp*> h_inst_ops;
+  PinnedVec<const FillGen*> h_inst_gen;
+  PinnedVec<int32_t> h_prefix;
+  int nops = 0;
   double* d_fillbuf = nullptr;
   size_t cap_fillbuf = 0;
-  double* d_nodebuf = nullptr;
-  size_t cap_nodebuf = 0;
   InstParams* d_ip = nullptr;
   std::vector<InstParams> h_ip;
   double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
   int cap_red = 0;
-  DevError* d_err = nullptr;
+  DevError* d_err = nullptr;  // one record per instance
   DevError* h_err = nullptr;
+  int call_fail = 0;          // code of the first instance that failed in the current call
+  int fail_nb = -1;           // block of a host-detected regrid failure
   RegridOp* d_rops = nullptr;
   int cap_rops = 0;
   int32_t *d_cmapU = nullptr, *d_cmapB = nullptr;
@@ -145,6 +223,25 @@ namespace {
 int set_error(amr_ctx* c, int code, int inst, int nb, int i, int j) {
   if (c->diag.code == 0) c->diag = amr_diag{code, inst, nb, i, j, 0};
   return code;
+}
+
+// An instance that hits one of the reference's throw sites stops where the
+// reference would abort; the other instances of the context keep running.
+void fail_instance(amr_ctx* c, int q, int code, int nb, int i, int j) {
+  Instance& I = c->inst[q];
+  if (I.status) return;
+  I.status = code;
+  I.err = amr_diag{code, q, nb, i, j, 0};
+  I.done = true;
+  c->topo_dirty = true;
+  set_error(c, code, q, nb, i, j);
+  if (!c->call_fail) c->call_fail = code;
+}
+
+int running(const amr_ctx* c) {
+  int n = 0;
+  for (auto& I : c->inst) n += I.done ? 0 : 1;
+  return n;
 }
 
 template <class T>
@@ -380,61 +477,6 @@ void build_schedule(Instance& I) {
 // program serves all fields: only the restriction arithmetic depends on the
 // field kind.
 
-// restriction node: `simple` when its four operand cells still hold their
-// pre-fill values (the common case), else explicit operand references
-struct RNode {
-  int32_t gs;
-  uint16_t cell;
-  bool simple;
-  int64_t ref[4];
-};
-
-// host scratch reused across compiles (cur[] is reset only where touched)
-struct FillScratch {
-  std::vector<int64_t> cur;
-  std::vector<int32_t> touched;
-  std::vector<RNode> nodes;
-  std::vector<int64_t> memo;
-  bool force_general = false;  // test hook: emit every restriction as kFillGeneral
-};
-
-// page-locked growable array, so uploads are true async DMA (falls back to
-// ordinary memory where no device is present, e.g. for the host test hooks)
-template <class T>
-struct PinnedVec {
-  T* p = nullptr;
-  size_t n = 0, cap = 0;
-  bool pinned = false;
-  ~PinnedVec() { release(p, pinned); }
-  static void release(T* q, bool pin) {
-    if (!q) return;
-    if (pin) hipHostFree(q);
-    else std::free(q);
-  }
-  void clear() { n = 0; }
-  void reserve(size_t m) {
-    if (m <= cap) return;
-    size_t nc = std::max(m, cap * 2 + 1024);
-    T* q = nullptr;
-    bool pin = hipHostMalloc((void**)&q, nc * sizeof(T)) == hipSuccess;
-    if (!pin) {
-      q = (T*)std::malloc(nc * sizeof(T));
-      if (!q) throw std::bad_alloc();
-    }
-    if (n) std::memcpy(q, p, n * sizeof(T));
-    release(p, pinned);
-    p = q;
-    pinned = pin;
-    cap = nc;
-  }
-  void push_back(const T& v) {
-    if (n == cap) reserve(n + 1);
-    p[n++] = v;
-  }
-  size_t size() const { return n; }
-  T* data() { return p; }
-};
-
 struct FillCompiler {
   const Instance& I;
   int NT, ng, N2;
@@ -669,114 +711,54 @@ struct FillCompiler {
   }
 };
 
-// Device program of all running instances: 16-byte destination ops, the
-// explicit-operand table for non-simple restrictions and nested nodes (only
-// when a restriction reads an element written earlier in the same fill),
-// grouped by evaluation level.
-struct Program {
-  PinnedVec<FillOp> ops;
-  std::vector<FillGen> gen;
-  std::vector<std::vector<FillGen>> nested_lv;  // [level-1][i]
-  std::vector<FillGen> nested;
-  std::vector<int32_t> level_start{0};
-};
-
-constexpr int64_t kTmpLevelShift = 40;
-
-// value of a nested node as a temporary reference kNodeBit|level<<40|index
-int64_t materialize(Program& P, FillScratch& S, int NT, int64_t n) {
-  if (S.memo[n] >= 0) return S.memo[n];
-  const RNode& nd = S.nodes[n];
-  FillGen g{};
-  int lvl = 1;
+// Program of one instance: 16-byte destination ops plus the explicit-operand
+// table of restrictions that read an element already written in the same
+// fill.  Nested restrictions (a restriction reading another restriction's
+// destination) cannot occur for nghosts 2 and 4 -- the fine elements a
+// restriction reads are interior cells or, for nghosts 4, ghost cells that
+// only the same entry's later injection writes -- and are rejected.
+int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
+  I.ops.clear();
+  I.gen.clear();
+  const int N2 = NT * NT;
   const int off[4] = {0, NT, 1, NT + 1};
-  for (int q = 0; q < 4; q++) {
-    int64_t r = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
-    if (r & kNodeBit) {
-      r = materialize(P, S, NT, r & ~kNodeBit);
-      lvl = std::max(lvl, (int)((r & ~kNodeBit) >> kTmpLevelShift) + 1);
-    }
-    g.ref[q] = r;
-  }
-  if ((int)P.nested_lv.size() < lvl) P.nested_lv.resize(lvl);
-  P.nested_lv[lvl - 1].push_back(g);
-  const int64_t t = kNodeBit | ((int64_t)lvl << kTmpLevelShift) | (int64_t)(P.nested_lv[lvl - 1].size() - 1);
-  S.memo[n] = t;
-  return t;
-}
-
-void compile_programs(amr_ctx* c, Program& P, FillScratch& S) {
-  P.ops.clear();
-  P.gen.clear();
-  P.nested_lv.clear();
-  const int NT = c->NT, N2 = NT * NT;
-  const int off[4] = {0, NT, 1, NT + 1};
-  for (auto& I : c->inst) {
-    if (I.done) continue;
-    FillCompiler fc(I, NT, c->ng, S);
-    fc.run();
-    S.memo.assign(S.nodes.size(), -1);
-    P.ops.reserve(P.ops.size() + S.touched.size());
-    for (int32_t a : S.touched) {
-      const int nb = a / N2, cell = a % N2;
-      const int64_t v = S.cur[a];
-      FillOp op{};
-      op.dst_gs = I.gs[nb];
-      op.dst_cell = (uint16_t)cell;
-      if (!(v & kNodeBit)) {
-        if (v == src_ref(I.gs[nb], cell)) continue;  // value unchanged
-        op.op = kFillCopy;
-        op.src = (int32_t)(v >> 16);
-        op.src_cell = (uint16_t)(v & 0xffff);
+  FillCompiler fc(I, NT, ng, S);
+  fc.run();
+  int status = 0;
+  I.ops.reserve(S.touched.size());
+  for (int32_t a : S.touched) {
+    const int nb = a / N2, cell = a % N2;
+    const int64_t v = S.cur[a];
+    FillOp op{};
+    op.dst_gs = I.gs[nb];
+    op.dst_cell = (uint16_t)cell;
+    if (!(v & kNodeBit)) {
+      if (v == src_ref(I.gs[nb], cell)) continue;  // value unchanged
+      op.op = kFillCopy;
+      op.src = (int32_t)(v >> 16);
+      op.src_cell = (uint16_t)(v & 0xffff);
+    } else {
+      const RNode& nd = S.nodes[v & ~kNodeBit];
+      if (nd.simple && !S.force_general) {
+        op.op = kFillRestrict;
+        op.src = nd.gs;
+        op.src_cell = nd.cell;
       } else {
-        const RNode& nd = S.nodes[v & ~kNodeBit];
-        if (nd.simple && !S.force_general) {
-          op.op = kFillRestrict;
-          op.src = nd.gs;
-          op.src_cell = nd.cell;
-        } else {
-          FillGen g{};
-          for (int q = 0; q < 4; q++) {
-            int64_t r = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
-            if (r & kNodeBit) r = materialize(P, S, NT, r & ~kNodeBit);
-            g.ref[q] = r;
-          }
-          op.op = kFillGeneral;
-          op.src = (int32_t)P.gen.size();
-          P.gen.push_back(g);
+        FillGen g{};
+        for (int q = 0; q < 4; q++) {
+          g.ref[q] = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
+          if (g.ref[q] & kNodeBit) status = AMR_ERR_CONFIG;
         }
+        op.op = kFillGeneral;
+        op.src = (int32_t)I.gen.size();
+        I.gen.push_back(g);
       }
-      P.ops.push_back(op);
     }
-    fc.finish();
+    I.ops.push_back(op);
   }
-  // final node numbering: levels in order
-  std::vector<int64_t> lv_off(P.nested_lv.size() + 1, 0);
-  for (size_t l = 0; l < P.nested_lv.size(); l++) lv_off[l + 1] = lv_off[l] + (int64_t)P.nested_lv[l].size();
-  auto remap = [&](int64_t r) {
-    if (!(r & kNodeBit)) return r;
-    const int64_t t = r & ~kNodeBit;
-    const int64_t lvl = t >> kTmpLevelShift, idx = t & ((int64_t(1) << kTmpLevelShift) - 1);
-    return kNodeBit | (lv_off[lvl - 1] + idx);
-  };
-  P.nested.clear();
-  P.level_start.assign(1, 0);
-  for (auto& lvv : P.nested_lv) {
-    for (auto g : lvv) {
-      for (auto& r : g.ref) r = remap(r);
-      P.nested.push_back(g);
-    }
-    P.level_start.push_back((int32_t)P.nested.size());
-  }
-  for (auto& g : P.gen)
-    for (auto& r : g.ref) r = remap(r);
+  fc.finish();
+  return status;
 }
-
-struct ProgState {
-  FillScratch scratch;
-  Program prog;
-};
-void destroy_prog(ProgState* p) { delete p; }
 
 // ------------------------------------------------- tree bookkeeping
 // updatelpup.cpp:4-118: link four new sons of nb into the neighbour table
@@ -878,7 +860,7 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
         s[1] = s[0] + 1; s[2] = s[0] + 2; s[3] = s[0] + 3;
         lp[nb][2] = s[0];
         I.lastActive += 4;
-        if (I.lastActive - 1 >= I.maxblocks) return set_error(c, AMR_ERR_OVERFLOW, ii, nb, 0, 0);
+        if (I.lastActive - 1 >= I.maxblocks) { c->fail_nb = nb; return AMR_ERR_OVERFLOW; }
         grow_tree(I, I.lastActive);
       } else {
         s[0] = lp[nb][2]; s[1] = lp[s[0]][5]; s[2] = lp[s[0]][7]; s[3] = lp[s[2]][5];
@@ -887,7 +869,7 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
       op.parent_gs = slot_of(c, I, nb);
       for (int q = 0; q < 4; q++) {
         op.son_gs[q] = slot_of(c, I, s[q]);
-        if (op.son_gs[q] < 0) return set_error(c, AMR_ERR_CAPACITY, ii, s[q], 0, 0);
+        if (op.son_gs[q] < 0) { c->fail_nb = s[q]; return AMR_ERR_CAPACITY; }
       }
       op.inst = ii;
       op.nb = nb;
@@ -1077,15 +1059,17 @@ int upload_params(amr_ctx* c) {
 // leaf work list and fill program of the running instances
 int rebuild_topology(amr_ctx* c) {
   HostTimed ht(c, PF_HOST_TOPO);
+  HIPCHK(hipStreamSynchronize(c->stream));  // previous uploads consumed
   c->h_leaves.clear();
   c->leaf_inst_begin.assign(c->inst.size() + 1, 0);
+  std::vector<int> dirty;
   for (size_t q = 0; q < c->inst.size(); q++) {
     Instance& I = c->inst[q];
     c->leaf_inst_begin[q] = (int)c->h_leaves.size();
     if (I.done) continue;
     for (int nb = 0; nb < I.lastActive; nb++)
       if (I.leafs[nb]) c->h_leaves.push_back(LeafItem{I.gs[nb], (int32_t)q, nb, I.lp[nb][0]});
-    build_schedule(I);
+    if (I.prog_dirty) dirty.push_back((int)q);
   }
   c->leaf_inst_begin[c->inst.size()] = (int)c->h_leaves.size();
   c->nleaf = (int)c->h_leaves.size();
@@ -1093,26 +1077,71 @@ int rebuild_topology(amr_ctx* c) {
   if (c->nleaf)
     HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
                           c->stream));
-  if (!c->prog) c->prog = new ProgState();
-  HIPCHK(hipStreamSynchronize(c->stream));  // the previous program upload has been consumed
-  Program& P = c->prog->prog;
-  compile_programs(c, P, c->prog->scratch);
-  c->nops = (int)P.ops.size();
-  c->ngen = (int)P.gen.size();
-  c->nnodes = (int)P.nested.size();
-  c->node_levels = P.level_start;
-  if (int e = ensure(c->d_ops, c->cap_ops, std::max(1, c->nops))) return e;
-  if (int e = ensure(c->d_gen, c->cap_gen, std::max(1, c->ngen))) return e;
-  if (int e = ensure(c->d_nodes, c->cap_nodes, std::max(1, c->nnodes))) return e;
-  if (c->nops)
-    HIPCHK(hipMemcpyAsync(c->d_ops, P.ops.data(), sizeof(FillOp) * c->nops, hipMemcpyHostToDevice, c->stream));
-  if (c->ngen)
-    HIPCHK(hipMemcpyAsync(c->d_gen, P.gen.data(), sizeof(FillGen) * c->ngen, hipMemcpyHostToDevice, c->stream));
-  if (c->nnodes)
-    HIPCHK(hipMemcpyAsync(c->d_nodes, P.nested.data(), sizeof(FillGen) * c->nnodes, hipMemcpyHostToDevice,
-                          c->stream));
+  // recompile the programs of instances whose tree changed, in parallel
+  int nthreads = (int)std::min<size_t>(dirty.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  if (nthreads < 1) nthreads = 1;
+  if ((int)c->scratch.size() < nthreads) c->scratch.resize(nthreads);
+  std::vector<int> status(dirty.size(), 0);
+  auto work = [&](int t) {
+    for (size_t q = t; q < dirty.size(); q += nthreads) {
+      Instance& I = c->inst[dirty[q]];
+      build_schedule(I);
+      status[q] = compile_instance(I, c->NT, c->ng, c->scratch[t]);
+    }
+  };
+  if (nthreads == 1 || dirty.size() < 4) {
+    work(0);
+    if (nthreads > 1)
+      for (int t = 1; t < nthreads; t++) work(t);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthreads; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+  }
+  bool failed = false;
+  for (size_t q = 0; q < dirty.size(); q++)
+    if (status[q]) {
+      fail_instance(c, dirty[q], status[q], -1, 0, 0);
+      failed = true;
+    }
+  if (failed) return rebuild_topology(c);  // drop the failed instances' leaves
+  for (int qi : dirty) {
+    Instance& I = c->inst[qi];
+    const int n = (int)I.ops.size(), ng = (int)I.gen.size();
+    if (int e = ensure(I.d_ops, I.cap_ops, std::max(1, n))) return e;
+    if (int e = ensure(I.d_gen, I.cap_gen, std::max(1, ng))) return e;
+    if (n) HIPCHK(hipMemcpyAsync(I.d_ops, I.ops.data(), sizeof(FillOp) * n, hipMemcpyHostToDevice, c->stream));
+    if (ng) HIPCHK(hipMemcpyAsync(I.d_gen, I.gen.data(), sizeof(FillGen) * ng, hipMemcpyHostToDevice, c->stream));
+    I.prog_dirty = false;
+  }
+  // segment tables over the running instances
+  const int ni = (int)c->inst.size();
+  c->h_inst_ops.clear();
+  c->h_inst_gen.clear();
+  c->h_prefix.clear();
+  int total = 0;
+  c->h_prefix.push_back(0);
+  for (auto& I : c->inst) {
+    c->h_inst_ops.push_back(I.d_ops);
+    c->h_inst_gen.push_back(I.d_gen);
+    total += I.done ? 0 : (int)I.ops.size();
+    c->h_prefix.push_back(total);
+  }
+  c->nops = total;
+  if (c->cap_tables < ni + 1) {
+    hipFree(c->d_inst_ops);
+    hipFree(c->d_inst_gen);
+    hipFree(c->d_prefix);
+    HIPCHK(hipMalloc((void**)&c->d_inst_ops, sizeof(FillOp*) * (ni + 1)));
+    HIPCHK(hipMalloc((void**)&c->d_inst_gen, sizeof(FillGen*) * (ni + 1)));
+    HIPCHK(hipMalloc((void**)&c->d_prefix, sizeof(int32_t) * (ni + 1)));
+    c->cap_tables = ni + 1;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_inst_ops, c->h_inst_ops.data(), sizeof(FillOp*) * ni, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_inst_gen, c->h_inst_gen.data(), sizeof(FillGen*) * ni, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_prefix, c->h_prefix.data(), sizeof(int32_t) * (ni + 1), hipMemcpyHostToDevice, c->stream));
   if (int e = ensure_bytes(c->d_fillbuf, c->cap_fillbuf, (size_t)4 * 8 * std::max(1, c->nops))) return e;
-  if (int e = ensure_bytes(c->d_nodebuf, c->cap_nodebuf, (size_t)4 * 8 * std::max(1, c->nnodes))) return e;
   const int old_cap = c->cap_red;
   if (int e = ensure(c->d_lam, c->cap_red, std::max(1, c->nleaf))) return e;
   if (c->cap_red != old_cap || !c->d_grad) {
@@ -1141,8 +1170,8 @@ FillSet make_set(amr_ctx* c, std::initializer_list<std::pair<int, int>> fk) {
 
 void run_fill(amr_ctx* c, const FillSet& s) {
   Timed tm(c, PF_FILL);
-  launch_fill(c->NT, c->d_ops, c->nops, c->d_gen, c->d_nodes, c->node_levels.data(),
-              (int)c->node_levels.size() - 1, c->nnodes, s, c->d_fillbuf, c->d_nodebuf, c->stream);
+  launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->nops, s, c->d_fillbuf,
+              c->stream);
 }
 
 void run_stage(amr_ctx* c, int step) {
@@ -1168,17 +1197,24 @@ void run_ct(amr_ctx* c, int step) {
   }
 }
 
+// waits for the stream and retires the instances whose kernels reported an
+// error; returns nonzero only for a HIP failure
 int sync_errors(amr_ctx* c) {
-  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError), hipMemcpyDeviceToHost, c->stream));
+  const int ni = (int)c->inst.size();
+  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * ni, hipMemcpyDeviceToHost, c->stream));
   {
     HostTimed ht(c, PF_HOST_WAIT);
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   flush_profile(c);
-  if (c->h_err->code) {
-    const DevError& e = *c->h_err;
-    return set_error(c, e.code, e.inst, e.nb, e.i, e.j);
+  bool any = false;
+  for (int q = 0; q < ni; q++) {
+    const DevError& e = c->h_err[q];
+    if (!e.code) continue;
+    any = true;
+    fail_instance(c, q, e.code, e.nb, e.i, e.j);
   }
+  if (any) HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(DevError) * ni, c->stream));
   return 0;
 }
 
@@ -1244,8 +1280,15 @@ int regrid_all(amr_ctx* c) {
     for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) grad[c->h_leaves[l].nb] = c->h_grad[l];
     const int before = I.lastActive;
     const size_t nr = refs.size(), nc = coars.size();
-    if (int e = regrid_decide(c, (int)q, grad, refs, coars)) return e;
-    if (refs.size() != nr || coars.size() != nc || I.lastActive != before) c->topo_dirty = true;
+    if (int e = regrid_decide(c, (int)q, grad, refs, coars)) {
+      if (e == AMR_ERR_HIP) return e;
+      fail_instance(c, (int)q, e, c->fail_nb, 0, 0);
+      continue;
+    }
+    if (refs.size() != nr || coars.size() != nc || I.lastActive != before) {
+      c->topo_dirty = true;
+      I.prog_dirty = true;
+    }
   }
   if (int e = apply_regrid_ops(c, refs, true)) return e;
   if (int e = apply_regrid_ops(c, coars, false)) return e;
@@ -1253,8 +1296,15 @@ int regrid_all(amr_ctx* c) {
 }
 
 // main.cpp:128-247 for all running instances
+// Returns nonzero for a HIP failure, or when no instance is left running
+// and one failed during this call (its code); otherwise instance failures
+// are reported through amr_info().status.
 int step_all(amr_ctx* c, int* n_active) {
-  if (c->diag.code) return c->diag.code;
+  c->call_fail = 0;
+  if (!running(c)) {
+    if (n_active) *n_active = 0;
+    return c->diag.code;
+  }
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
@@ -1278,9 +1328,10 @@ int step_all(amr_ctx* c, int* n_active) {
     I.dt_dom = I.dt_main;
     I.count = I.count + 1;
     I.dt_main = I.cfl * lambda;
-    if (I.dt_main < 0 || std::isnan(I.dt_main)) return set_error(c, AMR_ERR_DT, (int)q, -1, 0, 0);
+    if (I.dt_main < 0 || std::isnan(I.dt_main)) fail_instance(c, (int)q, AMR_ERR_DT, -1, 0, 0);
   }
   if (int e = regrid_all(c)) return e;
+  if (int e = sync_errors(c)) return e;  // refine/coarsen errors
   for (auto& I : c->inst) {
     if (!I.done && !(I.t <= I.tEnd)) {
       I.done = true;
@@ -1289,7 +1340,13 @@ int step_all(amr_ctx* c, int* n_active) {
     active += I.done ? 0 : 1;
   }
   if (n_active) *n_active = active;
-  return 0;
+  return (active == 0 && c->call_fail) ? c->call_fail : 0;
+}
+
+// end of a phase-level call: the first instance failure of the call, if any
+int finish_call(amr_ctx* c) {
+  if (int e = sync_errors(c)) return e;
+  return c->call_fail;
 }
 
 int check_inst(amr_ctx* c, int inst) { return (c && inst >= 0 && inst < (int)c->inst.size()) ? 0 : AMR_ERR_CONFIG; }
@@ -1348,9 +1405,9 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
-      hipMalloc((void**)&c->d_err, sizeof(DevError)) != hipSuccess ||
-      hipMemset(c->d_err, 0, sizeof(DevError)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_err, sizeof(DevError)) != hipSuccess ||
+      hipMalloc((void**)&c->d_err, sizeof(DevError) * n_inst) != hipSuccess ||
+      hipMemset(c->d_err, 0, sizeof(DevError) * n_inst) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_err, sizeof(DevError) * n_inst) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     amr_destroy(c);
     return AMR_ERR_HIP;
@@ -1373,9 +1430,12 @@ int amr_destroy(amr_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < kNumFields; f++)
     if (c->F.f[f]) hipFree(c->F.f[f]);
-  hipFree(c->d_leaves); hipFree(c->d_ops); hipFree(c->d_gen); hipFree(c->d_nodes); hipFree(c->d_fillbuf);
-  hipFree(c->d_nodebuf);
-  destroy_prog(c->prog);
+  hipFree(c->d_leaves); hipFree(c->d_fillbuf);
+  hipFree(c->d_inst_ops); hipFree(c->d_inst_gen); hipFree(c->d_prefix);
+  for (auto& I : c->inst) {
+    hipFree(I.d_ops);
+    hipFree(I.d_gen);
+  }
   hipFree(c->d_ip); hipFree(c->d_lam); hipFree(c->d_grad); hipFree(c->d_err); hipFree(c->d_rops);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
   if (c->h_lam) hipHostFree(c->h_lam);
@@ -1395,6 +1455,7 @@ int amr_destroy(amr_ctx* c) {
 // basegrid.cpp:4-89 + IC2Dtype.cpp + main.cpp:97-110
 int amr_init(amr_ctx* c) {
   if (!c) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   static const int32_t root_lp[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
                                         {0, 0, 0, 0, -1, 3, 0, -1}, {0, 0, 0, 0, 2, -1, 1, -1}};
   const int NT = c->NT, N2 = NT * NT, side = NT - 2 * c->ng;
@@ -1432,7 +1493,8 @@ int amr_init(amr_ctx* c) {
   c->d_W = nullptr;
   if (int e = regrid_all(c)) return e;
   c->topo_dirty = true;
-  return sync_errors(c);
+  if (int e = sync_errors(c)) return e;
+  return running(c) ? AMR_OK : c->call_fail;
 }
 
 int amr_step(amr_ctx* c, int* n_active) {
@@ -1444,19 +1506,24 @@ int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
   if (!c) return AMR_ERR_CONFIG;
   int64_t n = 0, zu = 0;
   const int64_t cells = (int64_t)c->nxi * c->nxi;
-  int active = 0;
-  for (auto& I : c->inst) active += I.done ? 0 : 1;
+  int active = running(c);
+  std::vector<int64_t> leaves(c->inst.size());
   while (active > 0 && (max_steps < 0 || n < max_steps)) {
-    int64_t leaves = 0;
-    for (auto& I : c->inst)
+    for (size_t q = 0; q < c->inst.size(); q++) {
+      const Instance& I = c->inst[q];
+      leaves[q] = 0;
       if (!I.done)
-        for (int nb = 0; nb < I.lastActive; nb++) leaves += I.leafs[nb];
-    if (int e = step_all(c, &active)) {
+        for (int nb = 0; nb < I.lastActive; nb++) leaves[q] += I.leafs[nb];
+    }
+    const int e = step_all(c, &active);
+    // a step counts for the instances that completed it
+    for (size_t q = 0; q < c->inst.size(); q++)
+      if (!c->inst[q].status) zu += leaves[q] * cells;
+    if (e) {
       if (steps) *steps = n;
       if (zone_updates) *zone_updates = zu;
       return e;
     }
-    zu += leaves * cells;
     n++;
   }
   if (steps) *steps = n;
@@ -1530,7 +1597,8 @@ int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* 
     if (slot_of(c, I, nb) < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
   }
   I.lastActive = n;
-  I.done = false;
+  I.done = I.status != 0;
+  I.prog_dirty = true;
   c->topo_dirty = true;
   return AMR_OK;
 }
@@ -1547,36 +1615,41 @@ int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int3
 
 int amr_fill(amr_ctx* c, int field, int kind) {
   if (!c || field < 0 || field >= kNumFields || kind < 0 || kind > 2) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   run_fill(c, make_set(c, {{field, kind}}));
-  return sync_errors(c);
+  return finish_call(c);
 }
 
 int amr_stage(amr_ctx* c, int step) {
   if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
   run_stage(c, step);
-  return sync_errors(c);
+  return finish_call(c);
 }
 
 int amr_ct(amr_ctx* c, int step) {
   if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
   run_ct(c, step);
-  return sync_errors(c);
+  return finish_call(c);
 }
 
 int amr_cfl(amr_ctx* c, int inst, double* lambda) {
   if (check_inst(c, inst) || !lambda) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
   if (int e = cfl_and_indicators(c, nullptr)) return e;
+  if (c->call_fail) return c->call_fail;
   double lam = 1.E6;
   for (int l = c->leaf_inst_begin[inst]; l < c->leaf_inst_begin[inst + 1]; l++) lam = std_min(lam, c->h_lam[l]);
   *lambda = lam;
@@ -1585,12 +1658,13 @@ int amr_cfl(amr_ctx* c, int inst, double* lambda) {
 
 int amr_regrid(amr_ctx* c) {
   if (!c) return AMR_ERR_CONFIG;
+  c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
   if (int e = cfl_and_indicators(c, nullptr)) return e;
   if (int e = regrid_all(c)) return e;
-  return sync_errors(c);
+  return finish_call(c);
 }
 
 int amr_schedule(amr_ctx* c, int inst, int cap, int32_t* ib3, int* n) {
@@ -1651,13 +1725,9 @@ namespace {
 struct HostFill {
   const double* Q;
   int nv, NT, N2, kind;
-  const std::vector<double>& nodeval;
   const std::vector<FillGen>& gen;
   double elem(int64_t gs, int cell, int k) const { return Q[((size_t)gs * nv + k) * N2 + cell]; }
-  double ref(int64_t r, int k) const {
-    if (r & kNodeBit) return nodeval[(r & ~kNodeBit) * 8 + k];
-    return elem(r >> 16, (int)(r & 0xffff), k);
-  }
+  double ref(int64_t r, int k) const { return elem(r >> 16, (int)(r & 0xffff), k); }
   double combine(int k, double a, double b, double c, double d) const {
     double v = 0.;
     if (kind == kFillCell) { v += a; v += b; v += c; v += d; return 0.25 * v; }
@@ -1700,23 +1770,19 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
     I.gs.push_back(nb);
   }
   build_schedule(I);
-  ProgState ps;
-  ps.scratch.force_general = force_general != 0;
-  compile_programs(&c, ps.prog, ps.scratch);
-  Program& P = ps.prog;
+  FillScratch S;
+  S.force_general = force_general != 0;
+  if (compile_instance(I, NT, ng, S)) return -1;
   const int N2 = NT * NT;
-  std::vector<double> nodeval(P.nested.size() * 8, 0.0), buf(P.ops.size() * 8, 0.0);
-  HostFill hf{Q, nv, NT, N2, kind, nodeval, P.gen};
-  for (size_t l = 0; l + 1 < P.level_start.size(); l++)
-    for (int e = P.level_start[l]; e < P.level_start[l + 1]; e++)
-      for (int k = 0; k < nv; k++) nodeval[e * 8 + k] = hf.general(P.nested[e], k);
-  FillOp* ops = P.ops.data();
-  for (size_t e = 0; e < P.ops.size(); e++)
+  std::vector<double> buf(I.ops.size() * 8, 0.0);
+  HostFill hf{Q, nv, NT, N2, kind, I.gen};
+  FillOp* ops = I.ops.data();
+  for (size_t e = 0; e < I.ops.size(); e++)
     for (int k = 0; k < nv; k++) buf[e * 8 + k] = hf.value(ops[e], k);
-  for (size_t e = 0; e < P.ops.size(); e++)
+  for (size_t e = 0; e < I.ops.size(); e++)
     for (int k = 0; k < nv; k++) Q[((size_t)ops[e].dst_gs * nv + k) * N2 + ops[e].dst_cell] = buf[e * 8 + k];
-  if (nnodes) *nnodes = (int)P.nested.size();
-  return (int)P.ops.size();
+  if (nnodes) *nnodes = 0;
+  return (int)I.ops.size();
 }
 
 // coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]

@@ -17,12 +17,15 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// per-instance error records: the first error of an instance wins, so a
+// failing instance never hides or corrupts another instance's record
 __device__ inline void report(DevError* err, int code, int inst, int nb, int i, int j) {
-  if (atomicCAS(&err->code, 0, code) == 0) {
-    err->inst = inst;
-    err->nb = nb;
-    err->i = i;
-    err->j = j;
+  DevError* e = err + inst;
+  if (atomicCAS(&e->code, 0, code) == 0) {
+    e->inst = inst;
+    e->nb = nb;
+    e->i = i;
+    e->j = j;
   }
 }
 
@@ -245,7 +248,6 @@ __global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const Leaf
 struct FillCtx {
   const double* base;
   int nv, kind, N2, NT;
-  const double* nodes;  // node values of this field, [node][8]
   const FillGen* gen;
 };
 
@@ -253,8 +255,8 @@ __device__ inline double fill_elem(const FillCtx& f, int64_t gs, int cell, int k
   return f.base[((size_t)gs * f.nv + k) * f.N2 + cell];
 }
 
+// explicit operand of a general restriction (always a pre-fill element)
 __device__ inline double fill_ref(const FillCtx& f, int64_t ref, int k) {
-  if (ref & kNodeBit) return f.nodes[(ref & ~kNodeBit) * 8 + k];
   return fill_elem(f, ref >> 16, (int)(ref & 0xffff), k);
 }
 
@@ -293,38 +295,45 @@ __device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
   return gen_value(f, f.gen[op.src], k);
 }
 
-__global__ __launch_bounds__(kThreads) void fill_node_kernel(const FillGen* __restrict__ nodes, int begin, int end,
-                                                             int nnodes, FillSet set, double* nodebuf, int NT) {
-  const int e = begin + blockIdx.x * kThreads + threadIdx.x;
-  if (e >= end) return;
-  const int fi = blockIdx.y;
-  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, nodebuf + (size_t)fi * nnodes * 8, nullptr};
-  const FillGen g = nodes[e];
-  for (int k = 0; k < f.nv; k++) nodebuf[((size_t)fi * nnodes + e) * 8 + k] = gen_value(f, g, k);
+// The program is a concatenation of per-instance segments: op e belongs to
+// the instance q with prefix[q] <= e < prefix[q+1].
+struct FillProgram {
+  const FillOp* const* ops;
+  const FillGen* const* gen;
+  const int32_t* prefix;
+  int ninst, nops;
+};
+
+__device__ inline int find_segment(const FillProgram& P, int e) {
+  int lo = 0, hi = P.ninst;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (P.prefix[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void fill_gather_kernel(const FillOp* __restrict__ ops, int nops,
-                                                               const FillGen* __restrict__ gen, FillSet set,
-                                                               double* buf, const double* nodebuf, int nnodes,
-                                                               int NT) {
+__global__ __launch_bounds__(kThreads) void fill_gather_kernel(FillProgram P, FillSet set, double* buf, int NT) {
   const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= nops) return;
+  if (e >= P.nops) return;
+  const int q = find_segment(P, e);
   const int fi = blockIdx.y;
-  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, nodebuf + (size_t)fi * nnodes * 8, gen};
-  const FillOp op = ops[e];
-  for (int k = 0; k < f.nv; k++) buf[((size_t)fi * nops + e) * 8 + k] = fill_value(f, op, k);
+  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, P.gen[q]};
+  const FillOp op = P.ops[q][e - P.prefix[q]];
+  for (int k = 0; k < f.nv; k++) buf[((size_t)fi * P.nops + e) * 8 + k] = fill_value(f, op, k);
 }
 
-__global__ __launch_bounds__(kThreads) void fill_scatter_kernel(const FillOp* __restrict__ ops, int nops,
-                                                                FillSet set, const double* buf, int N2) {
+__global__ __launch_bounds__(kThreads) void fill_scatter_kernel(FillProgram P, FillSet set, const double* buf, int N2) {
   const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= nops) return;
+  if (e >= P.nops) return;
+  const int q = find_segment(P, e);
   const int fi = blockIdx.y;
-  const FillOp op = ops[e];
+  const FillOp op = P.ops[q][e - P.prefix[q]];
   const int nv = set.nv[fi];
   double* base = set.base[fi];
   for (int k = 0; k < nv; k++)
-    base[((size_t)op.dst_gs * nv + k) * N2 + op.dst_cell] = buf[((size_t)fi * nops + e) * 8 + k];
+    base[((size_t)op.dst_gs * nv + k) * N2 + op.dst_cell] = buf[((size_t)fi * P.nops + e) * 8 + k];
 }
 
 // ------------------------------------------------------ CFL and criteria
@@ -628,19 +637,13 @@ void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
   AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, step)));
 }
 
-void launch_fill(int NT, const FillOp* ops, int nops, const FillGen* gen, const FillGen* nodes,
-                 const int32_t* node_level_start, int nlevels, int nnodes, const FillSet& set, double* buf,
-                 double* nodebuf, hipStream_t s) {
+void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* prefix,
+                 int ninst, int nops, const FillSet& set, double* buf, hipStream_t s) {
   if (nops <= 0 || set.n <= 0) return;
-  for (int l = 0; l < nlevels; l++) {
-    const int b = node_level_start[l], e = node_level_start[l + 1];
-    if (e <= b) continue;
-    dim3 g((e - b + kThreads - 1) / kThreads, set.n);
-    fill_node_kernel<<<g, kThreads, 0, s>>>(nodes, b, e, nnodes, set, nodebuf, NT);
-  }
+  FillProgram P{inst_ops, inst_gen, prefix, ninst, nops};
   dim3 grid((nops + kThreads - 1) / kThreads, set.n);
-  fill_gather_kernel<<<grid, kThreads, 0, s>>>(ops, nops, gen, set, buf, nodebuf, nnodes, NT);
-  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(ops, nops, set, buf, NT * NT);
+  fill_gather_kernel<<<grid, kThreads, 0, s>>>(P, set, buf, NT);
+  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(P, set, buf, NT * NT);
 }
 
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,

@@ -36,9 +36,11 @@ void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, h
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
                     hipStream_t s);
 void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int step, hipStream_t s);
-void launch_fill(int NT, const FillOp* ops, int nops, const FillGen* gen, const FillGen* nodes,
-                 const int32_t* node_level_start, int nlevels, int nnodes, const FillSet& set, double* buf,
-                 double* nodebuf, hipStream_t s);
+// ghost fill of up to four fields through a program made of per-instance
+// segments (device pointer tables inst_ops/inst_gen, prefix[ninst+1] of op
+// counts), gather from the pre-fill state into buf, then scatter
+void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* prefix,
+                 int ninst, int nops, const FillSet& set, double* buf, hipStream_t s);
 // per-leaf CFL bound min(dx,dy)/lambda and refinement indicator; when W is
 // non-null it supplies the primitives (initial regrid) instead of U.
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,

@@ -77,6 +77,63 @@ def cpu_baseline(steps, threads):
                        f"({zu} zone-updates in {tm['loop_seconds']:.2f} s), host CPU: {cpu_model()}")
 
 
+def ensemble_bench(steps, full, n_inst):
+    """Batched ensemble of seeded parameterised rotors against the same
+    instances run one after another, each alone in its own context (plain
+    launches).  Both legs time exactly `steps` steps of every instance from
+    its initial condition, so their zone-updates are identical; with `full`
+    the batch also runs every instance to t = 0.2."""
+    import torch
+
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import SEED, ensemble_configs
+
+    cfgs = ensemble_configs(n_inst)
+    batch = A.Simulation(cfgs)
+    batch.init()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nb, zb = batch.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    failed = [q for q in range(len(cfgs)) if batch.info(q)["status"]]
+    batch.close()
+    tp, zp = 0.0, 0
+    for q, cfg in enumerate(cfgs):
+        if q in failed:
+            continue  # an aborted instance's partial step is not a zone-update in either leg
+        sim = A.Simulation(cfg)
+        sim.init()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, z = sim.run(steps)
+        torch.cuda.synchronize()
+        tp += time.perf_counter() - t0
+        zp += z
+        sim.close()
+    out = {"instances": len(cfgs), "seed": SEED, "aborted_instances": failed,
+           "workload": "seeded rotor_p ensemble: rho_in U[5,20], v0 U[1,3], Bx U[0.5,2]/sqrt(4pi), p 5, "
+                       "max level U{3..6}, MC-MUSCL, HLLC (even) / HLLD (odd), CT, RK2, CFL 0.5",
+           "head_to_head_steps": steps,
+           "batched": {"zone_updates": zb, "seconds": tb, "value": zb / tb},
+           "plain_sequential": {"zone_updates": zp, "seconds": tp, "value": zp / tp if tp else None},
+           "speedup": (zb / tb) / (zp / tp) if tp and zp else None}
+    if full:
+        batch = A.Simulation(cfgs)
+        batch.init()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, z = batch.run(-1)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        ts = [batch.info(q)["t"] for q in range(len(cfgs))]
+        out["full_run"] = {"steps": n, "zone_updates": z, "seconds": tf, "value": z / tf,
+                           "aborted_instances": [q for q in range(len(cfgs)) if batch.info(q)["status"]],
+                           "t_min": min(ts), "t_max": max(ts)}
+        batch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +142,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="also time a batched ensemble of N seeded rotors against N plain runs")
+    ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ens-full", action="store_true", help="also run the batched ensemble to t = 0.2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -149,6 +210,8 @@ def main():
                      "zones_per_launch": zones_per_launch},
         "cpu_baseline": None,
     }
+    if args.ensemble:
+        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble)
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)

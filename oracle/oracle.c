@@ -31,7 +31,7 @@ static inline double stdmax(double a, double b) { return (a < b) ? b : a; }
 
 enum { L_1ST, L_NOSL, L_MINMOD, L_VANLEER, L_SUPERBEE, L_MC, L_KOREN, L_CADA, L_WENO3, L_WENO5, L_BAD };
 enum { F_RUSA, F_HLLE, F_HLLC, F_HLLD, F_BAD };
-enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_BAD };
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_BAD };
 
 static int lim_id(const char *s) {
   if (!strcmp(s, "1st")) return L_1ST;
@@ -54,8 +54,8 @@ static int flux_id(const char *s) {
   return F_BAD;
 }
 static int ic_id(const char *s) {
-  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor"};
-  for (int q = 0; q < 8; q++)
+  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p"};
+  for (int q = 0; q < 9; q++)
     if (!strcmp(s, names[q])) return q;
   return IC_BAD;
 }
@@ -79,6 +79,7 @@ struct ora_dom {
   int count;
   double *scr; /* dwdx dwdy wxL wxR wyL wyR, each [8][NT][NT] */
   int32_t err[4];
+  double icp[8];
 };
 
 #define TILE(d, fld, nb) ((d)->f[fld] + (size_t)(nb) * NVAR[fld] * (d)->NT * (d)->NT)
@@ -586,6 +587,7 @@ ora_dom *ora_create(const ora_config *c) {
   d->lim = lim_id(c->limiter);
   d->flux = flux_id(c->flux);
   d->ic = ic_id(c->ic);
+  memcpy(d->icp, c->ic_params, sizeof d->icp);
   if (d->lim == L_BAD || d->flux == F_BAD || d->ic == IC_BAD) { free(d); return NULL; }
   /* meshblock.cpp:90-96 */
   d->dx[0] = c->lenx / (sqrt(NBROOTS) * (d->NT - 2 * d->ng - 1));
@@ -632,6 +634,7 @@ static void initial_condition(ora_dom *d, int nb) {
     case IC_BWY1: SET4(p4, 1.0, 1.0, 0.1, 0.1); SET4(r4, 1.0, 1.0, 0.125, 0.125);
       SET4(by4, 0.75, 0.75, 0.75, 0.75); SET4(bx4, 1.0, 1.0, -1.0, -1.0); break;
     case IC_ROTOR: gamma = 1.4; tEnd = 0.295; p4[0] = 0.5; bx4[0] = 2.5 / (sqrt(4 * PI_REF)); break;
+    case IC_ROTOR_P: gamma = d->icp[6]; tEnd = d->icp[7]; break;
   }
 #undef SET4
   double *W = TILE(d, ORA_W, nb);
@@ -654,6 +657,26 @@ static void initial_condition(ora_dom *d, int nb) {
           AT(W, NT, 0, i, j) = (1 + 9. * fr);
           AT(W, NT, 1, i, j) = (-(y - 0.5) * fr / 0.1);
           AT(W, NT, 2, i, j) = (x - 0.5) * fr / 0.1;
+        } else {
+          AT(W, NT, 0, i, j) = 1.;
+        }
+      } else if (d->ic == IC_ROTOR_P) {
+        /* build-defined parameterised rotor: (rho_in, v0, p, Bx, r0, r1) with a
+           linear taper between r0 and r1 (the reference rotor's shape) */
+        const double rho_in = d->icp[0], v0 = d->icp[1], r0 = d->icp[4], r1 = d->icp[5];
+        double radius = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
+        double fr = (r1 - radius) / (r1 - r0);
+        for (int k = 0; k < 8; k++) AT(W, NT, k, i, j) = 0.;
+        AT(W, NT, 4, i, j) = d->icp[2];
+        AT(W, NT, 5, i, j) = d->icp[3];
+        if (radius <= r0) {
+          AT(W, NT, 0, i, j) = rho_in;
+          AT(W, NT, 1, i, j) = -v0 * (y - 0.5) / r0;
+          AT(W, NT, 2, i, j) = v0 * (x - 0.5) / r0;
+        } else if (radius > r0 && radius < r1) {
+          AT(W, NT, 0, i, j) = 1. + (rho_in - 1.) * fr;
+          AT(W, NT, 1, i, j) = -v0 * (y - 0.5) * fr / r0;
+          AT(W, NT, 2, i, j) = v0 * (x - 0.5) * fr / r0;
         } else {
           AT(W, NT, 0, i, j) = 1.;
         }

@@ -46,6 +46,8 @@ typedef struct {
   char ic[16];     /* IC2Dtype.cpp names: BWx BWx1 BWy BWy1 SST SST1 SSTy rotor */
   char limiter[16];/* slopelimiter.cpp names, WENO3, WENO5 */
   char flux[8];    /* RUSA HLLE HLLC HLLD */
+  double ic_params[8]; /* rotor_p (build-defined, no reference counterpart):
+                          rho_in, v0, p, Bx, r0, r1, gamma, t_end */
 } ora_config;
 
 typedef struct ora_dom ora_dom;

@@ -19,7 +19,8 @@ FILL_CC, FILL_FC, FILL_EC = 0, 1, 2
 
 class OraConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("nghosts", C.c_int), ("maxlevs", C.c_int), ("lenx", C.c_double),
-                ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16), ("flux", C.c_char * 8)]
+                ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16), ("flux", C.c_char * 8),
+                ("ic_params", C.c_double * 8)]
 
 
 _lib = None
@@ -81,8 +82,12 @@ def _ip(a):
 class Domain:
     """One oracle simulation (the reference's meshblock + main loop)."""
 
-    def __init__(self, ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0):
+    def __init__(self, ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0,
+                 ic_params=None):
         cfg = OraConfig(nx, nghosts, maxlevs, lenx, cfl, ic.encode(), limiter.encode(), flux.encode())
+        if ic_params is not None:
+            for q, v in enumerate(ic_params):
+                cfg.ic_params[q] = float(v)
         self.h = lib().ora_create(C.byref(cfg))
         if not self.h:
             raise ValueError("bad oracle configuration")

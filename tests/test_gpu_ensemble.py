@@ -1,0 +1,157 @@
+"""Batched ensembles: many independent rotors of uneven AMR depth in one context.
+
+Bar: every instance of a batch is bitwise identical to the same instance run
+alone (tree, all slot fields, t and dt), the parameterised rotor is bitwise
+identical to the oracle, and an instance that hits one of the reference's
+throw sites stops alone with the oracle's error code while the rest go on."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import compare_states, gpu_state, lockstep, oracle_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(cfg, steps):
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(cfg)
+    sim.init()
+    for _ in range(steps):
+        if sim.step() == 0:
+            break
+    return sim
+
+
+@pytest.mark.parametrize("inst", [0, 1])
+def test_rotor_p_vs_oracle_bitwise(inst):
+    """Seeded parameterised rotors (HLLC for even, HLLD for odd instances)."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    cfg = instance_config(inst)
+    cfg["maxlevs"] = 4
+    sim = A.Simulation(cfg)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "initial state differs: " + "; ".join(d[:4])
+    lockstep(sim, dom, 80, full_every=20, label=f"rotor_p {inst}")
+
+
+def test_batch_equals_alone_bitwise():
+    """Fourteen seeded instances (levels 3-6, HLLC and HLLD) advanced together."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    ids = list(range(14))
+    cfgs = [instance_config(i) for i in ids]
+    assert {c["maxlevs"] for c in cfgs} == {3, 4, 5, 6}
+    steps = 40
+    batch = A.Simulation(cfgs)
+    batch.init()
+    for _ in range(steps):
+        batch.step()
+    for q, cfg in enumerate(cfgs):
+        alone = _alone(cfg, steps)
+        ib, ia = batch.info(q), alone.info()
+        assert ib["status"] == ia["status"] == 0
+        assert (ib["t"], ib["dt"], ib["count"], ib["nleaf"]) == (ia["t"], ia["dt"], ia["count"], ia["nleaf"])
+        d = compare_states(gpu_state(batch, q), gpu_state(alone))
+        assert not d, f"instance {ids[q]}: " + "; ".join(d[:4])
+
+
+def test_failing_instances_are_isolated():
+    """At p = 1 the reference aborts many seeded rotors early (p <= 0 after the
+    first refinements).  Each such instance must stop with the oracle's code at
+    the oracle's step; the survivors must equal their alone runs."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    steps = 4
+    cfgs, expect = [], []
+    for i in range(16):
+        cfg = instance_config(i, p=1.0)
+        cfg["maxlevs"] = min(cfg["maxlevs"], 4)
+        dom = O.Domain(**cfg)
+        code, at = dom.init(), 0
+        for s in range(steps):
+            if code:
+                break
+            code, at = dom.step(), s + 1
+        cfgs.append(cfg)
+        expect.append((code, at if code else None))
+    assert any(e[0] for e in expect) and not all(e[0] for e in expect)
+    batch = A.Simulation(cfgs)
+    batch.init()  # instances that fail here are retired, the call succeeds
+    got_step = [0 if batch.info(q)["status"] else None for q in range(len(cfgs))]
+    for s in range(steps):
+        batch.step()
+        for q in range(len(cfgs)):
+            if got_step[q] is None and batch.info(q)["status"]:
+                got_step[q] = s + 1
+    for q, cfg in enumerate(cfgs):
+        inf = batch.info(q)
+        assert (inf["status"], got_step[q]) == expect[q], f"instance {q}: gpu {inf['status']} at {got_step[q]}, " \
+                                                          f"oracle {expect[q]}"
+        if inf["status"] == 0:
+            d = compare_states(gpu_state(batch, q), gpu_state(_alone(cfg, steps)))
+            assert not d, f"survivor {q}: " + "; ".join(d[:4])
+
+
+def test_single_instance_failure_still_raises():
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    cfg = None
+    for i in range(64):
+        c = instance_config(i, p=1.0)
+        c["maxlevs"] = 3
+        dom = O.Domain(**c)
+        if dom.init() != 0:
+            cfg = c
+            break
+    assert cfg is not None
+    sim = A.Simulation(cfg)
+    with pytest.raises(A.AmrError) as ei:
+        sim.init()
+    assert ei.value.code == 1 and sim.info()["status"] == 1
+
+
+def test_full_ensemble_runs_and_samples_match():
+    """All 256 seeded instances in one context for 12 steps.  One instance per
+    depth equals its alone run; every instance that aborts does so with the
+    oracle's code at the oracle's step."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import ensemble_configs
+
+    cfgs = ensemble_configs()
+    steps = 12
+    batch = A.Simulation(cfgs)
+    batch.init()
+    failed_at = {q: 0 for q in range(len(cfgs)) if batch.info(q)["status"]}
+    for s in range(steps):
+        batch.step()
+        for q in range(len(cfgs)):
+            if q not in failed_at and batch.info(q)["status"]:
+                failed_at[q] = s + 1
+    picked = {}
+    for q, c in enumerate(cfgs):
+        if q not in failed_at:
+            picked.setdefault(c["maxlevs"], q)
+    assert len(picked) == 4
+    for q in picked.values():
+        alone = _alone(cfgs[q], steps)
+        assert alone.info()["status"] == 0
+        d = compare_states(gpu_state(batch, q), gpu_state(alone))
+        assert not d, f"instance {q}: " + "; ".join(d[:4])
+    assert len(failed_at) < len(cfgs) // 8
+    for q, at in failed_at.items():
+        dom = O.Domain(**cfgs[q])
+        code, k = dom.init(), 0
+        while not code and k < at:
+            code, k = dom.step(), k + 1
+        assert (code, k) == (batch.info(q)["status"], at), f"instance {q}: oracle {code} at {k}, gpu " \
+                                                             f"{batch.info(q)['status']} at {at}"
