@@ -1785,6 +1785,29 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
   return (int)I.ops.size();
 }
 
+// Compiled fill program of the given tree: up to `cap` ops copied to `ops`
+// (16-byte FillOp records), the explicit-operand table size to *ngen; returns
+// the op count (negative on error).
+int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs, int cap,
+                      void* ops, int* ngen) {
+  Instance I;
+  I.lastActive = n;
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.icoord[nb] = {icoord[2 * nb], icoord[2 * nb + 1]};
+    I.leafs[nb] = (uint8_t)leafs[nb];
+    I.gs.push_back(nb);
+  }
+  build_schedule(I);
+  FillScratch S;
+  if (compile_instance(I, NT, ng, S)) return -1;
+  const int m = (int)I.ops.size();
+  if (ops) std::memcpy(ops, I.ops.data(), sizeof(FillOp) * std::min(m, cap));
+  if (ngen) *ngen = (int)I.gen.size();
+  return m;
+}
+
 // coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]
 int amr_debug_coarsen_maps(int NT, int ng, int32_t* mU, int32_t* mB) {
   std::vector<int32_t> u, b;

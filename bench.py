@@ -36,6 +36,10 @@ ROTOR_STEPS = 4763  # steps of the full rotor run to t = 0.295 without FMA contr
 # stage 1 reads U (64 B) and face B (16 B), writes Us (64 B); stage 2 reads
 # Us and U (128 B) and face B (16 B), writes U (64 B) -> mean 176 B
 STAGE_BYTES_PER_ZONE = 176.0
+# HBM bytes per leaf zone one stage launch actually moves, from rocprofv3 FETCH_SIZE
+# (x2.0, copy-calibrated) + WRITE_SIZE (x1.0) on this workload:
+# profiles/stage_traffic_rotor_lv5.md (272.7 read + 270.0 written)
+STAGE_TRAFFIC_BYTES_PER_ZONE = 542.7
 HBM_PEAK_GBS = 8000.0
 
 
@@ -77,7 +81,7 @@ def cpu_baseline(steps, threads):
                        f"({zu} zone-updates in {tm['loop_seconds']:.2f} s), host CPU: {cpu_model()}")
 
 
-def ensemble_bench(steps, full, n_inst):
+def ensemble_bench(steps, full, n_inst, plain=True, warmup=0):
     """Batched ensemble of seeded parameterised rotors against the same
     instances run one after another, each alone in its own context (plain
     launches).  Both legs time exactly `steps` steps of every instance from
@@ -91,6 +95,8 @@ def ensemble_bench(steps, full, n_inst):
     cfgs = ensemble_configs(n_inst)
     batch = A.Simulation(cfgs)
     batch.init()
+    if warmup:
+        batch.run(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     nb, zb = batch.run(steps)
@@ -99,7 +105,7 @@ def ensemble_bench(steps, full, n_inst):
     failed = [q for q in range(len(cfgs)) if batch.info(q)["status"]]
     batch.close()
     tp, zp = 0.0, 0
-    for q, cfg in enumerate(cfgs):
+    for q, cfg in enumerate(cfgs if plain and not warmup else []):
         if q in failed:
             continue  # an aborted instance's partial step is not a zone-update in either leg
         sim = A.Simulation(cfg)
@@ -114,7 +120,7 @@ def ensemble_bench(steps, full, n_inst):
     out = {"instances": len(cfgs), "seed": SEED, "aborted_instances": failed,
            "workload": "seeded rotor_p ensemble: rho_in U[5,20], v0 U[1,3], Bx U[0.5,2]/sqrt(4pi), p 5, "
                        "max level U{3..6}, MC-MUSCL, HLLC (even) / HLLD (odd), CT, RK2, CFL 0.5",
-           "head_to_head_steps": steps,
+           "head_to_head_steps": steps, "batched_warmup_steps": warmup,
            "batched": {"zone_updates": zb, "seconds": tb, "value": zb / tb},
            "plain_sequential": {"zone_updates": zp, "seconds": tp, "value": zp / tp if tp else None},
            "speedup": (zb / tb) / (zp / tp) if tp and zp else None}
@@ -146,6 +152,10 @@ def main():
                     help="also time a batched ensemble of N seeded rotors against N plain runs")
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-full", action="store_true", help="also run the batched ensemble to t = 0.2")
+    ap.add_argument("--ens-warmup", type=int, default=0,
+                    help="untimed batched steps before the timed ones (skips the plain leg)")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="time only the batched ensemble (for profiling); prints its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -159,6 +169,10 @@ def main():
     import amr2d_mhd_amd as A
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.only_ensemble:
+        print(json.dumps(ensemble_bench(args.ens_steps, args.ens_full, args.ensemble or 256, plain=False,
+                                        warmup=args.ens_warmup)))
+        return 0
     if args.copy_calibration:
         x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -204,14 +218,15 @@ def main():
                    "t_end_reached": info["t"], "leaves_at_end": info["nleaf"],
                    "zone_updates": zu, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": STAGE_TRAFFIC_BYTES_PER_ZONE * zones_per_launch,
                      "kernel": "stage_kernel<20,2>", "mean_launch_us": stage_ms / stage_n * 1e3,
                      "launches": stage_n, "bytes_per_zone": STAGE_BYTES_PER_ZONE,
                      "zones_per_launch": zones_per_launch},
         "cpu_baseline": None,
     }
     if args.ensemble:
-        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble)
+        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup)
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)
