@@ -71,11 +71,15 @@ _SIGS = {
                                     C.c_void_p, C.c_void_p]),
     "amr_recon_batch": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
+    "amr_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_set_stage_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "amr_debug_host_fill": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "amr_debug_coarsen_maps": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "amr_debug_compile_ns": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "amr_debug_compile": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_void_p]),
     "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
@@ -90,6 +94,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libamr2d_mhd.so not built at {LIB_PATH}; run `make -C amr2d_mhd_amd/csrc`")
+        # PyTorch-ROCm ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Whichever copy loads first serves the whole
+        # process, and torch's GPU support fails on the other one, so torch
+        # is imported (when present) before this library pulls in a runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)

@@ -70,13 +70,22 @@ struct InstParams {
 constexpr int64_t kNodeBit = int64_t(1) << 62;
 inline __host__ __device__ int64_t src_ref(int64_t gs, int cell) { return (gs << 16) | cell; }
 
-enum { kFillCopy = 0, kFillRestrict = 1, kFillGeneral = 2 };
+enum { kFillCopy = 0, kFillRestrict = 1, kFillGeneral = 2, kFillNop = 3 };
+
+// A fill program runs in chunks of kFillChunk ops, one workgroup each: the
+// workgroup reads every operand of its chunk, synchronises, then writes.  The
+// compiler keeps every destination that another op of the program reads
+// (a stale-value read) in the same chunk as that reader, so no chunk writes
+// an element another chunk reads; instance segments are padded with
+// kFillNop to whole chunks.
+constexpr int kFillChunk = 256;
 
 // One destination element of a fill (16 bytes):
 //   kFillCopy      value = pre-fill element (src, src_cell)
 //   kFillRestrict  fine->coarse restriction of the 2x2 cells starting at
 //                  (src, src_cell): (i1,j1), (i1,j1+1), (i1+1,j1), (i1+1,j1+1)
 //   kFillGeneral   restriction whose operands are given by FillGen[src]
+//   kFillNop       padding
 // The combine rule (cell 2x2 mean, face-pair mean, edge injection) is chosen
 // by the field being filled.
 struct FillOp {

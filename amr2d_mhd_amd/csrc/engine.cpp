@@ -56,9 +56,10 @@ int ic_code(const char* s) {
 }
 
 // restriction node: `simple` when its four operand cells still hold their
-// pre-fill values (the common case), else explicit operand references
+// pre-fill values (the common case), else explicit operand references.
+// During compilation references use the instance's own block ids (nb).
 struct RNode {
-  int32_t gs;
+  int32_t nb;
   uint16_t cell;
   bool simple;
   int64_t ref[4];
@@ -66,10 +67,20 @@ struct RNode {
 
 // host scratch reused across compiles (cur[] is reset only where touched)
 struct FillScratch {
-  std::vector<int64_t> cur;
-  std::vector<int32_t> touched;
+  std::vector<int64_t> cur;       // symbolic value per element, -1 = pre-fill
+  std::vector<int32_t> touched;   // written elements, (nb << 16) | cell, in first-write order
   std::vector<RNode> nodes;
-  std::vector<int64_t> memo;
+  struct RingSrc {
+    int32_t op;
+    int32_t nref;
+    int64_t ref[4];  // block-local operands
+  };
+  std::vector<RingSrc> ringsrc;   // ops with an operand in a ghost ring
+  std::vector<int32_t> uf;        // union-find over ops linked by stale reads (-1: not linked)
+  std::vector<int32_t> order;
+  std::vector<FillOp> moved;
+  std::vector<uint8_t> ring;      // per cell of a tile: 1 in the ghost ring
+  int ring_nt = 0;
   bool force_general = false;  // test hook: emit every restriction as kFillGeneral
 };
 
@@ -175,7 +186,7 @@ struct amr_ctx {
   // host copies kept alive until the stream has consumed them
   std::vector<RegridOp> h_refs, h_coars;
   std::vector<FillScratch> scratch;  // one per compile thread
-  // fill program tables: per-instance segment pointers and op-count prefix
+  // fill program tables: per-instance segment pointers and chunk-count prefix
   const FillOp** d_inst_ops = nullptr;
   const FillGen** d_inst_gen = nullptr;
   int32_t* d_prefix = nullptr;
@@ -183,11 +194,28 @@ struct amr_ctx {
   PinnedVec<const FillOp*> h_inst_ops;
   PinnedVec<const FillGen*> h_inst_gen;
   PinnedVec<int32_t> h_prefix;
-  int nops = 0;
-  double* d_fillbuf = nullptr;
-  size_t cap_fillbuf = 0;
+  int nchunks = 0;
   InstParams* d_ip = nullptr;
-  std::vector<InstParams> h_ip;
+  PinnedVec<InstParams> h_ip;  // pinned: the step graph copies from it
+  // the device work of one step replayed as a HIP graph; re-captured when
+  // the topology (work lists, program tables, buffer addresses) changes
+  bool graph_on = true, graph_dirty = true, regrid_launched = false;
+  // stage-kernel variant: fixed (0..kStageVariants-1) or chosen per leaf-count
+  // bucket by the tuner (-1): the first launches in a bucket try every variant
+  // kTuneTrials times, timed by the kernel's own device-clock span, then the
+  // fastest per leaf is kept for that bucket
+  int stage_variant = -1;
+  struct TuneBucket {
+    double us_per_leaf[kStageVariants] = {0};
+    int n[kStageVariants] = {0};
+    int chosen = -1;
+  };
+  TuneBucket tune[24];
+  unsigned long long* d_span = nullptr;  // [stage 1 start, end, stage 2 start, end]
+  unsigned long long* h_span = nullptr;
+  int span_variant[2] = {-1, -1};
+  double wall_khz = 0;
+  hipGraphExec_t gexec = nullptr;
   double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
   int cap_red = 0;
   DevError* d_err = nullptr;  // one record per instance
@@ -255,14 +283,6 @@ int ensure(T*& p, int& cap, int n) {
   return 0;
 }
 
-int ensure_bytes(double*& p, size_t& cap, size_t n) {
-  if (n <= cap) return 0;
-  if (p) hipFree(p);
-  size_t nc = std::max(n, cap * 2);
-  if (hipMalloc((void**)&p, sizeof(double) * nc) != hipSuccess) { p = nullptr; cap = 0; return AMR_ERR_HIP; }
-  cap = nc;
-  return 0;
-}
 
 // ----------------------------------------------------- timing families
 // With `attached`, the caller hands the two events to the launch itself
@@ -491,17 +511,17 @@ struct FillCompiler {
   size_t addr(int nb, int i, int j) const { return (size_t)nb * N2 + j * NT + i; }
   int64_t read(int nb, int i, int j) const {
     const int64_t v = S.cur[addr(nb, i, j)];
-    return v >= 0 ? v : src_ref(I.gs[nb], j * NT + i);
+    return v >= 0 ? v : src_ref(nb, j * NT + i);
   }
   void write(int nb, int i, int j, int64_t v) {
     const size_t a = addr(nb, i, j);
-    if (S.cur[a] < 0) S.touched.push_back((int32_t)a);
+    if (S.cur[a] < 0) S.touched.push_back((nb << 16) | (j * NT + i));
     S.cur[a] = v;
   }
   void copy(int dn, int di, int dj, int sn, int si, int sj) { write(dn, di, dj, read(sn, si, sj)); }
   void restrict_(int dn, int di, int dj, int fn, int i1, int j1) {
     RNode nd{};
-    nd.gs = I.gs[fn];
+    nd.nb = fn;
     nd.cell = (uint16_t)(j1 * NT + i1);
     nd.simple = !S.force_general && S.cur[addr(fn, i1, j1)] < 0 && S.cur[addr(fn, i1, j1 + 1)] < 0 &&
                 S.cur[addr(fn, i1 + 1, j1)] < 0 && S.cur[addr(fn, i1 + 1, j1 + 1)] < 0;
@@ -515,7 +535,7 @@ struct FillCompiler {
     write(dn, di, dj, kNodeBit | (int64_t)(S.nodes.size() - 1));
   }
   void finish() {
-    for (int32_t a : S.touched) S.cur[a] = -1;
+    for (int32_t t : S.touched) S.cur[(size_t)(t >> 16) * N2 + (t & 0xffff)] = -1;
   }
   // corner of an entry: the diagonal block `cb` found through the neighbour
   // table is copied when it is a leaf (with same-level indices even if it is
@@ -717,46 +737,141 @@ struct FillCompiler {
 // destination) cannot occur for nghosts 2 and 4 -- the fine elements a
 // restriction reads are interior cells or, for nghosts 4, ghost cells that
 // only the same entry's later injection writes -- and are rejected.
+//
+// Chunking: an op may read (as a pre-fill value) an element that another op
+// writes -- the reference's stale corner reads.  Such readers and writers
+// are linked into groups that are laid out inside one kFillChunk chunk, so
+// the single-pass kernel (read all, barrier, write all per chunk) keeps the
+// sequential semantics.  The segment is padded with kFillNop to whole chunks.
 int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
   I.ops.clear();
   I.gen.clear();
   const int N2 = NT * NT;
   const int off[4] = {0, NT, 1, NT + 1};
+  if (S.ring_nt != NT) {
+    S.ring.assign(N2, 0);
+    for (int c = 0; c < N2; c++) {
+      const int i = c % NT, j = c / NT;
+      S.ring[c] = (i < ng || i >= NT - ng || j < ng || j >= NT - ng) ? 1 : 0;
+    }
+    S.ring_nt = NT;
+  }
   FillCompiler fc(I, NT, ng, S);
   fc.run();
   int status = 0;
-  I.ops.reserve(S.touched.size());
-  for (int32_t a : S.touched) {
-    const int nb = a / N2, cell = a % N2;
+  // 1. ops in first-write order; a written element's cur[] entry becomes its
+  //    op index (tagged) for the stale-read search below
+  constexpr int64_t kOpTag = int64_t(1) << 61;
+  S.ringsrc.clear();
+  S.order.clear();
+  const uint8_t* ring = S.ring.data();
+  const int32_t* gs = I.gs.data();
+  I.ops.reserve(S.touched.size() + 2 * kFillChunk);
+  for (int32_t t : S.touched) {
+    const int nb = t >> 16, cell = t & 0xffff;
+    const size_t a = (size_t)nb * N2 + cell;
     const int64_t v = S.cur[a];
     FillOp op{};
-    op.dst_gs = I.gs[nb];
+    op.dst_gs = gs[nb];
     op.dst_cell = (uint16_t)cell;
+    FillScratch::RingSrc rs;
+    rs.nref = 0;
     if (!(v & kNodeBit)) {
-      if (v == src_ref(I.gs[nb], cell)) continue;  // value unchanged
+      if (v == src_ref(nb, cell)) {  // value unchanged
+        S.cur[a] = -1;
+        continue;
+      }
       op.op = kFillCopy;
-      op.src = (int32_t)(v >> 16);
+      op.src = gs[v >> 16];
       op.src_cell = (uint16_t)(v & 0xffff);
+      if (ring[op.src_cell]) { rs.nref = 1; rs.ref[0] = v; }
     } else {
       const RNode& nd = S.nodes[v & ~kNodeBit];
       if (nd.simple && !S.force_general) {
         op.op = kFillRestrict;
-        op.src = nd.gs;
+        op.src = gs[nd.nb];
         op.src_cell = nd.cell;
+        if (ring[nd.cell] | ring[nd.cell + 1] | ring[nd.cell + NT] | ring[nd.cell + NT + 1]) {
+          rs.nref = 4;
+          for (int q = 0; q < 4; q++) rs.ref[q] = src_ref(nd.nb, nd.cell + off[q]);
+        }
       } else {
         FillGen g{};
+        rs.nref = 4;
         for (int q = 0; q < 4; q++) {
-          g.ref[q] = nd.simple ? src_ref(nd.gs, nd.cell + off[q]) : nd.ref[q];
-          if (g.ref[q] & kNodeBit) status = AMR_ERR_CONFIG;
+          const int64_t r = nd.simple ? src_ref(nd.nb, nd.cell + off[q]) : nd.ref[q];
+          if (r & kNodeBit) status = AMR_ERR_CONFIG;
+          rs.ref[q] = r;
+          g.ref[q] = src_ref(gs[r >> 16], (int)(r & 0xffff));
         }
         op.op = kFillGeneral;
         op.src = (int32_t)I.gen.size();
         I.gen.push_back(g);
       }
     }
+    if (rs.nref) {
+      rs.op = (int32_t)I.ops.size();
+      S.ringsrc.push_back(rs);
+    }
+    S.cur[a] = kOpTag | (int64_t)I.ops.size();
     I.ops.push_back(op);
   }
+  const int n = (int)I.ops.size();
+  // 2. link every op that reads (pre-fill) an element another op writes with
+  //    that writer; fills write ghost-ring elements only, so only operands in
+  //    a ring can be written ones
+  bool linked = false;
+  auto root = [&](int x) {
+    while (S.uf[x] != x) x = S.uf[x] = S.uf[S.uf[x]];
+    return x;
+  };
+  for (const auto& rs : S.ringsrc)
+    for (int t = 0; t < rs.nref; t++) {
+      const int64_t ref = rs.ref[t];
+      const int64_t w = S.cur[(size_t)(ref >> 16) * N2 + (ref & 0xffff)];
+      const int q = rs.op;
+      if (w < 0 || !(w & kOpTag) || (int)(w & ~kOpTag) == q) continue;
+      if (!linked) {
+        S.uf.assign(n, -1);
+        linked = true;
+      }
+      const int wi = (int)(w & ~kOpTag);
+      if (S.uf[q] < 0) { S.uf[q] = q; S.order.push_back(q); }
+      if (S.uf[wi] < 0) { S.uf[wi] = wi; S.order.push_back(wi); }
+      const int ra = root(q), rb = root(wi);
+      if (ra != rb) S.uf[std::max(ra, rb)] = std::min(ra, rb);
+    }
   fc.finish();
+  // 3. linked groups move behind the other ops, each inside one chunk; their
+  //    old positions become padding
+  FillOp nop{};
+  nop.op = kFillNop;
+  auto pad_to_chunk = [&]() {
+    while (I.ops.size() % kFillChunk) I.ops.push_back(nop);
+  };
+  if (linked) {
+    for (int q : S.order) S.uf[q] = root(q);
+    std::sort(S.order.begin(), S.order.end(),
+              [&](int a, int b) { return S.uf[a] != S.uf[b] ? S.uf[a] < S.uf[b] : a < b; });
+    S.moved.clear();
+    FillOp* p = I.ops.data();
+    for (int q : S.order) {
+      S.moved.push_back(p[q]);
+      p[q] = nop;
+    }
+    pad_to_chunk();
+    size_t g = 0;
+    while (g < S.order.size()) {
+      size_t h = g;
+      while (h < S.order.size() && S.uf[S.order[h]] == S.uf[S.order[g]]) h++;
+      const size_t m = h - g;
+      if (m > (size_t)kFillChunk) return AMR_ERR_CONFIG;
+      if (I.ops.size() % kFillChunk + m > (size_t)kFillChunk) pad_to_chunk();
+      for (size_t t = g; t < h; t++) I.ops.push_back(S.moved[t]);
+      g = h;
+    }
+  }
+  pad_to_chunk();
   return status;
 }
 
@@ -1036,10 +1151,11 @@ void coarsen_maps(int NT, int ng, std::vector<int32_t>& mU, std::vector<int32_t>
 namespace {
 
 void fill_params(amr_ctx* c) {
-  c->h_ip.resize(c->inst.size());
+  c->h_ip.reserve(c->inst.size());
+  c->h_ip.n = c->inst.size();
   for (size_t q = 0; q < c->inst.size(); q++) {
     const Instance& I = c->inst[q];
-    InstParams& P = c->h_ip[q];
+    InstParams& P = c->h_ip.data()[q];
     P.gamma = I.gamma;
     P.dt = I.dt_dom;
     for (int l = 0; l < kMaxLev; l++) P.dx[l] = I.dx[l];
@@ -1125,10 +1241,10 @@ int rebuild_topology(amr_ctx* c) {
   for (auto& I : c->inst) {
     c->h_inst_ops.push_back(I.d_ops);
     c->h_inst_gen.push_back(I.d_gen);
-    total += I.done ? 0 : (int)I.ops.size();
+    total += I.done ? 0 : (int)(I.ops.size() / kFillChunk);
     c->h_prefix.push_back(total);
   }
-  c->nops = total;
+  c->nchunks = total;
   if (c->cap_tables < ni + 1) {
     hipFree(c->d_inst_ops);
     hipFree(c->d_inst_gen);
@@ -1141,7 +1257,6 @@ int rebuild_topology(amr_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_inst_ops, c->h_inst_ops.data(), sizeof(FillOp*) * ni, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_inst_gen, c->h_inst_gen.data(), sizeof(FillGen*) * ni, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_prefix, c->h_prefix.data(), sizeof(int32_t) * (ni + 1), hipMemcpyHostToDevice, c->stream));
-  if (int e = ensure_bytes(c->d_fillbuf, c->cap_fillbuf, (size_t)4 * 8 * std::max(1, c->nops))) return e;
   const int old_cap = c->cap_red;
   if (int e = ensure(c->d_lam, c->cap_red, std::max(1, c->nleaf))) return e;
   if (c->cap_red != old_cap || !c->d_grad) {
@@ -1154,6 +1269,7 @@ int rebuild_topology(amr_ctx* c) {
     HIPCHK(hipHostMalloc((void**)&c->h_grad, sizeof(double) * c->cap_red));
   }
   c->topo_dirty = false;
+  c->graph_dirty = true;
   return 0;
 }
 
@@ -1170,13 +1286,71 @@ FillSet make_set(amr_ctx* c, std::initializer_list<std::pair<int, int>> fk) {
 
 void run_fill(amr_ctx* c, const FillSet& s) {
   Timed tm(c, PF_FILL);
-  launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->nops, s, c->d_fillbuf,
-              c->stream);
+  launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->nchunks, s, c->stream);
 }
 
-void run_stage(amr_ctx* c, int step) {
+constexpr int kTuneTrials = 2;
+
+int tune_bucket(int nleaf) {
+  int b = 0;
+  while ((2 << b) <= nleaf && b < 23) b++;
+  return b;
+}
+
+// variant of the next stage launch; sets *timed when the launch is a trial
+// (trials run only inside full steps, whose spans are collected)
+int stage_choice(amr_ctx* c, bool* timed, bool allow_trial = true) {
+  *timed = false;
+  if (c->stage_variant >= 0) return c->stage_variant;
+  const auto& B = c->tune[tune_bucket(c->nleaf)];
+  if (B.chosen >= 0) return B.chosen;
+  if (!allow_trial) return 0;
+  for (int v = 0; v < kStageVariants; v++)
+    if (B.n[v] < kTuneTrials) {
+      *timed = true;
+      return v;
+    }
+  return 0;
+}
+
+bool tuning(amr_ctx* c) {
+  bool t;
+  stage_choice(c, &t);
+  return t;
+}
+
+void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
   Timed tm(c, PF_STAGE, true);
-  launch_stage(c->NT, c->ng, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream, tm.start(), tm.stop());
+  bool timed;
+  const int v = stage_choice(c, &timed, allow_trial);
+  unsigned long long* span = timed ? c->d_span + 2 * (step - 1) : nullptr;
+  c->span_variant[step - 1] = timed ? v : -1;
+  if (timed) {
+    // the trial counts as pending until the step's results come back
+    c->tune[tune_bucket(c->nleaf)].n[v] += 1;
+  }
+  launch_stage(c->NT, c->ng, v, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream, tm.start(),
+               tm.stop(), span);
+}
+
+// fold the spans of this step's trial launches into the tuner
+void tune_collect(amr_ctx* c) {
+  auto& B = c->tune[tune_bucket(c->nleaf)];
+  for (int q = 0; q < 2; q++) {
+    const int v = c->span_variant[q];
+    if (v < 0) continue;
+    c->span_variant[q] = -1;
+    const unsigned long long t0 = ~c->h_span[2 * q], t1 = c->h_span[2 * q + 1];
+    if (t1 > t0 && c->wall_khz > 0) B.us_per_leaf[v] += (double)(t1 - t0) / (c->wall_khz * 1e-3) / c->nleaf;
+  }
+  bool all = true;
+  for (int v = 0; v < kStageVariants; v++) all &= B.n[v] >= kTuneTrials;
+  if (all && B.chosen < 0) {
+    int best = 0;
+    for (int v = 1; v < kStageVariants; v++)
+      if (B.us_per_leaf[v] < B.us_per_leaf[best]) best = v;
+    B.chosen = best;
+  }
 }
 
 // CT2D.cpp:11-124
@@ -1197,11 +1371,11 @@ void run_ct(amr_ctx* c, int step) {
   }
 }
 
-// waits for the stream and retires the instances whose kernels reported an
-// error; returns nonzero only for a HIP failure
-int sync_errors(amr_ctx* c) {
+// waits for the stream (the error records already queued for download) and
+// retires the instances whose kernels reported an error; returns nonzero only
+// for a HIP failure
+int wait_errors(amr_ctx* c) {
   const int ni = (int)c->inst.size();
-  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * ni, hipMemcpyDeviceToHost, c->stream));
   {
     HostTimed ht(c, PF_HOST_WAIT);
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1216,6 +1390,11 @@ int sync_errors(amr_ctx* c) {
   }
   if (any) HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(DevError) * ni, c->stream));
   return 0;
+}
+
+int sync_errors(amr_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
+  return wait_errors(c);
 }
 
 // regrid data transfer in reference order: all refinements, then all
@@ -1290,12 +1469,79 @@ int regrid_all(amr_ctx* c) {
       I.prog_dirty = true;
     }
   }
+  c->regrid_launched = !refs.empty() || !coars.empty();
   if (int e = apply_regrid_ops(c, refs, true)) return e;
   if (int e = apply_regrid_ops(c, coars, false)) return e;
   return 0;
 }
 
 // main.cpp:128-247 for all running instances
+// Device work of one step (main.cpp:128-230) up to the download of the CFL
+// bounds, indicators and error records: stream-ordered launches only, so it
+// can be captured into a graph.
+int enqueue_step(amr_ctx* c) {
+  const bool trial = tuning(c);
+  if (trial) HIPCHK(hipMemsetAsync(c->d_span, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->inst.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  run_fill(c, make_set(c, {{kU, kFillCell}, {kBi, kFillFace}}));
+  run_stage(c, 1);
+  run_fill(c, make_set(c, {{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}}));
+  run_ct(c, 1);
+  run_fill(c, make_set(c, {{kUs, kFillCell}}));
+  run_stage(c, 2);
+  run_fill(c, make_set(c, {{kFF, kFillFace}, {kGG, kFillFace}}));
+  run_ct(c, 2);
+  run_fill(c, make_set(c, {{kU, kFillCell}}));
+  {
+    Timed tm(c, PF_CFL);
+    launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
+                        c->stream);
+  }
+  if (c->nleaf) {
+    HIPCHK(hipMemcpyAsync(c->h_lam, c->d_lam, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
+  if (trial)
+    HIPCHK(hipMemcpyAsync(c->h_span, c->d_span, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+// (Re)capture the step into the context's graph: update the executable graph
+// in place when the captured topology allows it, else instantiate anew.
+int capture_step(amr_ctx* c) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int e = enqueue_step(c);
+  const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (e || ce != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (!e) std::fprintf(stderr, "amr2d_mhd: graph capture failed: %s\n", hipGetErrorString(ce));
+    return e ? e : AMR_ERR_HIP;
+  }
+  if (c->gexec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(c->gexec, g, &bad, &res) != hipSuccess) {
+      (void)hipGetLastError();
+      hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+  }
+  if (!c->gexec) {
+    const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      hipGraphDestroy(g);
+      std::fprintf(stderr, "amr2d_mhd: graph instantiate failed: %s\n", hipGetErrorString(ie));
+      return AMR_ERR_HIP;
+    }
+  }
+  hipGraphDestroy(g);
+  c->graph_dirty = false;
+  return 0;
+}
+
 // Returns nonzero for a HIP failure, or when no instance is left running
 // and one failed during this call (its code); otherwise instance failures
 // are reported through amr_info().status.
@@ -1307,17 +1553,18 @@ int step_all(amr_ctx* c, int* n_active) {
   }
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
-  if (int e = upload_params(c)) return e;
-  run_fill(c, make_set(c, {{kU, kFillCell}, {kBi, kFillFace}}));
-  run_stage(c, 1);
-  run_fill(c, make_set(c, {{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}}));
-  run_ct(c, 1);
-  run_fill(c, make_set(c, {{kUs, kFillCell}}));
-  run_stage(c, 2);
-  run_fill(c, make_set(c, {{kFF, kFillFace}, {kGG, kFillFace}}));
-  run_ct(c, 2);
-  run_fill(c, make_set(c, {{kU, kFillCell}}));
-  if (int e = cfl_and_indicators(c, nullptr)) return e;
+  fill_params(c);
+  const bool device_timing = (c->prof_mask & 0x1fu) != 0;
+  const bool trial = tuning(c);
+  if (c->graph_on && !device_timing && !trial) {
+    if (c->graph_dirty || !c->gexec)
+      if (int e = capture_step(c)) return e;
+    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    if (int e = enqueue_step(c)) return e;
+  }
+  if (int e = wait_errors(c)) return e;
+  if (trial) tune_collect(c);
   int active = 0;
   for (size_t q = 0; q < c->inst.size(); q++) {
     Instance& I = c->inst[q];
@@ -1331,7 +1578,8 @@ int step_all(amr_ctx* c, int* n_active) {
     if (I.dt_main < 0 || std::isnan(I.dt_main)) fail_instance(c, (int)q, AMR_ERR_DT, -1, 0, 0);
   }
   if (int e = regrid_all(c)) return e;
-  if (int e = sync_errors(c)) return e;  // refine/coarsen errors
+  if (c->regrid_launched)
+    if (int e = sync_errors(c)) return e;  // refine/coarsen errors
   for (auto& I : c->inst) {
     if (!I.done && !(I.t <= I.tEnd)) {
       I.done = true;
@@ -1408,9 +1656,17 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
       hipMalloc((void**)&c->d_err, sizeof(DevError) * n_inst) != hipSuccess ||
       hipMemset(c->d_err, 0, sizeof(DevError) * n_inst) != hipSuccess ||
       hipHostMalloc((void**)&c->h_err, sizeof(DevError) * n_inst) != hipSuccess ||
+      hipMalloc((void**)&c->d_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     amr_destroy(c);
     return AMR_ERR_HIP;
+  }
+  {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
+      c->wall_khz = khz;
+    if (c->wall_khz <= 0) c->stage_variant = 0;  // no device clock: no tuning
   }
   std::vector<int32_t> mU, mB;
   coarsen_maps(c->NT, c->ng, mU, mB);
@@ -1430,7 +1686,7 @@ int amr_destroy(amr_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < kNumFields; f++)
     if (c->F.f[f]) hipFree(c->F.f[f]);
-  hipFree(c->d_leaves); hipFree(c->d_fillbuf);
+  hipFree(c->d_leaves);
   hipFree(c->d_inst_ops); hipFree(c->d_inst_gen); hipFree(c->d_prefix);
   for (auto& I : c->inst) {
     hipFree(I.d_ops);
@@ -1447,6 +1703,9 @@ int amr_destroy(amr_ctx* c) {
     hipEventDestroy(e[0]);
     hipEventDestroy(e[1]);
   }
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  hipFree(c->d_span);
+  if (c->h_span) hipHostFree(c->h_span);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return AMR_OK;
@@ -1628,7 +1887,7 @@ int amr_stage(amr_ctx* c, int step) {
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
-  run_stage(c, step);
+  run_stage(c, step, false);
   return finish_call(c);
 }
 
@@ -1694,6 +1953,31 @@ int amr_recon_batch(int NT, int nghosts, const char* limiter, double dx, int nt,
   return AMR_OK;
 }
 
+int amr_set_stage_variant(amr_ctx* c, int variant) {
+  if (!c || variant < -1 || variant >= kStageVariants) return AMR_ERR_CONFIG;
+  c->stage_variant = (variant < 0 && c->wall_khz <= 0) ? 0 : variant;
+  c->graph_dirty = true;
+  return AMR_OK;
+}
+
+int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_leaf) {
+  if (!c || nbuckets < 0) return AMR_ERR_CONFIG;
+  for (int b = 0; b < nbuckets && b < 24; b++) {
+    if (chosen) chosen[b] = c->tune[b].chosen;
+    for (int v = 0; v < kStageVariants && us_per_leaf; v++)
+      us_per_leaf[b * kStageVariants + v] =
+          c->tune[b].n[v] ? c->tune[b].us_per_leaf[v] / c->tune[b].n[v] : 0.0;
+  }
+  return AMR_OK;
+}
+
+int amr_set_graph(amr_ctx* c, int on) {
+  if (!c) return AMR_ERR_CONFIG;
+  c->graph_on = on != 0;
+  c->graph_dirty = true;
+  return AMR_OK;
+}
+
 int amr_profile_enable(amr_ctx* c, int mask) {
   if (!c) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1749,10 +2033,12 @@ struct HostFill {
 extern "C" {
 
 // Applies the fill program compiled for the given tree to Q[n][nv][NT][NT]
-// in place (emulating the device's gather/scatter on the host); returns the
-// number of destination ops (negative on error); *nnodes receives the number
-// of nested restriction nodes.  force_general != 0 emits every restriction
-// through the explicit-operand path.
+// in place, emulating the device chunk by chunk (read all values of a chunk,
+// then write them); returns the number of live destination ops, or -1 on a
+// compile error, -2 for a segment that is not whole chunks, -3 when an
+// element one chunk writes is read by another chunk.  *nnodes receives the
+// number of nested restriction nodes.  force_general != 0 emits every
+// restriction through the explicit-operand path.
 int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
                         double* Q, int nv, int kind, int* nnodes, int force_general) {
   amr_ctx c;
@@ -1774,15 +2060,42 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
   S.force_general = force_general != 0;
   if (compile_instance(I, NT, ng, S)) return -1;
   const int N2 = NT * NT;
-  std::vector<double> buf(I.ops.size() * 8, 0.0);
   HostFill hf{Q, nv, NT, N2, kind, I.gen};
-  FillOp* ops = I.ops.data();
-  for (size_t e = 0; e < I.ops.size(); e++)
-    for (int k = 0; k < nv; k++) buf[e * 8 + k] = hf.value(ops[e], k);
-  for (size_t e = 0; e < I.ops.size(); e++)
-    for (int k = 0; k < nv; k++) Q[((size_t)ops[e].dst_gs * nv + k) * N2 + ops[e].dst_cell] = buf[e * 8 + k];
+  const FillOp* ops = I.ops.data();
+  const size_t m = I.ops.size();
+  if (m % kFillChunk) return -2;
+  // the chunk invariant: no element written by one chunk is read by another
+  std::vector<int32_t> chunk_of((size_t)n * N2, -1);
+  for (size_t e = 0; e < m; e++)
+    if (ops[e].op != kFillNop) chunk_of[(size_t)ops[e].dst_gs * N2 + ops[e].dst_cell] = (int32_t)(e / kFillChunk);
+  const int off[4] = {0, NT, 1, NT + 1};
+  for (size_t e = 0; e < m; e++) {
+    const FillOp& op = ops[e];
+    std::vector<int64_t> refs;
+    if (op.op == kFillCopy) refs.push_back(src_ref(op.src, op.src_cell));
+    else if (op.op == kFillRestrict)
+      for (int t = 0; t < 4; t++) refs.push_back(src_ref(op.src, op.src_cell + off[t]));
+    else if (op.op == kFillGeneral)
+      for (int t = 0; t < 4; t++) refs.push_back(I.gen[op.src].ref[t]);
+    for (int64_t r : refs) {
+      const int32_t ch = chunk_of[(size_t)(r >> 16) * N2 + (r & 0xffff)];
+      if (ch >= 0 && ch != (int32_t)(e / kFillChunk)) return -3;
+    }
+  }
+  // emulate the device: per chunk, read every value, then write
+  std::vector<double> buf((size_t)kFillChunk * 8, 0.0);
+  for (size_t c0 = 0; c0 < m; c0 += kFillChunk) {
+    for (size_t e = c0; e < c0 + kFillChunk; e++)
+      if (ops[e].op != kFillNop)
+        for (int k = 0; k < nv; k++) buf[(e - c0) * 8 + k] = hf.value(ops[e], k);
+    for (size_t e = c0; e < c0 + kFillChunk; e++)
+      if (ops[e].op != kFillNop)
+        for (int k = 0; k < nv; k++) Q[((size_t)ops[e].dst_gs * nv + k) * N2 + ops[e].dst_cell] = buf[(e - c0) * 8 + k];
+  }
   if (nnodes) *nnodes = 0;
-  return (int)I.ops.size();
+  int live = 0;
+  for (size_t e = 0; e < m; e++) live += ops[e].op != kFillNop;
+  return live;
 }
 
 // Compiled fill program of the given tree: up to `cap` ops copied to `ops`
@@ -1806,6 +2119,31 @@ int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* i
   if (ops) std::memcpy(ops, I.ops.data(), sizeof(FillOp) * std::min(m, cap));
   if (ngen) *ngen = (int)I.gen.size();
   return m;
+}
+
+// Host cost of the schedule + fill-program compile for the given tree:
+// mean nanoseconds over `reps` compiles that reuse one scratch (as the
+// engine does).
+double amr_debug_compile_ns(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
+                            int reps) {
+  Instance I;
+  I.lastActive = n;
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.icoord[nb] = {icoord[2 * nb], icoord[2 * nb + 1]};
+    I.leafs[nb] = (uint8_t)leafs[nb];
+    I.gs.push_back(nb);
+  }
+  FillScratch S;
+  build_schedule(I);
+  if (compile_instance(I, NT, ng, S)) return -1;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < reps; r++) {
+    build_schedule(I);
+    compile_instance(I, NT, ng, S);
+  }
+  return std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / std::max(1, reps);
 }
 
 // coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]

@@ -43,20 +43,34 @@ __device__ inline void cons_to_prim(const double* u, double g, double* w) {
   w[4] = p - (g - 1) * (0.5 * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]));
 }
 
+// Kernel span from the device's constant-rate wall clock (100 MHz): span[0]
+// accumulates the complement of the earliest workgroup start, span[1] the
+// latest workgroup end (both start at zero; vector atomics).
+__device__ inline void stamp_start(unsigned long long* span) {
+  atomicMax(span, ~(unsigned long long)wall_clock64());
+}
+__device__ inline void stamp_end(unsigned long long* span) { atomicMax(span + 1, (unsigned long long)wall_clock64()); }
+
 // ------------------------------------------------------------------ stage
 // MUSCL2D.cpp:13-121 + the RK update of main.cpp:147-157 / 179-189 for one
 // leaf: primitives of the whole tile, edge states, normal-B override from the
 // faces, Riemann fluxes on [nxminb,nxmax]^2, then every cell of the tile is
 // advanced (the residual is zero outside [nxminb,nxmax]^2).
-template <int NT, int NG>
-__global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
-                                                         const InstParams* __restrict__ ip, int step,
-                                                         DevError* err) {
+//
+// Variants (same arithmetic, same results): TPB lanes per leaf, and the face
+// fluxes kept in LDS for the RK update (FLDS) or read back from the global
+// ff/gg the kernel has just written (smaller LDS footprint, more resident
+// workgroups per CU).
+template <int NT, int NG, int TPB, bool FLDS, int WPE>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
+    FieldPtrs F, const LeafItem* __restrict__ leaves, const InstParams* __restrict__ ip, int step, DevError* err,
+    unsigned long long* span) {
   constexpr int N2 = NT * NT;
   constexpr int LO = NG - 1, HI = NT - 1 - NG, NF = HI - LO + 1;
+  constexpr int NFL = FLDS ? 8 * NF * (NF + 1) : 1;
   __shared__ double sW[8 * N2];
-  __shared__ double sF[8 * NF * (NF + 1)];  // ff[i][j], i in [LO,HI+1], j in [LO,HI]
-  __shared__ double sG[8 * NF * (NF + 1)];  // gg[i][j], i in [LO,HI], j in [LO,HI+1]
+  __shared__ double sF[NFL];  // ff[i][j], i in [LO,HI+1], j in [LO,HI]
+  __shared__ double sG[NFL];  // gg[i][j], i in [LO,HI], j in [LO,HI+1]
   __shared__ int sNan;
 
   const LeafItem L = leaves[blockIdx.x];
@@ -66,13 +80,16 @@ __global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const Leaf
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
   const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
-  double* __restrict__ ff = F.f[kFF] + t8;
-  double* __restrict__ gg = F.f[kGG] + t8;
+  double* ff = F.f[kFF] + t8;
+  double* gg = F.f[kGG] + t8;
   const int tid = threadIdx.x;
-  if (tid == 0) sNan = 0;
+  if (tid == 0) {
+    sNan = 0;
+    if (span) stamp_start(span);
+  }
   __syncthreads();
 
-  for (int c = tid; c < N2; c += kThreads) {
+  for (int c = tid; c < N2; c += TPB) {
     double u[8], w[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) u[k] = Qin[k * N2 + c];
@@ -89,11 +106,13 @@ __global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const Leaf
       if (bad) sNan = 1;
     }
   }
-  // stale edge fluxes ff[LO][j], gg[i][LO] enter the residual of ghost cells
-  for (int q = tid; q < 8 * NF; q += kThreads) {
-    const int k = q / NF, r = q % NF;
-    sF[k * NF * (NF + 1) + r * (NF + 1)] = ff[k * N2 + (LO + r) * NT + LO];
-    sG[k * NF * (NF + 1) + 0 * NF + r] = gg[k * N2 + LO * NT + LO + r];
+  if (FLDS) {
+    // stale edge fluxes ff[LO][j], gg[i][LO] enter the residual of ghost cells
+    for (int q = tid; q < 8 * NF; q += TPB) {
+      const int k = q / NF, r = q % NF;
+      sF[k * NF * (NF + 1) + r * (NF + 1)] = ff[k * N2 + (LO + r) * NT + LO];
+      sG[k * NF * (NF + 1) + 0 * NF + r] = gg[k * N2 + LO * NT + LO + r];
+    }
   }
   __syncthreads();
   if (sNan) {
@@ -101,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const Leaf
     return;
   }
 
-  for (int q = tid; q < 2 * NF * NF; q += kThreads) {
+  for (int q = tid; q < 2 * NF * NF; q += TPB) {
     const int ydir = q >= NF * NF;
     const int r = ydir ? q - NF * NF : q;
     const int i = LO + r % NF, j = LO + r / NF;
@@ -145,38 +164,48 @@ __global__ __launch_bounds__(kThreads) void stage_kernel(FieldPtrs F, const Leaf
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         ff[k * N2 + c + 1] = fl[k];
-        sF[k * NF * (NF + 1) + (j - LO) * (NF + 1) + (i + 1 - LO)] = fl[k];
+        if (FLDS) sF[k * NF * (NF + 1) + (j - LO) * (NF + 1) + (i + 1 - LO)] = fl[k];
       }
     } else {
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         gg[k * N2 + c + NT] = fl[k];
-        sG[k * NF * (NF + 1) + (j + 1 - LO) * NF + (i - LO)] = fl[k];
+        if (FLDS) sG[k * NF * (NF + 1) + (j + 1 - LO) * NF + (i - LO)] = fl[k];
       }
     }
   }
-  __syncthreads();
+  __syncthreads();  // also orders this workgroup's ff/gg stores before the reads below
 
   const double* U = F.f[kU] + t8;
   double* Us = F.f[kUs] + t8;
   double* Uw = F.f[kU] + t8;
-  for (int c = tid; c < N2; c += kThreads) {
+  for (int c = tid; c < N2; c += TPB) {
     const int i = c % NT, j = c / NT;
     const bool in = i >= LO && i <= HI && j >= LO && j <= HI;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       double res = 0.;
       if (in) {
-        const double* Fk = sF + k * NF * (NF + 1);
-        const double* Gk = sG + k * NF * (NF + 1);
-        const int fi = (j - LO) * (NF + 1) + (i - LO);
-        const int gi = (j - LO) * NF + (i - LO);
-        res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+        if (FLDS) {
+          const double* Fk = sF + k * NF * (NF + 1);
+          const double* Gk = sG + k * NF * (NF + 1);
+          const int fi = (j - LO) * (NF + 1) + (i - LO);
+          const int gi = (j - LO) * NF + (i - LO);
+          res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+        } else {
+          const double* Fk = ff + k * N2;
+          const double* Gk = gg + k * N2;
+          res = (Fk[c + 1] - Fk[c]) / dxl + (Gk[c + NT] - Gk[c]) / dxl;
+        }
       }
       const int e = k * N2 + c;
       if (step == 1) Us[e] = U[e] - dt * res;
       else Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
     }
+  }
+  if (span) {
+    __syncthreads();
+    if (tid == 0) stamp_end(span);
   }
 }
 
@@ -242,9 +271,11 @@ __global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const Leaf
 }
 
 // -------------------------------------------------------------- ghost fill
-// Gather/scatter execution of a compiled fill program: every destination's
-// value is computed from the pre-fill state (plus nested restriction nodes,
-// evaluated level by level), then all destinations are written.
+// Single-pass execution of a compiled fill program.  Every destination's value
+// is a function of the pre-fill state only; a workgroup evaluates all values
+// of its chunk of ops, synchronises, then writes them.  The compiler places
+// each op that reads an element some other op writes in the same chunk as
+// that writer, so reads always see pre-fill values.
 struct FillCtx {
   const double* base;
   int nv, kind, N2, NT;
@@ -295,45 +326,49 @@ __device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
   return gen_value(f, f.gen[op.src], k);
 }
 
-// The program is a concatenation of per-instance segments: op e belongs to
-// the instance q with prefix[q] <= e < prefix[q+1].
+// The program is a concatenation of per-instance segments of whole chunks:
+// workgroup w runs chunk w - cprefix[q] of the instance q with
+// cprefix[q] <= w < cprefix[q+1].
 struct FillProgram {
   const FillOp* const* ops;
   const FillGen* const* gen;
-  const int32_t* prefix;
-  int ninst, nops;
+  const int32_t* cprefix;
+  int ninst;
 };
 
-__device__ inline int find_segment(const FillProgram& P, int e) {
+__device__ inline int find_segment(const FillProgram& P, int w) {
   int lo = 0, hi = P.ninst;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (P.prefix[mid] <= e) lo = mid;
+    if (P.cprefix[mid] <= w) lo = mid;
     else hi = mid;
   }
   return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void fill_gather_kernel(FillProgram P, FillSet set, double* buf, int NT) {
-  const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= P.nops) return;
-  const int q = find_segment(P, e);
-  const int fi = blockIdx.y;
-  FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, P.gen[q]};
-  const FillOp op = P.ops[q][e - P.prefix[q]];
-  for (int k = 0; k < f.nv; k++) buf[((size_t)fi * P.nops + e) * 8 + k] = fill_value(f, op, k);
+template <int NV>
+__device__ inline void fill_field(const FillCtx& f, double* base, const FillOp& op, bool live) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = live ? fill_value(f, op, k) : 0.;
+  __syncthreads();  // every read of the chunk precedes every write
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) base[((size_t)op.dst_gs * NV + k) * f.N2 + op.dst_cell] = v[k];
+  }
 }
 
-__global__ __launch_bounds__(kThreads) void fill_scatter_kernel(FillProgram P, FillSet set, const double* buf, int N2) {
-  const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= P.nops) return;
-  const int q = find_segment(P, e);
-  const int fi = blockIdx.y;
-  const FillOp op = P.ops[q][e - P.prefix[q]];
-  const int nv = set.nv[fi];
-  double* base = set.base[fi];
-  for (int k = 0; k < nv; k++)
-    base[((size_t)op.dst_gs * nv + k) * N2 + op.dst_cell] = buf[((size_t)fi * P.nops + e) * 8 + k];
+__global__ __launch_bounds__(kThreads) void fill_kernel(FillProgram P, FillSet set, int NT) {
+  const int w = blockIdx.x;
+  const int q = find_segment(P, w);
+  const FillOp op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + threadIdx.x];
+  const bool live = op.op != kFillNop;
+  for (int fi = 0; fi < set.n; fi++) {
+    const FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, P.gen[q]};
+    if (set.nv[fi] == 8) fill_field<8>(f, set.base[fi], op, live);
+    else if (set.nv[fi] == 2) fill_field<2>(f, set.base[fi], op, live);
+    else fill_field<1>(f, set.base[fi], op, live);
+  }
 }
 
 // ------------------------------------------------------ CFL and criteria
@@ -606,16 +641,34 @@ bool tile_supported(int NT, int ng) { return (NT == 20 && ng == 2) || (NT == 24 
     else if (NT == 12 && NG == 2) { constexpr int T = 12, G = 2; (void)G; CALL; }   \
   } while (0)
 
-void launch_stage(int NT, int ng, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip,
-                  int step, DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
-  if (nleaf <= 0) return;
+template <int T, int G, int TPB, bool FLDS, int WPE = 1>
+void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
+                          DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                          unsigned long long* span) {
   if (ev_start && ev_stop) {
     // events attached to the dispatch itself: they time the kernel, not the launch
-    AMR_DISPATCH_TILE(NT, ng, (hipExtLaunchKernelGGL((stage_kernel<T, G>), dim3(nleaf), dim3(kThreads), 0, s,
-                                                     ev_start, ev_stop, 0, F, leaves, ip, step, err)));
+    hipExtLaunchKernelGGL((stage_kernel<T, G, TPB, FLDS, WPE>), dim3(nleaf), dim3(TPB), 0, s, ev_start, ev_stop, 0,
+                          F, leaves, ip, step, err, span);
     return;
   }
-  AMR_DISPATCH_TILE(NT, ng, (stage_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, leaves, ip, step, err)));
+  stage_kernel<T, G, TPB, FLDS, WPE><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span);
+}
+
+void launch_stage(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
+                  const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
+                  hipEvent_t ev_stop, unsigned long long* span) {
+  if (nleaf <= 0) return;
+#define AMR_STAGE(TPB, FLDS, WPE)                                                                         \
+  AMR_DISPATCH_TILE(NT, ng, (launch_stage_variant<T, G, TPB, FLDS, WPE>(F, leaves, nleaf, ip, step, err, s, \
+                                                                        ev_start, ev_stop, span)))
+  switch (variant) {
+    case 1: AMR_STAGE(512, true, 1); break;
+    case 2: AMR_STAGE(768, true, 1); break;
+    case 3: AMR_STAGE(256, false, 4); break;
+    case 4: AMR_STAGE(512, false, 4); break;
+    default: AMR_STAGE(256, true, 1); break;
+  }
+#undef AMR_STAGE
 }
 
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s) {
@@ -637,13 +690,12 @@ void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
   AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, step)));
 }
 
-void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* prefix,
-                 int ninst, int nops, const FillSet& set, double* buf, hipStream_t s) {
-  if (nops <= 0 || set.n <= 0) return;
-  FillProgram P{inst_ops, inst_gen, prefix, ninst, nops};
-  dim3 grid((nops + kThreads - 1) / kThreads, set.n);
-  fill_gather_kernel<<<grid, kThreads, 0, s>>>(P, set, buf, NT);
-  fill_scatter_kernel<<<grid, kThreads, 0, s>>>(P, set, buf, NT * NT);
+void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* cprefix,
+                 int ninst, int nchunks, const FillSet& set, hipStream_t s) {
+  static_assert(kFillChunk == kThreads, "one workgroup per fill chunk");
+  if (nchunks <= 0 || set.n <= 0) return;
+  FillProgram P{inst_ops, inst_gen, cprefix, ninst};
+  fill_kernel<<<nchunks, kThreads, 0, s>>>(P, set, NT);
 }
 
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,

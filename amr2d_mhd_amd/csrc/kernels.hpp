@@ -10,7 +10,7 @@ struct FieldPtrs {
   double* f[kNumFields];
 };
 
-// up to four fields filled by one gather/scatter pass
+// up to four fields filled by one fill launch
 struct FillSet {
   double* base[4];
   int32_t nv[4];
@@ -29,18 +29,27 @@ struct RegridOp {
 
 bool tile_supported(int NT, int ng);
 
-void launch_stage(int NT, int ng, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip,
-                  int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
-                  hipEvent_t ev_stop = nullptr);
+// stage kernel variants (identical results): lanes per leaf block and where
+// the face fluxes live for the RK update --
+//   0: 256 lanes, fluxes in LDS      1: 512 lanes, LDS      2: 768 lanes, LDS
+//   3: 256 lanes, fluxes read back from ff/gg, compiled for 4 waves per SIMD
+//   4: 512 lanes, read back, 4 waves per SIMD
+// The fewer-lane variants suit large leaf counts (throughput), the wide ones
+// small leaf counts (per-block latency).  A non-null `span` receives the
+// launch's device wall-clock span (see stamp_start/stamp_end).
+constexpr int kStageVariants = 5;
+void launch_stage(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
+                  const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
+                  hipEvent_t ev_stop = nullptr, unsigned long long* span = nullptr);
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s);
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
                     hipStream_t s);
 void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int step, hipStream_t s);
 // ghost fill of up to four fields through a program made of per-instance
-// segments (device pointer tables inst_ops/inst_gen, prefix[ninst+1] of op
-// counts), gather from the pre-fill state into buf, then scatter
-void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* prefix,
-                 int ninst, int nops, const FillSet& set, double* buf, hipStream_t s);
+// segments of whole kFillChunk-op chunks (device pointer tables
+// inst_ops/inst_gen, cprefix[ninst+1] of chunk counts), one workgroup per chunk
+void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* cprefix,
+                 int ninst, int nchunks, const FillSet& set, hipStream_t s);
 // per-leaf CFL bound min(dx,dy)/lambda and refinement indicator; when W is
 // non-null it supplies the primitives (initial regrid) instead of U.
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,

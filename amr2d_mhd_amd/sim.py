@@ -140,6 +140,21 @@ class Simulation:
     # --------------------------------------------------------------- timing
     PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait")
 
+    def set_graph(self, on=True):
+        """Replay each step's device work as a HIP graph (default) or launch it directly."""
+        _check(self.h, lib().amr_set_graph(self.h, 1 if on else 0))
+
+    def set_stage_variant(self, variant):
+        """Select the stage-kernel variant (0-4, identical results; -1 = tuned per leaf-count bucket)."""
+        _check(self.h, lib().amr_set_stage_variant(self.h, int(variant)))
+
+    def stage_tuning(self, nbuckets=24):
+        """Tuner decisions: {bucket: (chosen variant, [mean us per leaf of each variant])}."""
+        chosen = np.zeros(nbuckets, np.int32)
+        us = np.zeros((nbuckets, 5))
+        _check(self.h, lib().amr_stage_tuning(self.h, nbuckets, chosen.ctypes.data, us.ctypes.data))
+        return {b: (int(chosen[b]), us[b].tolist()) for b in range(nbuckets) if chosen[b] >= 0 or us[b].any()}
+
     def profile(self, families=PROFILE_FAMILIES):
         """Start HIP-event timing of the given kernel families (resets totals)."""
         mask = 0

@@ -123,6 +123,27 @@ int amr_riemann_batch(int solver, int dir, double gamma, int n, const double *wL
 int amr_recon_batch(int NT, int nghosts, const char *limiter, double dx, int nt, const double *W, double *out,
                     void *stream);
 
+/* Graph replay of the step (no counterpart in the reference): with `on`
+ * (the default) amr_step/amr_run capture the device work of one step (the
+ * parameter upload, ghost fills, both RK stages, CT, the CFL/criteria kernel
+ * and the downloads that follow it) into a HIP graph once per topology and
+ * replay it; the graph is re-captured after every tree change.  Device-side
+ * kernel timing (amr_profile_enable bits 0-4) uses direct launches. */
+int amr_set_graph(amr_ctx *ctx, int on);
+
+/* Stage-kernel variant (no counterpart in the reference; all variants give
+ * identical results): lanes per leaf block and where the face fluxes live
+ * for the RK update -- 0: 256 lanes, fluxes in LDS; 1: 512 lanes, LDS;
+ * 2: 768 lanes, LDS; 3: 256 lanes, fluxes read back from ff/gg, 4 waves per
+ * SIMD; 4: 512 lanes, read back, 4 waves per SIMD.  -1 (the default) tunes
+ * per leaf-count bucket (bucket b holds 2^b <= nleaf < 2^(b+1)): the first
+ * stage launches in a bucket try every variant twice, timed by the kernel's
+ * device-clock span, and the fastest per leaf block is kept. */
+int amr_set_stage_variant(amr_ctx *ctx, int variant);
+/* Tuner state: chosen[b] (-1 = not decided) and the mean measured
+ * microseconds per leaf block of each variant, us_per_leaf[b][5]. */
+int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per_leaf);
+
 /* Kernel timing (no counterpart in the reference, which only keeps OpenMP
  * wall-clock accumulators, main.cpp:124-127): launches of the kernel families
  * selected by `mask` (bit 0 stage, 1 ghost fill, 2 CT, 3 CFL/criteria,
