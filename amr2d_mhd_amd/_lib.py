@@ -22,7 +22,8 @@ ERRORS = {0: "ok", 1: "non-positive pressure", 2: "NaN velocity", 3: "NaN flux",
 class SimConfig(C.Structure):
     _fields_ = [("nx", C.c_int32), ("nghosts", C.c_int32), ("maxlevs", C.c_int32), ("pad", C.c_int32),
                 ("lenx", C.c_double), ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16),
-                ("flux", C.c_char * 8), ("ic_params", C.c_double * 8)]
+                ("flux", C.c_char * 8), ("ic_params", C.c_double * 8), ("periodic", C.c_int32),
+                ("rk_stages", C.c_int32)]
 
 
 class InstInfo(C.Structure):
@@ -112,8 +113,10 @@ def lib():
 
 
 def make_config(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0,
-                ic_params=None):
+                ic_params=None, periodic=False, rk_stages=2):
     cfg = SimConfig()
+    cfg.periodic = 1 if periodic else 0
+    cfg.rk_stages = int(rk_stages)
     cfg.nx, cfg.nghosts, cfg.maxlevs, cfg.lenx, cfg.cfl = nx, nghosts, maxlevs, lenx, cfl
     cfg.ic, cfg.limiter, cfg.flux = ic.encode(), limiter.encode(), flux.encode()
     if ic_params is not None:

@@ -61,7 +61,7 @@ struct InstParams {
   int32_t limiter;
   int32_t solver;
   int32_t maxlevs;
-  int32_t pad;
+  int32_t rk;         // RK stages: 2 (timeIntegral.cpp) or 3 (SSP-RK3)
 };
 
 // Ghost-fill program.  A reference to a value is either a pre-fill element

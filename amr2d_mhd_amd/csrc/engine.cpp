@@ -47,9 +47,9 @@ int flux_code(const char* s) {
     if (!std::strcmp(s, names[q])) return q;
   return -1;
 }
-enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_COUNT };
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_COUNT };
 int ic_code(const char* s) {
-  static const char* names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p"};
+  static const char* names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p", "OT"};
   for (int q = 0; q < IC_COUNT; q++)
     if (!std::strcmp(s, names[q])) return q;
   return -1;
@@ -149,6 +149,7 @@ struct Instance {
   std::vector<std::array<int32_t, 3>> sched;
   double t = 0, dt_main = 0, dt_dom = 0;
   int count = 0, status = 0;
+  bool periodic = false;  // build-defined: wrapped root neighbours, domain [0, lenx]^2
   amr_diag err{};
   bool done = false;
   // ghost-fill program of this instance (global slot ids); recompiled only
@@ -174,6 +175,7 @@ enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_RE
 
 struct amr_ctx {
   int NT = 20, ng = 2, nxi = 16;
+  int rk = 2;  // RK stages of every instance: 2 (timeIntegral.cpp) or 3 (SSP-RK3, build-defined)
   std::vector<Instance> inst;
   FieldPtrs F{};
   int64_t cap_slots = 0, used_slots = 0;
@@ -370,14 +372,24 @@ void initial_tile(const Instance& I, int nb, int NT, int ng, std::vector<double>
     default: break;
   }
   const int lev = I.lp[nb][0];
-  const double dxl = I.dx[lev], dx0 = I.dx[0];
+  // cell centres (IC2Dtype.cpp:92-94); the periodic geometry starts at 0
+  const double dxl = I.dx[lev], dx0 = I.periodic ? 0. : I.dx[0];
   const double* rp = I.cfg.ic_params;
   for (int i = 0; i < NT; i++) {
     const double x = ((i + I.icoord[nb][0] - ng) + 0.5) * dxl - dx0;
     for (int j = 0; j < NT; j++) {
       const double y = ((j + I.icoord[nb][1] - ng) + 0.5) * dxl - dx0;
       double w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (I.ic == IC_ROTOR) {
+      if (I.ic == IC_OT) {
+        // build-defined Orszag-Tang vortex on [0,1]^2
+        const double pi = 3.141592653589793, B0 = 1. / sqrt(4. * pi);
+        w[0] = 25. / (36. * pi);
+        w[1] = -sin(2. * pi * y);
+        w[2] = sin(2. * pi * x);
+        w[4] = 5. / (12. * pi);
+        w[5] = -B0 * sin(2. * pi * y);
+        w[6] = B0 * sin(4. * pi * x);
+      } else if (I.ic == IC_ROTOR) {
         const double rad = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
         const double fr = (23. - 200. * rad) / 3.;
         w[4] = 0.5;
@@ -1162,6 +1174,7 @@ void fill_params(amr_ctx* c) {
     P.limiter = I.lim;
     P.solver = I.solver;
     P.maxlevs = I.maxlevs;
+    P.rk = c->rk;
   }
 }
 
@@ -1362,12 +1375,12 @@ void run_ct(amr_ctx* c, int step) {
   run_fill(c, make_set(c, {{kEMF, kFillEdge}}));
   {
     Timed tm(c, PF_CT);
-    launch_bupdate(c->NT, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->stream);
+    launch_bupdate(c->NT, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->rk, c->stream);
   }
-  run_fill(c, make_set(c, {{step == 1 ? kBis : kBi, kFillFace}}));
+  run_fill(c, make_set(c, {{step == c->rk ? kBi : kBis, kFillFace}}));
   {
     Timed tm(c, PF_CT);
-    launch_bsync(c->NT, c->F, c->d_leaves, c->nleaf, step, c->stream);
+    launch_bsync(c->NT, c->F, c->d_leaves, c->nleaf, step == c->rk, c->stream);
   }
 }
 
@@ -1492,6 +1505,12 @@ int enqueue_step(amr_ctx* c) {
   run_stage(c, 2);
   run_fill(c, make_set(c, {{kFF, kFillFace}, {kGG, kFillFace}}));
   run_ct(c, 2);
+  if (c->rk == 3) {
+    run_fill(c, make_set(c, {{kUs, kFillCell}}));
+    run_stage(c, 3);
+    run_fill(c, make_set(c, {{kFF, kFillFace}, {kGG, kFillFace}}));
+    run_ct(c, 3);
+  }
   run_fill(c, make_set(c, {{kU, kFillCell}}));
   {
     Timed tm(c, PF_CFL);
@@ -1634,13 +1653,23 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     const double p = std::pow(kNbRoots, g.maxlevs - 1);
     I.maxblocks = kNbRoots > p ? kNbRoots : (int)(std::pow(4, g.maxlevs) * 2);
     total_mb += I.maxblocks;
-    // meshblock.cpp:90-96
-    I.dx[0] = g.lenx / (std::sqrt((double)kNbRoots) * (c->NT - 2 * c->ng - 1));
+    // meshblock.cpp:90-96; a periodic domain needs its exact length, so it
+    // uses lenx / (sqrt(nbroots) * nx) instead
+    I.periodic = g.periodic != 0;
+    I.dx[0] = g.lenx / (std::sqrt((double)kNbRoots) * (c->NT - 2 * c->ng - (I.periodic ? 0 : 1)));
     for (int l = 1; l < kMaxLev; l++) I.dx[l] = I.dx[l - 1] / 2.;
     // IC2Dtype.cpp:18-80 (gamma, tEnd per problem)
     if (I.ic == IC_ROTOR) { I.gamma = 1.4; I.tEnd = 0.295; }
     else if (I.ic == IC_ROTOR_P) { I.gamma = g.ic_params[6]; I.tEnd = g.ic_params[7]; }
+    else if (I.ic == IC_OT) { I.gamma = 5. / 3.; I.tEnd = 0.5; }
     else { I.gamma = 2.0; I.tEnd = 0.2; }
+    // all instances of a context share the RK scheme (one step sequence)
+    const int rk = g.rk_stages == 0 ? 2 : g.rk_stages;
+    if ((rk != 2 && rk != 3) || (q > 0 && rk != c->rk)) {
+      delete c;
+      return AMR_ERR_CONFIG;
+    }
+    c->rk = rk;
   }
   c->cap_slots = slot_capacity > 0 ? slot_capacity : total_mb;
   const size_t N2 = (size_t)c->NT * c->NT;
@@ -1717,13 +1746,16 @@ int amr_init(amr_ctx* c) {
   c->call_fail = 0;
   static const int32_t root_lp[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
                                         {0, 0, 0, 0, -1, 3, 0, -1}, {0, 0, 0, 0, 2, -1, 1, -1}};
+  // periodic roots: the outer neighbour pointers wrap instead of -1
+  static const int32_t root_lpp[4][8] = {{0, 0, 0, 0, 1, 1, 2, 2}, {0, 0, 0, 0, 0, 0, 3, 3},
+                                         {0, 0, 0, 0, 3, 3, 0, 0}, {0, 0, 0, 0, 2, 2, 1, 1}};
   const int NT = c->NT, N2 = NT * NT, side = NT - 2 * c->ng;
   std::vector<double> Wall;
   for (size_t q = 0; q < c->inst.size(); q++) {
     Instance& I = c->inst[q];
     grow_tree(I, kNbRoots);
     for (int r = 0; r < kNbRoots; r++) {
-      for (int s = 0; s < 8; s++) I.lp[r][s] = root_lp[r][s];
+      for (int s = 0; s < 8; s++) I.lp[r][s] = I.periodic ? root_lpp[r][s] : root_lp[r][s];
       I.icoord[r] = {(r & 1) ? side : 0, (r & 2) ? side : 0};
       I.leafs[r] = 1;
       I.iref[r] = 1;
@@ -1882,7 +1914,7 @@ int amr_fill(amr_ctx* c, int field, int kind) {
 }
 
 int amr_stage(amr_ctx* c, int step) {
-  if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
@@ -1892,7 +1924,7 @@ int amr_stage(amr_ctx* c, int step) {
 }
 
 int amr_ct(amr_ctx* c, int step) {
-  if (!c || (step != 1 && step != 2)) return AMR_ERR_CONFIG;
+  if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;

@@ -200,7 +200,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
       }
       const int e = k * N2 + c;
       if (step == 1) Us[e] = U[e] - dt * res;
-      else Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
+      else if (P.rk == 2) Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
+      // SSP-RK3 (Shu-Osher), build-defined
+      else if (step == 2) Us[e] = 0.75 * U[e] + 0.25 * (Us[e] - dt * res);
+      else Uw[e] = (1. / 3.) * U[e] + (2. / 3.) * (Us[e] - dt * res);
     }
   }
   if (span) {
@@ -224,10 +227,19 @@ __global__ __launch_bounds__(kThreads) void emf_kernel(FieldPtrs F, const LeafIt
   F.f[kEMF][(size_t)L.gs * N2 + c] = 0.25 * ((gg5[c - 1] + gg5[c]) - (ff6[c - NT] + ff6[c]));
 }
 
-// CT2D.cpp:78-92: face-B RK update from the edge EMF
+// CT2D.cpp:78-92: face-B RK update from the edge EMF.  Stage input B (Bi for
+// step 1, else Bis), written array Bn (Bi on the last stage, else Bis) and
+// the step's starting value B0 = Bi; SSP-RK3 stages 2-3 are build-defined.
+__device__ inline double face_rk(int rk, int step, double B, double Bn, double B0, double dt, double res) {
+  if (step == 1) return B - dt * res;
+  if (rk == 2) return 0.5 * (B + Bn - dt * res);
+  if (step == 2) return 0.75 * B0 + 0.25 * (B - dt * res);
+  return (1. / 3.) * B0 + (2. / 3.) * (B - dt * res);
+}
+
 template <int NT>
 __global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
-                                                           const InstParams* __restrict__ ip, int step) {
+                                                           const InstParams* __restrict__ ip, int step, int rk) {
   constexpr int N2 = NT * NT;
   const LeafItem L = leaves[blockIdx.y];
   const int c = blockIdx.x * kThreads + threadIdx.x;
@@ -237,30 +249,28 @@ __global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const Le
   const double dt = P.dt, dxl = P.dx[L.lev];
   const double* E = F.f[kEMF] + (size_t)L.gs * N2;
   const double* B = F.f[step == 1 ? kBi : kBis] + (size_t)L.gs * 2 * N2;
-  double* Bn = F.f[step == 1 ? kBis : kBi] + (size_t)L.gs * 2 * N2;
+  const double* B0 = F.f[kBi] + (size_t)L.gs * 2 * N2;
+  double* Bn = F.f[step == rk ? kBi : kBis] + (size_t)L.gs * 2 * N2;
   const int cjp = j < NT - 1 ? c + NT : c;
   const int cip = i < NT - 1 ? c + 1 : c;
   const double rx = (1. / dxl) * (E[cjp] - E[c]);
   const double ry = -(1. / dxl) * (E[cip] - E[c]);
-  if (step == 1) {
-    Bn[c] = B[c] - dt * rx;
-    Bn[N2 + c] = B[N2 + c] - dt * ry;
-  } else {
-    Bn[c] = 0.5 * (B[c] + Bn[c] - dt * rx);
-    Bn[N2 + c] = 0.5 * (B[N2 + c] + Bn[N2 + c] - dt * ry);
-  }
+  const double bx = face_rk(rk, step, B[c], Bn[c], B0[c], dt, rx);
+  const double by = face_rk(rk, step, B[N2 + c], Bn[N2 + c], B0[N2 + c], dt, ry);
+  Bn[c] = bx;
+  Bn[N2 + c] = by;
 }
 
 // CT2D.cpp:95-123: cell-centred B from the updated faces + energy correction
 template <int NT>
-__global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves, int step) {
+__global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves, int last) {
   constexpr int N2 = NT * NT;
   const LeafItem L = leaves[blockIdx.y];
   const int c = blockIdx.x * kThreads + threadIdx.x;
   if (c >= N2) return;
   const int i = c % NT, j = c / NT;
-  const double* Bn = F.f[step == 1 ? kBis : kBi] + (size_t)L.gs * 2 * N2;
-  double* q = F.f[step == 1 ? kUs : kU] + (size_t)L.gs * 8 * N2;
+  const double* Bn = F.f[last ? kBi : kBis] + (size_t)L.gs * 2 * N2;
+  double* q = F.f[last ? kU : kUs] + (size_t)L.gs * 8 * N2;
   const int ip = i < NT - 1 ? i + 1 : i, jp = j < NT - 1 ? j + 1 : j;
   const double bx = 0.5 * (Bn[j * NT + ip] + Bn[j * NT + ip - 1]);
   const double by = 0.5 * (Bn[N2 + jp * NT + i] + Bn[N2 + (jp - 1) * NT + i]);
@@ -678,16 +688,16 @@ void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, h
 }
 
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
-                    hipStream_t s) {
+                    int rk, hipStream_t s) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bupdate_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bupdate_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step, rk)));
 }
 
-void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int step, hipStream_t s) {
+void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int last, hipStream_t s) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, step)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, last)));
 }
 
 void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* cprefix,

@@ -42,9 +42,11 @@ void launch_stage(int NT, int ng, int variant, const FieldPtrs& F, const LeafIte
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
                   hipEvent_t ev_stop = nullptr, unsigned long long* span = nullptr);
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s);
+// face-B update of RK stage `step` of an `rk`-stage scheme; cell sync into U
+// (last stage) or Us
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
-                    hipStream_t s);
-void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int step, hipStream_t s);
+                    int rk, hipStream_t s);
+void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int last, hipStream_t s);
 // ghost fill of up to four fields through a program made of per-instance
 // segments of whole kFillChunk-op chunks (device pointer tables
 // inst_ops/inst_gen, cprefix[ninst+1] of chunk counts), one workgroup per chunk

@@ -46,10 +46,14 @@ typedef struct {
   int32_t pad;
   double lenx;         /* domain length parameter (lenx = leny) */
   double cfl;
-  char ic[16];         /* SST SST1 SSTy BWx BWx1 BWy BWy1 rotor | rotor_p (parameterised rotor) */
+  char ic[16];         /* SST SST1 SSTy BWx BWx1 BWy BWy1 rotor | rotor_p (parameterised rotor), OT */
   char limiter[16];    /* 1st noSL minmod VanLeer Superbee MC Koren Cada3rd WENO3 WENO5 */
   char flux[8];        /* RUSA HLLE HLLC HLLD */
   double ic_params[8]; /* rotor_p: rho_in, v0, p, Bx, r0, r1, gamma, t_end */
+  int32_t periodic;    /* 0: the reference's mirror boundaries (extboundary.cpp); 1: periodic
+                          domain [0, lenx]^2 with exact cell size lenx/(2 nx) (build-defined) */
+  int32_t rk_stages;   /* 0 or 2: RK2 (timeIntegral.cpp); 3: SSP-RK3 (build-defined); one value
+                          per context */
 } amr_sim_config;
 
 typedef struct {
@@ -99,8 +103,9 @@ int amr_set_time(amr_ctx *ctx, int inst, double t, double dt, double dt_next, in
 
 /* Individual phases of one step, for phase-level parity checks.
  *  amr_fill:  boundary.cpp:7 (kind 0), boundaryfc.cpp:8 (1), boundaryec.cpp:7 (2)
- *  amr_stage: MUSCL2D.cpp:13 + RK2ndv1 (timeIntegral.cpp:3) over the leaves
- *  amr_ct:    CT2D.cpp:11
+ *  amr_stage: MUSCL2D.cpp:13 + RK2ndv1 (timeIntegral.cpp:3) over the leaves;
+ *             step 1..rk_stages (SSP-RK3 stages 2-3 are build-defined)
+ *  amr_ct:    CT2D.cpp:11, same step numbering
  *  amr_cfl:   main.cpp:209-230, *lambda = min over leaves
  *  amr_regrid: admesh.cpp:17 */
 int amr_fill(amr_ctx *ctx, int field, int kind);

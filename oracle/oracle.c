@@ -31,7 +31,7 @@ static inline double stdmax(double a, double b) { return (a < b) ? b : a; }
 
 enum { L_1ST, L_NOSL, L_MINMOD, L_VANLEER, L_SUPERBEE, L_MC, L_KOREN, L_CADA, L_WENO3, L_WENO5, L_BAD };
 enum { F_RUSA, F_HLLE, F_HLLC, F_HLLD, F_BAD };
-enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_BAD };
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_BAD };
 
 static int lim_id(const char *s) {
   if (!strcmp(s, "1st")) return L_1ST;
@@ -54,8 +54,8 @@ static int flux_id(const char *s) {
   return F_BAD;
 }
 static int ic_id(const char *s) {
-  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p"};
-  for (int q = 0; q < 9; q++)
+  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p", "OT"};
+  for (int q = 0; q < 10; q++)
     if (!strcmp(s, names[q])) return q;
   return IC_BAD;
 }
@@ -80,6 +80,8 @@ struct ora_dom {
   double *scr; /* dwdx dwdy wxL wxR wyL wyR, each [8][NT][NT] */
   int32_t err[4];
   double icp[8];
+  int periodic; /* build-defined: wrapped root neighbours, exact geometry */
+  int rk;       /* 2 (timeIntegral.cpp) or 3 (build-defined SSP-RK3) */
 };
 
 #define TILE(d, fld, nb) ((d)->f[fld] + (size_t)(nb) * NVAR[fld] * (d)->NT * (d)->NT)
@@ -588,10 +590,15 @@ ora_dom *ora_create(const ora_config *c) {
   d->flux = flux_id(c->flux);
   d->ic = ic_id(c->ic);
   memcpy(d->icp, c->ic_params, sizeof d->icp);
+  d->periodic = c->periodic != 0;
+  d->rk = c->rk_stages == 3 ? 3 : 2;
+  if (c->rk_stages != 0 && c->rk_stages != 2 && c->rk_stages != 3) { free(d); return NULL; }
   if (d->lim == L_BAD || d->flux == F_BAD || d->ic == IC_BAD) { free(d); return NULL; }
-  /* meshblock.cpp:90-96 */
-  d->dx[0] = c->lenx / (sqrt(NBROOTS) * (d->NT - 2 * d->ng - 1));
-  d->dy[0] = c->lenx / (sqrt(NBROOTS) * (d->NT - 2 * d->ng - 1));
+  /* meshblock.cpp:90-96; a periodic domain needs its exact length, so it
+     uses lenx / (sqrt(nbroots) * nx) instead (build-defined) */
+  const int cells = d->periodic ? d->NT - 2 * d->ng : d->NT - 2 * d->ng - 1;
+  d->dx[0] = c->lenx / (sqrt(NBROOTS) * cells);
+  d->dy[0] = c->lenx / (sqrt(NBROOTS) * cells);
   for (int l = 1; l < d->maxlevs; l++) { d->dx[l] = d->dx[l - 1] / 2.; d->dy[l] = d->dy[l - 1] / 2.; }
   size_t mb = (size_t)d->maxblocks;
   d->lp = calloc(mb, sizeof *d->lp);
@@ -635,15 +642,28 @@ static void initial_condition(ora_dom *d, int nb) {
       SET4(by4, 0.75, 0.75, 0.75, 0.75); SET4(bx4, 1.0, 1.0, -1.0, -1.0); break;
     case IC_ROTOR: gamma = 1.4; tEnd = 0.295; p4[0] = 0.5; bx4[0] = 2.5 / (sqrt(4 * PI_REF)); break;
     case IC_ROTOR_P: gamma = d->icp[6]; tEnd = d->icp[7]; break;
+    case IC_OT: gamma = 5. / 3.; tEnd = 0.5; break;
   }
 #undef SET4
   double *W = TILE(d, ORA_W, nb);
   double dxl = d->dx[d->lp[nb][0]], dyl = d->dy[d->lp[nb][0]];
+  /* cell centres (IC2Dtype.cpp:92-94); the periodic geometry starts at 0 */
+  const double x0 = d->periodic ? 0. : d->dx[0], y0 = d->periodic ? 0. : d->dy[0];
   for (int i = 0; i < NT; i++) {
-    double x = ((i + d->icoord[nb][0] - ng) + 0.5) * dxl - d->dx[0];
+    double x = ((i + d->icoord[nb][0] - ng) + 0.5) * dxl - x0;
     for (int j = 0; j < NT; j++) {
-      double y = ((j + d->icoord[nb][1] - ng) + 0.5) * dyl - d->dy[0];
-      if (d->ic == IC_ROTOR) {
+      double y = ((j + d->icoord[nb][1] - ng) + 0.5) * dyl - y0;
+      if (d->ic == IC_OT) {
+        /* build-defined Orszag-Tang vortex on [0,1]^2 */
+        const double pi = 3.141592653589793, B0 = 1. / sqrt(4. * pi);
+        for (int k = 0; k < 8; k++) AT(W, NT, k, i, j) = 0.;
+        AT(W, NT, 0, i, j) = 25. / (36. * pi);
+        AT(W, NT, 1, i, j) = -sin(2. * pi * y);
+        AT(W, NT, 2, i, j) = sin(2. * pi * x);
+        AT(W, NT, 4, i, j) = 5. / (12. * pi);
+        AT(W, NT, 5, i, j) = -B0 * sin(2. * pi * y);
+        AT(W, NT, 6, i, j) = B0 * sin(4. * pi * x);
+      } else if (d->ic == IC_ROTOR) {
         double radius = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
         double fr = (23. - 200. * radius) / 3.;
         for (int k = 0; k < 8; k++) AT(W, NT, k, i, j) = 0.;
@@ -705,13 +725,16 @@ static void initial_condition(ora_dom *d, int nb) {
     }
 }
 
-/* basegrid.cpp:4-89: 4 roots, IDs 0|1 bottom, 2|3 top */
+/* basegrid.cpp:4-89: 4 roots, IDs 0|1 bottom, 2|3 top; a periodic domain
+   wraps the outer neighbour pointers instead of -1 (build-defined) */
 static void basegrid(ora_dom *d) {
   static const int32_t lp0[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
                                     {0, 0, 0, 0, -1, 3, 0, -1}, {0, 0, 0, 0, 2, -1, 1, -1}};
+  static const int32_t lpp[4][8] = {{0, 0, 0, 0, 1, 1, 2, 2}, {0, 0, 0, 0, 0, 0, 3, 3},
+                                    {0, 0, 0, 0, 3, 3, 0, 0}, {0, 0, 0, 0, 2, 2, 1, 1}};
   int side = d->NT - 2 * d->ng;
   for (int r = 0; r < 4; r++) {
-    memcpy(d->lp[r], lp0[r], sizeof lp0[r]);
+    memcpy(d->lp[r], d->periodic ? lpp[r] : lp0[r], sizeof lp0[r]);
     d->icoord[r][0] = (r & 1) ? side : 0;
     d->icoord[r][1] = (r & 2) ? side : 0;
     d->leafs[r] = 1;
@@ -1094,17 +1117,32 @@ int ora_stage(ora_dom *d, int step) {
           else
             res = 0.;
           if (step == 1) AT(Us, NT, k, i, j) = AT(U, NT, k, i, j) - dt * res;
-          else AT(U, NT, k, i, j) = 0.5 * (AT(U, NT, k, i, j) + AT(Us, NT, k, i, j) - dt * res);
+          else if (d->rk == 2) AT(U, NT, k, i, j) = 0.5 * (AT(U, NT, k, i, j) + AT(Us, NT, k, i, j) - dt * res);
+          /* SSP-RK3 (Shu-Osher), build-defined */
+          else if (step == 2) AT(Us, NT, k, i, j) = 0.75 * AT(U, NT, k, i, j) + 0.25 * (AT(Us, NT, k, i, j) - dt * res);
+          else AT(U, NT, k, i, j) = (1. / 3.) * AT(U, NT, k, i, j) + (2. / 3.) * (AT(Us, NT, k, i, j) - dt * res);
         }
   }
   return 0;
+}
+
+/* face-B stage update (CT2D.cpp:78-92) from the stage input B, the array
+ * being written Bn and the step's starting value B0 = Bi; SSP-RK3 stages 2
+ * and 3 are build-defined */
+static double face_rk(int rk, int step, double B, double Bn, double B0, double dt, double res) {
+  if (step == 1) return B - dt * res;
+  if (rk == 2) return 0.5 * (B + Bn - dt * res);
+  if (step == 2) return 0.75 * B0 + 0.25 * (B - dt * res);
+  return (1. / 3.) * B0 + (2. / 3.) * (B - dt * res);
 }
 
 /* CT2D.cpp:11-124 (CTtype 1): corner EMF, edge fill, face-B update, face
  * fill, then cell-centred B from faces with the energy correction. */
 int ora_ct(ora_dom *d, int step) {
   const int NT = d->NT;
-  const int qf = step == 1 ? ORA_US : ORA_U, bf = step == 1 ? ORA_BI : ORA_BIS, nf = step == 1 ? ORA_BIS : ORA_BI;
+  /* last stage updates Bi and U; earlier stages Bis and Us */
+  const int last = step == d->rk;
+  const int qf = last ? ORA_U : ORA_US, bf = step == 1 ? ORA_BI : ORA_BIS, nf = last ? ORA_BI : ORA_BIS;
   for (int nb = 0; nb < d->lastActive; nb++) {
     if (!d->leafs[nb]) continue;
     double *E = TILE(d, ORA_EMF, nb), *ff = TILE(d, ORA_FF, nb), *gg = TILE(d, ORA_GG, nb);
@@ -1123,11 +1161,11 @@ int ora_ct(ora_dom *d, int step) {
       for (int j = 0; j < NT; j++) {
         int ip = i < NT - 1 ? i + 1 : i, jp = j < NT - 1 ? j + 1 : j;
         double res = (1. / d->dy[lev]) * (AT(E, NT, 0, i, jp) - AT(E, NT, 0, i, j));
-        AT(Bn, NT, 0, i, j) = step == 1 ? AT(B, NT, 0, i, j) - dt * res
-                                        : 0.5 * (AT(B, NT, 0, i, j) + AT(Bn, NT, 0, i, j) - dt * res);
+        AT(Bn, NT, 0, i, j) = face_rk(d->rk, step, AT(B, NT, 0, i, j), AT(Bn, NT, 0, i, j), TILE(d, ORA_BI, nb)[i + NT * j],
+                                      dt, res);
         res = -(1. / d->dx[lev]) * (AT(E, NT, 0, ip, j) - AT(E, NT, 0, i, j));
-        AT(Bn, NT, 1, i, j) = step == 1 ? AT(B, NT, 1, i, j) - dt * res
-                                        : 0.5 * (AT(B, NT, 1, i, j) + AT(Bn, NT, 1, i, j) - dt * res);
+        AT(Bn, NT, 1, i, j) = face_rk(d->rk, step, AT(B, NT, 1, i, j), AT(Bn, NT, 1, i, j),
+                                      TILE(d, ORA_BI, nb)[NT * NT + i + NT * j], dt, res);
       }
   }
   ora_fill(d, nf, ORA_FILL_FC);
@@ -1651,6 +1689,13 @@ int ora_step(ora_dom *d) {
   ora_fill(d, ORA_FF, ORA_FILL_FC);
   ora_fill(d, ORA_GG, ORA_FILL_FC);
   ora_ct(d, 2);
+  if (d->rk == 3) {
+    ora_fill(d, ORA_US, ORA_FILL_CC);
+    if ((st = ora_stage(d, 3))) return st;
+    ora_fill(d, ORA_FF, ORA_FILL_FC);
+    ora_fill(d, ORA_GG, ORA_FILL_FC);
+    ora_ct(d, 3);
+  }
   ora_fill(d, ORA_U, ORA_FILL_CC);
   double lambda;
   if ((st = ora_cfl(d, &lambda))) return st;

@@ -43,11 +43,13 @@ typedef struct {
   int maxlevs;     /* defs.hpp:28 */
   double lenx;     /* input.txt lenx (= leny) */
   double cfl;      /* input.txt CFL */
-  char ic[16];     /* IC2Dtype.cpp names: BWx BWx1 BWy BWy1 SST SST1 SSTy rotor */
+  char ic[16];     /* IC2Dtype.cpp names: BWx BWx1 BWy BWy1 SST SST1 SSTy rotor; rotor_p, OT */
   char limiter[16];/* slopelimiter.cpp names, WENO3, WENO5 */
   char flux[8];    /* RUSA HLLE HLLC HLLD */
   double ic_params[8]; /* rotor_p (build-defined, no reference counterpart):
                           rho_in, v0, p, Bx, r0, r1, gamma, t_end */
+  int periodic;        /* build-defined: wrapped root neighbours, domain [0,lenx]^2 */
+  int rk_stages;       /* 0 or 2: timeIntegral.cpp RK2; 3: SSP-RK3 (build-defined) */
 } ora_config;
 
 typedef struct ora_dom ora_dom;
@@ -75,7 +77,7 @@ void ora_set_time(ora_dom *d, double t, double dom_dt, double dt_next, int count
 /* individual phases (for phase-level parity) */
 int ora_locate_bounds(ora_dom *d);
 int ora_fill(ora_dom *d, int field, int kind);  /* boundary / boundaryfc / boundaryec */
-int ora_stage(ora_dom *d, int step);            /* MUSCL2D + RK over all leaves */
+int ora_stage(ora_dom *d, int step);            /* MUSCL2D + RK over all leaves (step 1..rk) */
 int ora_ct(ora_dom *d, int step);               /* CT2D */
 int ora_cfl(ora_dom *d, double *lambda);        /* main.cpp:209-230 */
 int ora_regrid(ora_dom *d);                     /* admesh */

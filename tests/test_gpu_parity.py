@@ -208,3 +208,32 @@ def test_stage_variant_bitwise(variant):
     assert dom.init() == 0
     sim.init()
     lockstep(sim, dom, 40, full_every=10, label=f"stage variant {variant}")
+
+
+@pytest.mark.parametrize("lim,flux,rk,maxlevs,ng", [("WENO5", "HLLD", 3, 4, 4), ("MC", "HLLC", 2, 3, 2),
+                                                    ("minmod", "HLLE", 3, 4, 2)])
+def test_orszag_tang_periodic_bitwise(lim, flux, rk, maxlevs, ng):
+    """Orszag-Tang on the periodic domain (build-defined), RK2 and SSP-RK3."""
+    import amr2d_mhd_amd as A
+
+    cfg = dict(ic="OT", limiter=lim, flux=flux, cfl=0.4, nx=16, nghosts=ng, maxlevs=maxlevs, periodic=True,
+               rk_stages=rk)
+    sim = A.Simulation(cfg)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "initial state differs: " + "; ".join(d[:4])
+    lockstep(sim, dom, 40, full_every=10, label=f"OT {lim} {flux} rk{rk}")
+
+
+def test_rk3_mirror_boundaries_bitwise():
+    """SSP-RK3 with the reference's mirror boundaries (Brio-Wu, 3 levels)."""
+    import amr2d_mhd_amd as A
+
+    cfg = dict(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=3, rk_stages=3)
+    sim = A.Simulation(cfg)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    lockstep(sim, dom, 40, full_every=10, label="BWx rk3")
