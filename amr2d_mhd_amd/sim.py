@@ -61,8 +61,15 @@ class Simulation:
         return n.value
 
     def run(self, max_steps=-1):
+        """Advance up to max_steps steps (-1: until every instance is done);
+        returns (steps, zone-updates).  On failure the raised AmrError carries
+        the steps and zone-updates completed before it."""
         steps, zu = C.c_int64(0), C.c_int64(0)
-        _check(self.h, lib().amr_run(self.h, int(max_steps), C.byref(steps), C.byref(zu)))
+        try:
+            _check(self.h, lib().amr_run(self.h, int(max_steps), C.byref(steps), C.byref(zu)))
+        except AmrError as e:
+            e.steps, e.zone_updates = steps.value, zu.value
+            raise
         return steps.value, zu.value
 
     def info(self, inst=0):

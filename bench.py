@@ -106,13 +106,18 @@ def ensemble_bench(steps, full, n_inst, plain=True, warmup=0):
     batch.close()
     tp, zp = 0.0, 0
     for q, cfg in enumerate(cfgs if plain and not warmup else []):
-        if q in failed:
-            continue  # an aborted instance's partial step is not a zone-update in either leg
         sim = A.Simulation(cfg)
-        sim.init()
+        try:
+            sim.init()
+        except A.AmrError:
+            sim.close()
+            continue  # aborted in the initial regrid: no zone-updates in either leg
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, z = sim.run(steps)
+        try:
+            _, z = sim.run(steps)
+        except A.AmrError as e:  # the instance aborts alone, as in the batch
+            z = e.zone_updates
         torch.cuda.synchronize()
         tp += time.perf_counter() - t0
         zp += z
@@ -140,6 +145,115 @@ def ensemble_bench(steps, full, n_inst, plain=True, warmup=0):
     return out
 
 
+OT = dict(ic="OT", limiter="WENO5", flux="HLLD", cfl=0.4, nx=16, nghosts=4, maxlevs=6, lenx=1.0, periodic=True,
+          rk_stages=3)
+
+
+def ot_bench(steps, warmup):
+    """Orszag-Tang vortex (build-defined: periodic [0,1]^2, 4 roots of 16x16,
+    6 levels, WENO5 + HLLD, SSP-RK3, CT, gamma 5/3): `warmup` untimed steps
+    from t = 0, then `steps` timed steps (a negative count runs to t = 0.5)."""
+    import torch
+
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(OT)
+    sim.init()
+    if warmup:
+        sim.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n, zu = sim.run(steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    info = sim.info()
+    return {"workload": "Orszag-Tang, periodic, 4 roots of 16x16, 6 levels, WENO5 + HLLD, SSP-RK3, CT, CFL 0.4",
+            "warmup": warmup, "steps": n, "zone_updates": zu, "seconds": secs, "value": zu / secs if secs else None,
+            "unit": "zone-updates/s", "t_reached": info["t"], "leaves_at_end": info["nleaf"]}
+
+
+def adjoint_bench(n, steps, every, fd_dirs):
+    """Frozen-mesh adjoint (build-defined workload): Orszag-Tang controls on a
+    uniform periodic n x n grid (smoothly modulated and seeded-noise density,
+    pressure and velocity, so no limiter evaluation starts on a kink),
+    minmod-MUSCL + HLLE, CT, RK2, `steps` steps at a dt sequence frozen from
+    the forward run, checkpoints every `every` steps.  J = sum (rho_T -
+    rho_target)^2, target = the same set-up with the velocity scaled by 1.05.
+    Reports the forward and backward cost per step and, over `fd_dirs` seeded
+    random directions, the relative gap between the adjoint directional
+    derivative and central finite differences at eps 1e-6 and 1e-8."""
+    import torch
+
+    from amr2d_mhd_amd import adjoint as AD
+
+    g = 5.0 / 3.0
+    mesh = AD.FrozenMesh(n, 1.0 / n, g)
+    ctrl = AD.orszag_tang_controls(n, modulation=0.05, noise=1e-3)
+    U = AD.cons_from_controls(*ctrl, g).contiguous()
+    dts, Uc, Bxc, Byc = [], U, ctrl[3], ctrl[4]
+    for _ in range(steps):
+        dts.append(AD.cfl_dt(Uc, g, 1.0 / n, 0.4))
+        Uc, Bxc, Byc = mesh.step(Uc, Bxc, Byc, dts[-1])
+    tgt = AD.orszag_tang_controls(n, vscale=1.05, modulation=0.05, noise=1e-3)
+    with torch.no_grad():
+        UT, _, _ = AD.rollout(AD.cons_from_controls(*tgt, g).contiguous(), tgt[3], tgt[4], mesh, dts, every)
+        target = UT[0].clone()
+
+    def objective(c):
+        U0 = AD.cons_from_controls(*c, g).contiguous()
+        UT, _, _ = AD.rollout(U0, c[3].contiguous(), c[4].contiguous(), mesh, dts, every)
+        return ((UT[0] - target) ** 2).sum()
+
+    leaves = [t.clone().requires_grad_(True) for t in ctrl]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = objective(leaves)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # per-step kernel cost of one forward step and one adjoint step on the same state
+    reps = 20
+    torch.cuda.synchronize()
+    a0 = time.perf_counter()
+    for _ in range(reps):
+        mesh.step(U, ctrl[3], ctrl[4], dts[0])
+    torch.cuda.synchronize()
+    a1 = time.perf_counter()
+    gU, gBx, gBy = torch.ones_like(U), torch.ones_like(ctrl[3]), torch.ones_like(ctrl[4])
+    for _ in range(reps):
+        mesh.step_vjp(U, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy)
+    torch.cuda.synchronize()
+    a2 = time.perf_counter()
+    out = {"workload": f"frozen uniform periodic {n}x{n}, Orszag-Tang controls, minmod-MUSCL + HLLE, CT, RK2, "
+                       f"{steps} steps, checkpoints every {every}",
+           "J": float(J), "forward_ms_per_step": (t1 - t0) / steps * 1e3,
+           "backward_ms_per_step": (t2 - t1) / steps * 1e3,
+           "backward_over_forward": (t2 - t1) / (t1 - t0),
+           "step_ms": (a1 - a0) / reps * 1e3, "step_vjp_ms": (a2 - a1) / reps * 1e3,
+           "vjp_over_step": (a2 - a1) / (a1 - a0),
+           "note": "backward includes recomputing every checkpoint segment (one extra forward per step)"}
+    if fd_dirs:
+        gen = torch.Generator(device="cpu").manual_seed(20261202)
+        errs = {1e-6: [], 1e-8: []}
+        for _ in range(fd_dirs):
+            dirs = [torch.randn(t.shape, generator=gen, dtype=torch.float64).to(t.device) * t.abs().mean()
+                    for t in ctrl]
+            ad = float(sum((l.grad * d).sum() for l, d in zip(leaves, dirs)))
+            for eps in errs:
+                with torch.no_grad():
+                    jp = objective([t + eps * d for t, d in zip(ctrl, dirs)])
+                    jm = objective([t - eps * d for t, d in zip(ctrl, dirs)])
+                fd = float(jp - jm) / (2 * eps)
+                errs[eps].append(abs(fd - ad) / abs(fd))
+        import statistics
+        out["fd_check"] = {f"eps_{e:.0e}": {"directions": len(v), "median_rel_err": statistics.median(v),
+                                            "max_rel_err": max(v)} for e, v in errs.items()}
+    mesh.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +270,14 @@ def main():
                     help="untimed batched steps before the timed ones (skips the plain leg)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="time only the batched ensemble (for profiling); prints its JSON object")
+    ap.add_argument("--ot-steps", type=int, default=0,
+                    help="also time the Orszag-Tang workload for this many steps (-1: to t = 0.5)")
+    ap.add_argument("--ot-warmup", type=int, default=0)
+    ap.add_argument("--adjoint", type=int, default=0, metavar="N",
+                    help="also run the frozen-mesh adjoint on an N x N grid")
+    ap.add_argument("--adj-steps", type=int, default=400)
+    ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-fd", type=int, default=0, help="finite-difference check directions")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -225,6 +347,10 @@ def main():
                      "zones_per_launch": zones_per_launch},
         "cpu_baseline": None,
     }
+    if args.ot_steps:
+        out["orszag_tang"] = ot_bench(args.ot_steps, args.ot_warmup)
+    if args.adjoint:
+        out["adjoint"] = adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd)
     if args.ensemble:
         out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup)
     if not args.no_cpu:

@@ -161,6 +161,25 @@ int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per
 int amr_profile_enable(amr_ctx *ctx, int mask);
 int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
 
+/* ---- Frozen-mesh step and its discrete adjoint (no counterpart in the
+ * reference, which has no adjoint).  One uniform periodic N x N grid of cell
+ * size dx, DEVICE pointers: U[8][N][N] (row-major, i fastest), Bx[N][N] (x-face
+ * between cells i-1 and i), By[N][N] (y-face between rows j-1 and j).
+ * amr_adj_step is one RK2 step of the reference (main.cpp:128-247) on that
+ * grid with the minmod-MUSCL + HLLE configuration and CT, at the given frozen
+ * dt; on a single-level mesh it equals the engine bit for bit.
+ * amr_adj_step_vjp recomputes the step from its input and writes the exact
+ * discrete adjoint (U_b, Bx_b, By_b) = J^T (Uo_b, Bxo_b, Byo_b) (branch-wise at
+ * limiter and wave-speed switches).  Outputs must not alias inputs. */
+typedef struct amr_adj amr_adj;
+int amr_adj_create(int n, double dx, double gamma, amr_adj **out);
+int amr_adj_destroy(amr_adj *adj);
+int amr_adj_step(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt, double *Uo,
+                 double *Bxo, double *Byo, void *stream);
+int amr_adj_step_vjp(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt,
+                     const double *Uo_b, const double *Bxo_b, const double *Byo_b, double *U_b, double *Bx_b,
+                     double *By_b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
