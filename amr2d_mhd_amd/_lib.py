@@ -83,6 +83,12 @@ _SIGS = {
                                C.c_void_p, C.c_void_p]),
     "amr_adj_step_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_adj_tape_doubles": (C.c_int64, [C.c_void_p]),
+    "amr_adj_step_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_adj_vjp_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_debug_side_roe": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle_vjp": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "amr_debug_recon_minmod_vjp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),

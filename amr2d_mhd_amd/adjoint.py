@@ -2,11 +2,14 @@
 
 The forward is the reference's RK2 step (minmod MUSCL + HLLE, CT) on one
 uniform periodic N x N grid at a frozen dt sequence (``amr_adj_step``); the
-backward runs the hand-written adjoint kernels (``amr_adj_step_vjp``) with
-checkpoint-and-recompute over time steps: the forward keeps the state every
-``every`` steps, and the backward recomputes each segment from its checkpoint
-before sweeping it in reverse.  Tensors are float64 on the GPU; kernels run on
-torch's current stream.
+backward runs the hand-written reverse kernels (``amr_adj_vjp_tape``) over
+per-step tapes (the intermediates of one step, 24 N^2 doubles).  With
+``every`` > 0 the forward keeps the state every ``every`` steps and the
+backward recomputes each segment from its checkpoint, writing the segment's
+tapes, before sweeping it in reverse; with ``every`` = 0 the forward writes
+the tape of every step and the backward recomputes nothing (memory: steps x
+24 N^2 doubles).  Tensors are float64 on the GPU; kernels run on torch's
+current stream.
 
 State layout: U[8][N][N] (rho, rho v, E, B), Bx[N][N] (x-face between cells
 i-1 and i), By[N][N] (y-face between rows j-1 and j).
@@ -53,18 +56,42 @@ class FrozenMesh:
             if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
                 raise ValueError("frozen-mesh tensors must be contiguous float64 CUDA tensors")
 
-    def step(self, U, Bx, By, dt):
-        """One RK2 step; returns new (U, Bx, By)."""
+    def tape_doubles(self):
+        return int(lib().amr_adj_tape_doubles(self.h))
+
+    def new_tapes(self, count, device):
+        return torch.empty((count, self.tape_doubles()), dtype=torch.float64, device=device)
+
+    def step(self, U, Bx, By, dt, tape=None):
+        """One RK2 step; returns new (U, Bx, By).  `tape`, a float64 CUDA
+        tensor of tape_doubles() elements, receives the step's tape."""
         self._check(U, Bx, By)
+        if tape is not None:
+            self._check(tape)
+            if tape.numel() < self.tape_doubles():
+                raise ValueError("tape too small")
         Uo, Bxo, Byo = torch.empty_like(U), torch.empty_like(Bx), torch.empty_like(By)
-        rc = lib().amr_adj_step(self.h, _ptr(U), _ptr(Bx), _ptr(By), float(dt), _ptr(Uo), _ptr(Bxo), _ptr(Byo),
-                                _stream())
+        rc = lib().amr_adj_step_tape(self.h, _ptr(U), _ptr(Bx), _ptr(By), float(dt), _ptr(Uo), _ptr(Bxo),
+                                     _ptr(Byo), _ptr(tape) if tape is not None else None, _stream())
         if rc:
             raise AmrError(rc)
         return Uo, Bxo, Byo
 
+    def vjp(self, tape, Bx, By, dt, Uob, Bxob, Byob):
+        """Adjoint of the step whose tape is `tape` and input faces (Bx, By)."""
+        self._check(tape, Bx, By, Uob, Bxob, Byob)
+        if tape.numel() < self.tape_doubles():
+            raise ValueError("tape too small")
+        Ub, Bxb, Byb = torch.empty_like(Uob), torch.empty_like(Bxob), torch.empty_like(Byob)
+        rc = lib().amr_adj_vjp_tape(self.h, _ptr(tape), _ptr(Bx), _ptr(By), float(dt), _ptr(Uob), _ptr(Bxob),
+                                    _ptr(Byob), _ptr(Ub), _ptr(Bxb), _ptr(Byb), _stream())
+        if rc:
+            raise AmrError(rc)
+        return Ub, Bxb, Byb
+
     def step_vjp(self, U, Bx, By, dt, Uob, Bxob, Byob):
-        """Adjoint of one step taken from (U, Bx, By): J^T (Uob, Bxob, Byob)."""
+        """Adjoint of one step taken from (U, Bx, By): J^T (Uob, Bxob, Byob),
+        recomputing the step first."""
         self._check(U, Bx, By, Uob, Bxob, Byob)
         Ub, Bxb, Byb = torch.empty_like(U), torch.empty_like(Bx), torch.empty_like(By)
         rc = lib().amr_adj_step_vjp(self.h, _ptr(U), _ptr(Bx), _ptr(By), float(dt), _ptr(Uob), _ptr(Bxob),
@@ -80,34 +107,53 @@ class Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, Bx0, By0, mesh, dts, every):
         U, Bx, By = U0.contiguous(), Bx0.contiguous(), By0.contiguous()
-        ckpts = []
+        every = int(every)
+        ckpts, tapes = [], None
+        if every == 0 and len(dts):
+            tapes = mesh.new_tapes(len(dts), U.device)
         for s, dt in enumerate(dts):
-            if s % every == 0:
+            if every == 0:
+                ckpts.append((Bx, By))  # input faces of every step
+            elif s % every == 0:
                 ckpts.append((U, Bx, By))
-            U, Bx, By = mesh.step(U, Bx, By, dt)
-        ctx.mesh, ctx.dts, ctx.every, ctx.ckpts = mesh, list(dts), int(every), ckpts
+            U, Bx, By = mesh.step(U, Bx, By, dt, tapes[s] if tapes is not None else None)
+        ctx.mesh, ctx.dts, ctx.every, ctx.ckpts, ctx.tapes = mesh, list(dts), every, ckpts, tapes
         return U, Bx, By
 
     @staticmethod
     def backward(ctx, gU, gBx, gBy):
         mesh, dts, every = ctx.mesh, ctx.dts, ctx.every
-        gU = torch.zeros_like(ctx.ckpts[0][0]) if gU is None else gU.contiguous()
-        gBx = torch.zeros_like(ctx.ckpts[0][1]) if gBx is None else gBx.contiguous()
-        gBy = torch.zeros_like(ctx.ckpts[0][2]) if gBy is None else gBy.contiguous()
-        for seg in range(len(ctx.ckpts) - 1, -1, -1):
-            s0 = seg * every
-            s1 = min(s0 + every, len(dts))
-            # recompute the inputs of every step of the segment
-            states = [ctx.ckpts[seg]]
-            for s in range(s0, s1 - 1):
-                states.append(mesh.step(*states[-1], dts[s]))
-            for s in range(s1 - 1, s0 - 1, -1):
-                gU, gBx, gBy = mesh.step_vjp(*states[s - s0], dts[s], gU, gBx, gBy)
-        ctx.ckpts = None
+        if not dts:
+            return gU, gBx, gBy, None, None, None
+        ref = ctx.ckpts[0][-1]
+        gU = torch.zeros((8,) + tuple(ref.shape), dtype=ref.dtype, device=ref.device) if gU is None \
+            else gU.contiguous()
+        gBx = torch.zeros_like(ref) if gBx is None else gBx.contiguous()
+        gBy = torch.zeros_like(ref) if gBy is None else gBy.contiguous()
+        if every == 0:
+            for s in range(len(dts) - 1, -1, -1):
+                Bx, By = ctx.ckpts[s]
+                gU, gBx, gBy = mesh.vjp(ctx.tapes[s], Bx, By, dts[s], gU, gBx, gBy)
+        else:
+            tapes = mesh.new_tapes(min(every, len(dts)), ref.device)
+            for seg in range(len(ctx.ckpts) - 1, -1, -1):
+                s0 = seg * every
+                s1 = min(s0 + every, len(dts))
+                # recompute the segment, keeping each step's input faces and tape
+                U, Bx, By = ctx.ckpts[seg]
+                faces = []
+                for s in range(s0, s1):
+                    faces.append((Bx, By))
+                    U, Bx, By = mesh.step(U, Bx, By, dts[s], tapes[s - s0])
+                for s in range(s1 - 1, s0 - 1, -1):
+                    gU, gBx, gBy = mesh.vjp(tapes[s - s0], *faces[s - s0], dts[s], gU, gBx, gBy)
+        ctx.ckpts = ctx.tapes = None
         return gU, gBx, gBy, None, None, None
 
 
 def rollout(U0, Bx0, By0, mesh, dts, every=20):
+    """Differentiable frozen-dt rollout; `every` = 0 tapes every step in the
+    forward instead of checkpointing."""
     return Rollout.apply(U0, Bx0, By0, mesh, dts, every)
 
 

@@ -75,10 +75,9 @@ __device__ inline void face_states(const Grid& G, const double* W, const double*
 __global__ __launch_bounds__(kT) void k_flux(Grid G, const double* __restrict__ W, const double* __restrict__ Bx,
                                              const double* __restrict__ By, double* __restrict__ Fx,
                                              double* __restrict__ Fy) {
-  const size_t t = (size_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= 2 * G.n2) return;
-  const int dir = t >= G.n2;
-  const size_t f = dir ? t - G.n2 : t;
+  const size_t f = (size_t)blockIdx.x * kT + threadIdx.x;
+  if (f >= G.n2) return;
+  const int dir = blockIdx.y;
   const int i = (int)(f % G.n), j = (int)(f / G.n);
   double wl[8], wr[8], fl[8];
   size_t cell[4];
@@ -114,18 +113,26 @@ __global__ __launch_bounds__(kT) void k_emf(Grid G, const double* __restrict__ F
   E[c] = 0.25 * ((gy5[(size_t)j * G.n + wrap(i - 1, G.n)] + gy5[c]) - (fx6[(size_t)wrap(j - 1, G.n) * G.n + i] + fx6[c]));
 }
 
-// face update: mode 1 Bn = B0 - dt r; mode 2 Bn = 0.5 (Bs + B0 - dt r)
+// face update: mode 1 Bn = B0 - dt r; mode 2 Bn = 0.5 (Bs + B0 - dt r); the
+// result also goes to (Bcx, Bcy) when given
 __global__ __launch_bounds__(kT) void k_faces(Grid G, const double* __restrict__ B0x, const double* __restrict__ B0y,
                                               const double* __restrict__ Bsx, const double* __restrict__ Bsy,
                                               const double* __restrict__ E, double dt, int mode,
-                                              double* __restrict__ Bnx, double* __restrict__ Bny) {
+                                              double* __restrict__ Bnx, double* __restrict__ Bny,
+                                              double* __restrict__ Bcx, double* __restrict__ Bcy) {
   const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
   if (c >= G.n2) return;
   const int i = (int)(c % G.n), j = (int)(c / G.n);
   const double rx = (1. / G.dx) * (E[(size_t)wrap(j + 1, G.n) * G.n + i] - E[c]);
   const double ry = -(1. / G.dx) * (E[(size_t)j * G.n + wrap(i + 1, G.n)] - E[c]);
-  Bnx[c] = mode == 1 ? B0x[c] - dt * rx : 0.5 * (Bsx[c] + B0x[c] - dt * rx);
-  Bny[c] = mode == 1 ? B0y[c] - dt * ry : 0.5 * (Bsy[c] + B0y[c] - dt * ry);
+  const double bx = mode == 1 ? B0x[c] - dt * rx : 0.5 * (Bsx[c] + B0x[c] - dt * rx);
+  const double by = mode == 1 ? B0y[c] - dt * ry : 0.5 * (Bsy[c] + B0y[c] - dt * ry);
+  Bnx[c] = bx;
+  Bny[c] = by;
+  if (Bcx) {
+    Bcx[c] = bx;
+    Bcy[c] = by;
+  }
 }
 
 // cell sync; saves the pre-sync cell B (q5, q6) when `pre` is given
@@ -147,48 +154,76 @@ __global__ __launch_bounds__(kT) void k_bsync(Grid G, double* __restrict__ Q, co
 }
 
 // -------------------------------------------------------------- adjoint
-// bsync reverse: Qb (adjoint of the synced state) -> adjoint of the
-// pre-sync state (in place) and of the cell averages bx, by (bb[2][N2])
-__global__ __launch_bounds__(kT) void k_bsync_vjp(Grid G, double* __restrict__ Qb, const double* __restrict__ pre,
-                                                  const double* __restrict__ Bx, const double* __restrict__ By,
-                                                  double* __restrict__ bb) {
+// Tape of one step (doubles, N2 = N*N each unless stated): the primitive
+// states the two flux passes read (W1 = prim(U), W2 = prim(Us), 8 N2 each),
+// the stage-1 faces (Bxs, Bys), the pre-sync cell B of both stages (2 N2
+// each) and the output faces (Bxo, Byo).
+struct Tape {
+  double *W1, *W2, *Bxs, *Bys, *pre1, *pre2, *Bxo, *Byo;
+};
+constexpr size_t kTapePerCell = 24;
+
+Tape tape_at(double* base, size_t n2) {
+  Tape t;
+  t.W1 = base;
+  t.W2 = base + 8 * n2;
+  t.Bxs = base + 16 * n2;
+  t.Bys = base + 17 * n2;
+  t.pre1 = base + 18 * n2;
+  t.pre2 = base + 20 * n2;
+  t.Bxo = base + 22 * n2;
+  t.Byo = base + 23 * n2;
+  return t;
+}
+
+// bsync reverse.  Qb (adjoint of the synced state; read from Qin, written to
+// Qout, which may alias) -> adjoint of the pre-sync state and of the cell
+// averages bx, by of the faces (bb[2][N2]); E += 0.5 ((bx^2 + by^2) - (q5^2 +
+// q6^2)), q5 <- bx, q6 <- by
+__global__ __launch_bounds__(kT) void k_bsync_vjp(Grid G, const double* Qin, double* Qout,
+                                                  const double* __restrict__ pre, const double* __restrict__ Bx,
+                                                  const double* __restrict__ By, double* __restrict__ bb) {
   const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
   if (c >= G.n2) return;
   const int i = (int)(c % G.n), j = (int)(c / G.n);
   const double bx = 0.5 * (Bx[(size_t)j * G.n + wrap(i + 1, G.n)] + Bx[c]);
   const double by = 0.5 * (By[(size_t)wrap(j + 1, G.n) * G.n + i] + By[c]);
-  const double e = Qb[4 * G.n2 + c];
-  bb[c] = Qb[5 * G.n2 + c] + e * bx;
-  bb[G.n2 + c] = Qb[6 * G.n2 + c] + e * by;
-  Qb[5 * G.n2 + c] = -e * pre[c];
-  Qb[6 * G.n2 + c] = -e * pre[G.n2 + c];
+  const double e = Qin[4 * G.n2 + c];
+  const double q5 = Qin[5 * G.n2 + c], q6 = Qin[6 * G.n2 + c];
+  bb[c] = q5 + e * bx;
+  bb[G.n2 + c] = q6 + e * by;
+  if (Qout != Qin) {
+    for (int k = 0; k < 8; k++)
+      if (k != 5 && k != 6) Qout[k * G.n2 + c] = Qin[k * G.n2 + c];
+  }
+  Qout[5 * G.n2 + c] = -e * pre[c];
+  Qout[6 * G.n2 + c] = -e * pre[G.n2 + c];
 }
 
-// faces reverse, fused with the cell-average reverse of the following bsync:
-// Bb* (adjoint of the updated faces, in place: + 0.5 of both adjacent cell
-// averages), then the adjoint of B0 (accumulated into B0b*), of Bs (mode 2:
-// written to Bsb*) and of E (written to Eb)
-__global__ __launch_bounds__(kT) void k_faces_avg_vjp(Grid G, double* __restrict__ Bbx, double* __restrict__ Bby,
-                                                      const double* __restrict__ bb) {
+// face reverse of one stage fused with the cell-average reverse of the bsync
+// that follows it.  Bb = Bin + 0.5 (bb of both adjacent cells) is the adjoint
+// of the updated faces (written to Bb, which may alias Bin).  Mode 2 (Bn =
+// 0.5 (Bs + B0 - dt r)): B0b = 0.5 Bb and Bsb = 0.5 Bb are assigned; mode 1
+// (Bn = B0 - dt r): B0b += Bb.
+__global__ __launch_bounds__(kT) void k_faces_vjp(Grid G, const double* Binx, const double* Biny,
+                                                  const double* __restrict__ bb, int mode, double* Bbx, double* Bby,
+                                                  double* __restrict__ B0bx, double* __restrict__ B0by,
+                                                  double* __restrict__ Bsbx, double* __restrict__ Bsby) {
   const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
   if (c >= G.n2) return;
   const int i = (int)(c % G.n), j = (int)(c / G.n);
-  Bbx[c] += 0.5 * (bb[c] + bb[(size_t)j * G.n + wrap(i - 1, G.n)]);
-  Bby[c] += 0.5 * (bb[G.n2 + c] + bb[G.n2 + (size_t)wrap(j - 1, G.n) * G.n + i]);
-}
-
-__global__ __launch_bounds__(kT) void k_faces_vjp(Grid G, const double* __restrict__ Bbx, const double* __restrict__ Bby,
-                                                  double dt, int mode, double* __restrict__ B0bx,
-                                                  double* __restrict__ B0by, double* __restrict__ Bsbx,
-                                                  double* __restrict__ Bsby) {
-  const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
-  if (c >= G.n2) return;
-  const double s = mode == 1 ? 1. : 0.5;
-  B0bx[c] += s * Bbx[c];
-  B0by[c] += s * Bby[c];
+  const double bxb = Binx[c] + 0.5 * (bb[c] + bb[(size_t)j * G.n + wrap(i - 1, G.n)]);
+  const double byb = Biny[c] + 0.5 * (bb[G.n2 + c] + bb[G.n2 + (size_t)wrap(j - 1, G.n) * G.n + i]);
+  Bbx[c] = bxb;
+  Bby[c] = byb;
   if (mode == 2) {
-    Bsbx[c] = 0.5 * Bbx[c];
-    Bsby[c] = 0.5 * Bby[c];
+    B0bx[c] = 0.5 * bxb;
+    B0by[c] = 0.5 * byb;
+    Bsbx[c] = 0.5 * bxb;
+    Bsby[c] = 0.5 * byb;
+  } else {
+    B0bx[c] += bxb;
+    B0by[c] += byb;
   }
 }
 
@@ -207,9 +242,14 @@ __global__ __launch_bounds__(kT) void k_emf_b(Grid G, const double* __restrict__
 }
 
 // adjoint of the face fluxes from the stage update (residual adjoint
-// resb = -s dt Qb, s = 1 or 0.5) and from the EMF (components 6 of Fx, 5 of Fy)
+// resb = -s dt Qb, s = 1 or 0.5) and from the EMF (components 6 of Fx, 5 of
+// Fy), fused with the pass-through of Qb to the stage inputs: mode 2
+// (out = 0.5 (U + Us - dt res)) assigns Ub = Usb = 0.5 Qb; mode 1 (out = U -
+// dt res) adds Qb to Ub
 __global__ __launch_bounds__(kT) void k_flux_b(Grid G, const double* __restrict__ Qb, const double* __restrict__ Eb,
-                                               double dt, int mode, double* __restrict__ Fbx, double* __restrict__ Fby) {
+                                               double dt, int mode, double* __restrict__ Fbx,
+                                               double* __restrict__ Fby, double* __restrict__ Ub,
+                                               double* __restrict__ Usb) {
   const size_t f = (size_t)blockIdx.x * kT + threadIdx.x;
   if (f >= G.n2) return;
   const int i = (int)(f % G.n), j = (int)(f / G.n);
@@ -217,59 +257,79 @@ __global__ __launch_bounds__(kT) void k_flux_b(Grid G, const double* __restrict_
   const size_t lx = (size_t)j * G.n + wrap(i - 1, G.n), ly = (size_t)wrap(j - 1, G.n) * G.n + i;
   for (int k = 0; k < 8; k++) {
     const size_t o = k * G.n2;
-    Fbx[o + f] = (s * Qb[o + lx] - s * Qb[o + f]) / G.dx;
-    Fby[o + f] = (s * Qb[o + ly] - s * Qb[o + f]) / G.dx;
+    const double q = Qb[o + f];
+    Fbx[o + f] = (s * Qb[o + lx] - s * q) / G.dx;
+    Fby[o + f] = (s * Qb[o + ly] - s * q) / G.dx;
+    if (mode == 2) {
+      Ub[o + f] = 0.5 * q;
+      Usb[o + f] = 0.5 * q;
+    } else {
+      Ub[o + f] += q;
+    }
   }
   Fby[5 * G.n2 + f] += 0.25 * (Eb[f] + Eb[(size_t)j * G.n + wrap(i + 1, G.n)]);
   Fbx[6 * G.n2 + f] -= 0.25 * (Eb[f] + Eb[(size_t)wrap(j + 1, G.n) * G.n + i]);
 }
 
-// per-face reverse of reconstruction + HLLE: stencil contributions S[s][k]
-// (s = cells i-2 .. i+1 along the face normal) and the normal-B adjoint
-__global__ __launch_bounds__(kT) void k_flux_vjp(Grid G, const double* __restrict__ W, const double* __restrict__ Bx,
-                                                 const double* __restrict__ By, const double* __restrict__ Fbx,
-                                                 const double* __restrict__ Fby, double* __restrict__ Sx,
-                                                 double* __restrict__ Sy, double* __restrict__ Bnbx,
-                                                 double* __restrict__ Bnby) {
-  const size_t t = (size_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= 2 * G.n2) return;
-  const int dir = t >= G.n2;
-  const size_t f = dir ? t - G.n2 : t;
+// per-face reverse of reconstruction + HLLE for the faces of one direction
+// (blockIdx.y): stencil contributions S[s][k] (s = cells i-2 .. i+1 along the
+// face normal) and the normal-B adjoint (accumulated into Bnb)
+template <int DIR>
+__device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __restrict__ W,
+                                     const double* __restrict__ Bn, const double* __restrict__ Fb,
+                                     double* __restrict__ S, double* __restrict__ Bnb) {
   const int i = (int)(f % G.n), j = (int)(f / G.n);
-  double wl[8], wr[8], fb[8], wlb[8] = {0}, wrb[8] = {0};
+  double wl[8], wr[8], fb[8], wlb[8], wrb[8];
   size_t cell[4];
-  face_states(G, W, dir ? By : Bx, dir, i, j, wl, wr, cell);
-  const double* Fb = dir ? Fby : Fbx;
-  for (int k = 0; k < 8; k++) fb[k] = Fb[k * G.n2 + f];
-  if (dir) swap_xy(fb);  // adjoint of the output swap
+  face_states(G, W, Bn, DIR, i, j, wl, wr, cell);
+  for (int k = 0; k < 8; k++) {
+    fb[k] = Fb[k * G.n2 + f];
+    wlb[k] = 0.;
+    wrb[k] = 0.;
+  }
+  if (DIR) swap_xy(fb);  // adjoint of the output swap
   hlle_vjp(wl, wr, G.g, fb, wlb, wrb);
-  const int nc = dir ? 6 : 5;
-  if (dir) {
+  if (DIR) {
     swap_xy(wlb);  // adjoint of the input swap
     swap_xy(wrb);
   }
-  (dir ? Bnby : Bnbx)[f] += wlb[nc] + wrb[nc];
-  double* S = dir ? Sy : Sx;
+  constexpr int nc = DIR ? 6 : 5;
+  Bnb[f] += wlb[nc] + wrb[nc];
+  // Re-read the stencil for the reconstruction reverse rather than keep the
+  // 28 values face_states loaded alive across the HLLE reverse: the opaque
+  // copy of the pointer stops the compiler from reusing those loads, which
+  // would cost ~60 VGPRs and push the kernel to one wave per SIMD.
+  const double* Wr = W;
+  asm volatile("" : "+s"(Wr));
   for (int k = 0; k < 8; k++) {
     double vb[4] = {0., 0., 0., 0.};
     if (k != nc) {
       double v[4];
-      for (int s = 0; s < 4; s++) v[s] = W[k * G.n2 + cell[s]];
+      for (int s = 0; s < 4; s++) v[s] = Wr[k * G.n2 + cell[s]];
       recon_minmod_vjp(v, wlb[k], wrb[k], vb);
     }
     for (int s = 0; s < 4; s++) S[((size_t)s * 8 + k) * G.n2 + f] = vb[s];
   }
 }
 
-// per-cell sum of the stencil contributions, then the cons->prim reverse:
-// Qb += (dW/dQ)^T Wb
-__global__ __launch_bounds__(kT) void k_gather_prim_vjp(Grid G, const double* __restrict__ Q,
+template <int DIR>
+__global__ __launch_bounds__(kT) void k_flux_vjp(Grid G, const double* __restrict__ W, const double* __restrict__ Bn,
+                                                 const double* __restrict__ Fb, double* __restrict__ S,
+                                                 double* __restrict__ Bnb) {
+  const size_t f = (size_t)blockIdx.x * kT + threadIdx.x;
+  if (f >= G.n2) return;
+  flux_vjp_face<DIR>(G, f, W, Bn, Fb, S, Bnb);
+}
+
+// per-cell sum of the stencil contributions, then the cons->prim reverse
+// from the tape's primitive state: Qb += (dW/dQ)^T Wb
+__global__ __launch_bounds__(kT) void k_gather_prim_vjp(Grid G, const double* __restrict__ W,
                                                         const double* __restrict__ Sx, const double* __restrict__ Sy,
                                                         double* __restrict__ Qb) {
   const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
   if (c >= G.n2) return;
   const int i = (int)(c % G.n), j = (int)(c / G.n);
-  double wb[8], u[8], ub[8];
+  double wb[8], w[8], ub[8];
   for (int k = 0; k < 8; k++) {
     double a = 0.;
     for (int s = 0; s < 4; s++) {
@@ -279,19 +339,11 @@ __global__ __launch_bounds__(kT) void k_gather_prim_vjp(Grid G, const double* __
       a += Sy[((size_t)s * 8 + k) * G.n2 + fy];
     }
     wb[k] = a;
-    u[k] = Q[k * G.n2 + c];
+    w[k] = W[k * G.n2 + c];
     ub[k] = 0.;
   }
-  prim_vjp(u, G.g, wb, ub);
+  prim_vjp_w(w, G.g, wb, ub);
   for (int k = 0; k < 8; k++) Qb[k * G.n2 + c] += ub[k];
-}
-
-// Qb = alpha * Rb (+ the previous Qb when accumulate)
-__global__ __launch_bounds__(kT) void k_axpy(size_t n, double alpha, const double* __restrict__ Rb,
-                                             double* __restrict__ Qb, int accumulate) {
-  const size_t c = (size_t)blockIdx.x * kT + threadIdx.x;
-  if (c >= n) return;
-  Qb[c] = accumulate ? Qb[c] + alpha * Rb[c] : alpha * Rb[c];
 }
 
 unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -300,8 +352,8 @@ unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 struct amr_adj {
   Grid G;
-  // forward intermediates
-  double *W, *Fx, *Fy, *E, *Us, *Bxs, *Bys, *pre1, *pre2, *Uo, *Bxo, *Byo;
+  // forward scratch
+  double *Fx, *Fy, *E, *Us, *Uo, *Bxo, *Byo, *tape;
   // adjoint work
   double *Qb2, *Usb, *Fbx, *Fby, *Eb, *Bsbx, *Bsby, *Bbx, *Bby, *bb, *Sx, *Sy;
 };
@@ -318,24 +370,50 @@ namespace {
     }                                                                                          \
   } while (0)
 
-// forward stage sequence; keeps Us (post-sync), Bxs, Bys, the pre-sync cell
-// B of both stages (pre1, pre2) and, for the adjoint, the stage-2 faces
+// forward stage sequence, writing the step's tape
 int forward(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo, double* Bxo,
-            double* Byo, hipStream_t s) {
+            double* Byo, const Tape& T, hipStream_t s) {
   const Grid& G = a->G;
-  const unsigned b1 = blocks(G.n2), b2 = blocks(2 * G.n2);
-  k_prim<<<b1, kT, 0, s>>>(G, U, a->W);
-  k_flux<<<b2, kT, 0, s>>>(G, a->W, Bx, By, a->Fx, a->Fy);
+  const unsigned b1 = blocks(G.n2);
+  const dim3 b2(b1, 2);
+  k_prim<<<b1, kT, 0, s>>>(G, U, T.W1);
+  k_flux<<<b2, kT, 0, s>>>(G, T.W1, Bx, By, a->Fx, a->Fy);
   k_update<<<b1, kT, 0, s>>>(G, U, nullptr, a->Fx, a->Fy, dt, 1, a->Us);
   k_emf<<<b1, kT, 0, s>>>(G, a->Fx, a->Fy, a->E);
-  k_faces<<<b1, kT, 0, s>>>(G, Bx, By, nullptr, nullptr, a->E, dt, 1, a->Bxs, a->Bys);
-  k_bsync<<<b1, kT, 0, s>>>(G, a->Us, a->Bxs, a->Bys, a->pre1);
-  k_prim<<<b1, kT, 0, s>>>(G, a->Us, a->W);
-  k_flux<<<b2, kT, 0, s>>>(G, a->W, a->Bxs, a->Bys, a->Fx, a->Fy);
+  k_faces<<<b1, kT, 0, s>>>(G, Bx, By, nullptr, nullptr, a->E, dt, 1, T.Bxs, T.Bys, nullptr, nullptr);
+  k_bsync<<<b1, kT, 0, s>>>(G, a->Us, T.Bxs, T.Bys, T.pre1);
+  k_prim<<<b1, kT, 0, s>>>(G, a->Us, T.W2);
+  k_flux<<<b2, kT, 0, s>>>(G, T.W2, T.Bxs, T.Bys, a->Fx, a->Fy);
   k_update<<<b1, kT, 0, s>>>(G, U, a->Us, a->Fx, a->Fy, dt, 2, Uo);
   k_emf<<<b1, kT, 0, s>>>(G, a->Fx, a->Fy, a->E);
-  k_faces<<<b1, kT, 0, s>>>(G, Bx, By, a->Bxs, a->Bys, a->E, dt, 2, Bxo, Byo);
-  k_bsync<<<b1, kT, 0, s>>>(G, Uo, Bxo, Byo, a->pre2);
+  k_faces<<<b1, kT, 0, s>>>(G, Bx, By, T.Bxs, T.Bys, a->E, dt, 2, Bxo, Byo, T.Bxo, T.Byo);
+  k_bsync<<<b1, kT, 0, s>>>(G, Uo, Bxo, Byo, T.pre2);
+  ADJ_CHK(hipGetLastError());
+  return 0;
+}
+
+// reverse sweep of one step from its tape and input faces
+int reverse(amr_adj* a, const Tape& T, const double* Bx, const double* By, double dt, const double* Uob,
+            const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb, hipStream_t s) {
+  const Grid& G = a->G;
+  const unsigned b1 = blocks(G.n2);
+  const dim3 b2(b1, 2);
+  // ---- stage 2
+  k_bsync_vjp<<<b1, kT, 0, s>>>(G, Uob, a->Qb2, T.pre2, T.Bxo, T.Byo, a->bb);
+  k_faces_vjp<<<b1, kT, 0, s>>>(G, Bxob, Byob, a->bb, 2, a->Bbx, a->Bby, Bxb, Byb, a->Bsbx, a->Bsby);
+  k_emf_b<<<b1, kT, 0, s>>>(G, a->Bbx, a->Bby, dt, 2, a->Eb);
+  k_flux_b<<<b1, kT, 0, s>>>(G, a->Qb2, a->Eb, dt, 2, a->Fbx, a->Fby, Ub, a->Usb);
+  k_flux_vjp<0><<<b1, kT, 0, s>>>(G, T.W2, T.Bxs, a->Fbx, a->Sx, a->Bsbx);
+  k_flux_vjp<1><<<b1, kT, 0, s>>>(G, T.W2, T.Bys, a->Fby, a->Sy, a->Bsby);
+  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, T.W2, a->Sx, a->Sy, a->Usb);
+  // ---- stage 1
+  k_bsync_vjp<<<b1, kT, 0, s>>>(G, a->Usb, a->Usb, T.pre1, T.Bxs, T.Bys, a->bb);
+  k_faces_vjp<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, a->bb, 1, a->Bsbx, a->Bsby, Bxb, Byb, nullptr, nullptr);
+  k_emf_b<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, dt, 1, a->Eb);
+  k_flux_b<<<b1, kT, 0, s>>>(G, a->Usb, a->Eb, dt, 1, a->Fbx, a->Fby, Ub, nullptr);
+  k_flux_vjp<0><<<b1, kT, 0, s>>>(G, T.W1, Bx, a->Fbx, a->Sx, Bxb);
+  k_flux_vjp<1><<<b1, kT, 0, s>>>(G, T.W1, By, a->Fby, a->Sy, Byb);
+  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, T.W1, a->Sx, a->Sy, Ub);
   ADJ_CHK(hipGetLastError());
   return 0;
 }
@@ -349,13 +427,12 @@ int amr_adj_create(int n, double dx, double gamma, amr_adj** out) {
   amr_adj* a = new amr_adj();
   a->G = Grid{n, dx, gamma, (size_t)n * n};
   const size_t n2 = a->G.n2;
-  double** bufs8[] = {&a->W, &a->Fx, &a->Fy, &a->Us, &a->Uo, &a->Qb2, &a->Usb, &a->Fbx, &a->Fby};
-  double** bufs1[] = {&a->E, &a->Bxs, &a->Bys, &a->Bxo, &a->Byo, &a->Eb, &a->Bsbx, &a->Bsby, &a->Bbx, &a->Bby};
+  double** bufs8[] = {&a->Fx, &a->Fy, &a->Us, &a->Uo, &a->Qb2, &a->Usb, &a->Fbx, &a->Fby};
+  double** bufs1[] = {&a->E, &a->Bxo, &a->Byo, &a->Eb, &a->Bsbx, &a->Bsby, &a->Bbx, &a->Bby};
   bool ok = true;
   for (auto p : bufs8) ok = ok && hipMalloc((void**)p, 8 * n2 * sizeof(double)) == hipSuccess;
   for (auto p : bufs1) ok = ok && hipMalloc((void**)p, n2 * sizeof(double)) == hipSuccess;
-  ok = ok && hipMalloc((void**)&a->pre1, 2 * n2 * sizeof(double)) == hipSuccess;
-  ok = ok && hipMalloc((void**)&a->pre2, 2 * n2 * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&a->tape, kTapePerCell * n2 * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&a->bb, 2 * n2 * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&a->Sx, 32 * n2 * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&a->Sy, 32 * n2 * sizeof(double)) == hipSuccess;
@@ -369,60 +446,43 @@ int amr_adj_create(int n, double dx, double gamma, amr_adj** out) {
 
 int amr_adj_destroy(amr_adj* a) {
   if (!a) return AMR_OK;
-  double* all[] = {a->W,   a->Fx,   a->Fy,   a->E,    a->Us,   a->Bxs, a->Bys, a->pre1, a->pre2, a->Uo,  a->Bxo,
-                   a->Byo, a->Qb2,  a->Usb,  a->Fbx,  a->Fby,  a->Eb,  a->Bsbx, a->Bsby, a->Bbx,  a->Bby, a->bb,
-                   a->Sx,  a->Sy};
+  double* all[] = {a->Fx,  a->Fy,  a->E,    a->Us,   a->Uo,  a->Bxo, a->Byo, a->tape, a->Qb2, a->Usb,
+                   a->Fbx, a->Fby, a->Eb,   a->Bsbx, a->Bsby, a->Bbx, a->Bby, a->bb,   a->Sx,  a->Sy};
   for (double* p : all)
     if (p) hipFree(p);
   delete a;
   return AMR_OK;
 }
 
+int64_t amr_adj_tape_doubles(const amr_adj* a) { return a ? (int64_t)(kTapePerCell * a->G.n2) : 0; }
+
+int amr_adj_step_tape(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo,
+                      double* Bxo, double* Byo, double* tape, void* stream) {
+  if (!a || !U || !Bx || !By || !Uo || !Bxo || !Byo) return AMR_ERR_CONFIG;
+  return forward(a, U, Bx, By, dt, Uo, Bxo, Byo, tape_at(tape ? tape : a->tape, a->G.n2), (hipStream_t)stream);
+}
+
 int amr_adj_step(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo, double* Bxo,
                  double* Byo, void* stream) {
-  if (!a || !U || !Bx || !By || !Uo || !Bxo || !Byo) return AMR_ERR_CONFIG;
-  return forward(a, U, Bx, By, dt, Uo, Bxo, Byo, (hipStream_t)stream);
+  return amr_adj_step_tape(a, U, Bx, By, dt, Uo, Bxo, Byo, nullptr, stream);
+}
+
+int amr_adj_vjp_tape(amr_adj* a, const double* tape, const double* Bx, const double* By, double dt,
+                     const double* Uob, const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb,
+                     void* stream) {
+  if (!a || !tape || !Bx || !By || !Uob || !Bxob || !Byob || !Ub || !Bxb || !Byb) return AMR_ERR_CONFIG;
+  return reverse(a, tape_at(const_cast<double*>(tape), a->G.n2), Bx, By, dt, Uob, Bxob, Byob, Ub, Bxb, Byb,
+                 (hipStream_t)stream);
 }
 
 int amr_adj_step_vjp(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, const double* Uob,
                      const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb, void* stream) {
   if (!a || !U || !Bx || !By || !Uob || !Bxob || !Byob || !Ub || !Bxb || !Byb) return AMR_ERR_CONFIG;
   hipStream_t s = (hipStream_t)stream;
-  const Grid& G = a->G;
-  const size_t n2 = G.n2;
-  const unsigned b1 = blocks(n2), b2 = blocks(2 * n2), b8 = blocks(8 * n2);
-  // recompute the step, keeping its intermediates
-  if (int e = forward(a, U, Bx, By, dt, a->Uo, a->Bxo, a->Byo, s)) return e;
-  ADJ_CHK(hipMemsetAsync(Ub, 0, 8 * n2 * sizeof(double), s));
-  ADJ_CHK(hipMemsetAsync(Bxb, 0, n2 * sizeof(double), s));
-  ADJ_CHK(hipMemsetAsync(Byb, 0, n2 * sizeof(double), s));
-  // ---- stage 2 (reverse)
-  k_axpy<<<b8, kT, 0, s>>>(8 * n2, 1., Uob, a->Qb2, 0);
-  k_bsync_vjp<<<b1, kT, 0, s>>>(G, a->Qb2, a->pre2, a->Bxo, a->Byo, a->bb);
-  ADJ_CHK(hipMemcpyAsync(a->Bbx, Bxob, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  ADJ_CHK(hipMemcpyAsync(a->Bby, Byob, n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  k_faces_avg_vjp<<<b1, kT, 0, s>>>(G, a->Bbx, a->Bby, a->bb);
-  k_faces_vjp<<<b1, kT, 0, s>>>(G, a->Bbx, a->Bby, dt, 2, Bxb, Byb, a->Bsbx, a->Bsby);
-  k_emf_b<<<b1, kT, 0, s>>>(G, a->Bbx, a->Bby, dt, 2, a->Eb);
-  k_flux_b<<<b1, kT, 0, s>>>(G, a->Qb2, a->Eb, dt, 2, a->Fbx, a->Fby);
-  // U and Us each take half of the stage-2 output adjoint
-  k_axpy<<<b8, kT, 0, s>>>(8 * n2, 0.5, a->Qb2, Ub, 1);
-  k_axpy<<<b8, kT, 0, s>>>(8 * n2, 0.5, a->Qb2, a->Usb, 0);
-  k_prim<<<b1, kT, 0, s>>>(G, a->Us, a->W);
-  k_flux_vjp<<<b2, kT, 0, s>>>(G, a->W, a->Bxs, a->Bys, a->Fbx, a->Fby, a->Sx, a->Sy, a->Bsbx, a->Bsby);
-  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, a->Us, a->Sx, a->Sy, a->Usb);
-  // ---- stage 1 (reverse)
-  k_bsync_vjp<<<b1, kT, 0, s>>>(G, a->Usb, a->pre1, a->Bxs, a->Bys, a->bb);
-  k_faces_avg_vjp<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, a->bb);
-  k_faces_vjp<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, dt, 1, Bxb, Byb, nullptr, nullptr);
-  k_emf_b<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, dt, 1, a->Eb);
-  k_flux_b<<<b1, kT, 0, s>>>(G, a->Usb, a->Eb, dt, 1, a->Fbx, a->Fby);
-  k_axpy<<<b8, kT, 0, s>>>(8 * n2, 1., a->Usb, Ub, 1);
-  k_prim<<<b1, kT, 0, s>>>(G, U, a->W);
-  k_flux_vjp<<<b2, kT, 0, s>>>(G, a->W, Bx, By, a->Fbx, a->Fby, a->Sx, a->Sy, Bxb, Byb);
-  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, U, a->Sx, a->Sy, Ub);
-  ADJ_CHK(hipGetLastError());
-  return AMR_OK;
+  const Tape T = tape_at(a->tape, a->G.n2);
+  // recompute the step into the context's own tape
+  if (int e = forward(a, U, Bx, By, dt, a->Uo, a->Bxo, a->Byo, T, s)) return e;
+  return reverse(a, T, Bx, By, dt, Uob, Bxob, Byob, Ub, Bxb, Byb, s);
 }
 
 // host entry points of the pointwise reverse pieces (CPU tests)
@@ -432,6 +492,29 @@ void amr_debug_hlle_vjp(const double* wL, const double* wR, double gamma, const 
 }
 
 void amr_debug_hlle(const double* wL, const double* wR, double gamma, double* f) { flux_hlle(wL, wR, gamma, f); }
+
+// forward values of the reverse's own forward pieces next to those of the
+// engine's: out[0..7] = side_fwd (cf, e, H) of both sides and roe_fwd (lmin,
+// lmax); ref[0..7] = the same from side_state / roe_bounds
+void amr_debug_side_roe(const double* wL, const double* wR, double gamma, double* out, double* ref) {
+  FastT fL, fR;
+  double eL, HL, eR, HR, lmin, lmax;
+  side_fwd(wL, gamma, fL, eL, HL);
+  side_fwd(wR, gamma, fR, eR, HR);
+  RoeT rt;
+  roe_fwd(wL, wR, gamma, HL, HR, rt, lmin, lmax);
+  const double o[] = {fL.cf, eL, HL, fR.cf, eR, HR, lmin, lmax};
+  Side L, R;
+  side_state(wL, gamma, L);
+  side_state(wR, gamma, R);
+  double u, rlmin, rlmax;
+  roe_bounds(wL, wR, gamma, L.H, R.H, u, rlmin, rlmax);
+  const double r[] = {L.cf, L.e, L.H, R.cf, R.e, R.H, rlmin, rlmax};
+  for (int k = 0; k < 8; k++) {
+    out[k] = o[k];
+    ref[k] = r[k];
+  }
+}
 
 void amr_debug_recon_minmod_vjp(const double* v, double wlb, double wrb, double* vb, double* wlr) {
   for (int s = 0; s < 4; s++) vb[s] = 0.;

@@ -179,9 +179,12 @@ def adjoint_bench(n, steps, every, fd_dirs):
     minmod-MUSCL + HLLE, CT, RK2, `steps` steps at a dt sequence frozen from
     the forward run, checkpoints every `every` steps.  J = sum (rho_T -
     rho_target)^2, target = the same set-up with the velocity scaled by 1.05.
-    Reports the forward and backward cost per step and, over `fd_dirs` seeded
-    random directions, the relative gap between the adjoint directional
-    derivative and central finite differences at eps 1e-6 and 1e-8."""
+    Reports the forward and backward cost per step, checkpointed and with
+    every step's tape kept from the forward (every = 0), the per-call cost of
+    the step, taped step, taped reverse and recomputing reverse and, over
+    `fd_dirs` seeded random directions, the relative gap between the adjoint
+    directional derivative and central finite differences at eps 1e-6 and
+    1e-8."""
     import torch
 
     from amr2d_mhd_amd import adjoint as AD
@@ -199,41 +202,57 @@ def adjoint_bench(n, steps, every, fd_dirs):
         UT, _, _ = AD.rollout(AD.cons_from_controls(*tgt, g).contiguous(), tgt[3], tgt[4], mesh, dts, every)
         target = UT[0].clone()
 
-    def objective(c):
+    def objective(c, ev=every):
         U0 = AD.cons_from_controls(*c, g).contiguous()
-        UT, _, _ = AD.rollout(U0, c[3].contiguous(), c[4].contiguous(), mesh, dts, every)
+        UT, _, _ = AD.rollout(U0, c[3].contiguous(), c[4].contiguous(), mesh, dts, ev)
         return ((UT[0] - target) ** 2).sum()
 
-    leaves = [t.clone().requires_grad_(True) for t in ctrl]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    J = objective(leaves)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    J.backward()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    # per-step kernel cost of one forward step and one adjoint step on the same state
+    def timed(ev):
+        ls = [t.clone().requires_grad_(True) for t in ctrl]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = objective(ls, ev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return ls, J, t1 - t0, t2 - t1
+
+    timed(every)  # warm-up (allocator, first launches)
+    leaves, J, tf, tb = timed(every)
+    _, J0, tf0, tb0 = timed(0)
+    # per-step kernel cost of one forward step, one taped reverse step and one
+    # recomputing reverse step on the same state
     reps = 20
-    torch.cuda.synchronize()
-    a0 = time.perf_counter()
-    for _ in range(reps):
-        mesh.step(U, ctrl[3], ctrl[4], dts[0])
-    torch.cuda.synchronize()
-    a1 = time.perf_counter()
+    tape = mesh.new_tapes(1, U.device)[0]
     gU, gBx, gBy = torch.ones_like(U), torch.ones_like(ctrl[3]), torch.ones_like(ctrl[4])
-    for _ in range(reps):
-        mesh.step_vjp(U, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy)
-    torch.cuda.synchronize()
-    a2 = time.perf_counter()
+
+    def per_call(fn):
+        fn()
+        torch.cuda.synchronize()
+        a0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a0) / reps * 1e3
+
+    step_ms = per_call(lambda: mesh.step(U, ctrl[3], ctrl[4], dts[0]))
+    step_tape_ms = per_call(lambda: mesh.step(U, ctrl[3], ctrl[4], dts[0], tape))
+    vjp_ms = per_call(lambda: mesh.vjp(tape, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy))
+    step_vjp_ms = per_call(lambda: mesh.step_vjp(U, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy))
     out = {"workload": f"frozen uniform periodic {n}x{n}, Orszag-Tang controls, minmod-MUSCL + HLLE, CT, RK2, "
                        f"{steps} steps, checkpoints every {every}",
-           "J": float(J), "forward_ms_per_step": (t1 - t0) / steps * 1e3,
-           "backward_ms_per_step": (t2 - t1) / steps * 1e3,
-           "backward_over_forward": (t2 - t1) / (t1 - t0),
-           "step_ms": (a1 - a0) / reps * 1e3, "step_vjp_ms": (a2 - a1) / reps * 1e3,
-           "vjp_over_step": (a2 - a1) / (a1 - a0),
-           "note": "backward includes recomputing every checkpoint segment (one extra forward per step)"}
+           "J": float(J), "forward_ms_per_step": tf / steps * 1e3,
+           "backward_ms_per_step": tb / steps * 1e3,
+           "backward_over_forward": tb / tf,
+           "note": "backward recomputes every checkpoint segment, writing each step's tape, then runs the "
+                   "taped reverse sweep",
+           "taped_every_step": {"J": float(J0), "forward_ms_per_step": tf0 / steps * 1e3,
+                                "backward_ms_per_step": tb0 / steps * 1e3, "backward_over_forward": tb0 / tf0,
+                                "tape_bytes": steps * mesh.tape_doubles() * 8},
+           "step_ms": step_ms, "step_tape_ms": step_tape_ms, "vjp_tape_ms": vjp_ms, "step_vjp_ms": step_vjp_ms,
+           "vjp_tape_over_step": vjp_ms / step_ms}
     if fd_dirs:
         gen = torch.Generator(device="cpu").manual_seed(20261202)
         errs = {1e-6: [], 1e-8: []}

@@ -168,14 +168,26 @@ int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
  * amr_adj_step is one RK2 step of the reference (main.cpp:128-247) on that
  * grid with the minmod-MUSCL + HLLE configuration and CT, at the given frozen
  * dt; on a single-level mesh it equals the engine bit for bit.
- * amr_adj_step_vjp recomputes the step from its input and writes the exact
+ * amr_adj_step_tape is the same step that also writes the step's tape (the
+ * intermediates its reverse needs, amr_adj_tape_doubles(adj) doubles) to
+ * `tape` (a DEVICE buffer; NULL: the context's own one).  amr_adj_vjp_tape
+ * is the reverse sweep from a tape and the step's input faces: the exact
  * discrete adjoint (U_b, Bx_b, By_b) = J^T (Uo_b, Bxo_b, Byo_b) (branch-wise at
- * limiter and wave-speed switches).  Outputs must not alias inputs. */
+ * limiter and wave-speed switches), without recomputing the step.
+ * amr_adj_step_vjp recomputes the step from its input into the context's own
+ * tape, then runs the reverse sweep.  Outputs must not alias inputs; calls on
+ * one context are stream-ordered and share its work buffers. */
 typedef struct amr_adj amr_adj;
 int amr_adj_create(int n, double dx, double gamma, amr_adj **out);
 int amr_adj_destroy(amr_adj *adj);
+int64_t amr_adj_tape_doubles(const amr_adj *adj);
 int amr_adj_step(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt, double *Uo,
                  double *Bxo, double *Byo, void *stream);
+int amr_adj_step_tape(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt, double *Uo,
+                      double *Bxo, double *Byo, double *tape, void *stream);
+int amr_adj_vjp_tape(amr_adj *adj, const double *tape, const double *Bx, const double *By, double dt,
+                     const double *Uo_b, const double *Bxo_b, const double *Byo_b, double *U_b, double *Bx_b,
+                     double *By_b, void *stream);
 int amr_adj_step_vjp(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt,
                      const double *Uo_b, const double *Bxo_b, const double *Byo_b, double *U_b, double *Bx_b,
                      double *By_b, void *stream);

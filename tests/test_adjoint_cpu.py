@@ -71,6 +71,24 @@ def test_recon_minmod_vjp_matches_fd():
 
 
 @pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0, 2.0])
+def test_reverse_forward_pieces_equal_engine_bitwise(gamma):
+    """The forward values the HLLE reverse computes for itself (fast speed,
+    energy, enthalpy, Roe bounds) are those of the engine's flux code bit for
+    bit, so every branch the reverse takes is the forward's."""
+    rng = np.random.default_rng(int(gamma * 1000) + 5)
+    out, ref = np.zeros(8), np.zeros(8)
+    for trial in range(2000):
+        wL, wR = _state(rng), _state(rng)
+        if trial % 3 == 0:
+            wR[5] = wL[5]
+        if trial % 7 == 0:  # vanishing transverse field and normal field
+            wL[6:8] = 0.0
+            wR[5:8] = 0.0
+        lib().amr_debug_side_roe(_p(wL), _p(wR), gamma, _p(out), _p(ref))
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (trial, out, ref)
+
+
+@pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0, 2.0])
 def test_hlle_vjp_matches_fd(gamma):
     rng = np.random.default_rng(int(gamma * 100))
     n = 0
