@@ -88,6 +88,7 @@ _SIGS = {
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_adj_vjp_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_debug_adj_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_debug_side_roe": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle_vjp": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),

@@ -30,6 +30,8 @@ namespace {
 constexpr int kT = 256;
 
 __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+// i mod n for any i (tiles can be wider than small grids)
+__device__ inline int wrap_any(int i, int n) { return ((i % n) + n) % n; }
 
 struct Grid {
   int n;
@@ -274,13 +276,15 @@ __global__ __launch_bounds__(kT) void k_flux_b(Grid G, const double* __restrict_
 // per-face reverse of reconstruction + HLLE for the faces of one direction
 // (blockIdx.y): stencil contributions S[s][k] (s = cells i-2 .. i+1 along the
 // face normal) and the normal-B adjoint (accumulated into Bnb)
+// reverse of reconstruction + HLLE at one face (i, j) of direction DIR: the
+// adjoints (wlb, wrb) of its two edge states in the grid frame, the stencil
+// cells (i-2 .. i+1 along the normal) and the normal-B adjoint
 template <int DIR>
-__device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __restrict__ W,
-                                     const double* __restrict__ Bn, const double* __restrict__ Fb,
-                                     double* __restrict__ S, double* __restrict__ Bnb) {
-  const int i = (int)(f % G.n), j = (int)(f / G.n);
-  double wl[8], wr[8], fb[8], wlb[8], wrb[8];
-  size_t cell[4];
+__device__ inline void face_reverse(const Grid& G, int i, int j, const double* __restrict__ W,
+                                    const double* __restrict__ Bn, const double* __restrict__ Fb, double* wlb,
+                                    double* wrb, size_t* cell) {
+  const size_t f = (size_t)j * G.n + i;
+  double wl[8], wr[8], fb[8];
   face_states(G, W, Bn, DIR, i, j, wl, wr, cell);
   for (int k = 0; k < 8; k++) {
     fb[k] = Fb[k * G.n2 + f];
@@ -293,8 +297,14 @@ __device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __re
     swap_xy(wlb);  // adjoint of the input swap
     swap_xy(wrb);
   }
+}
+
+// stencil contributions vb[s] of variable k (zero for the normal B, which the
+// face takes from Bn) from the edge-state adjoints; out(k, vb)
+template <int DIR, class Out>
+__device__ inline void stencil_reverse(const Grid& G, const double* __restrict__ W, const size_t* cell,
+                                       const double* wlb, const double* wrb, Out out) {
   constexpr int nc = DIR ? 6 : 5;
-  Bnb[f] += wlb[nc] + wrb[nc];
   // Re-read the stencil for the reconstruction reverse rather than keep the
   // 28 values face_states loaded alive across the HLLE reverse: the opaque
   // copy of the pointer stops the compiler from reusing those loads, which
@@ -308,8 +318,23 @@ __device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __re
       for (int s = 0; s < 4; s++) v[s] = Wr[k * G.n2 + cell[s]];
       recon_minmod_vjp(v, wlb[k], wrb[k], vb);
     }
-    for (int s = 0; s < 4; s++) S[((size_t)s * 8 + k) * G.n2 + f] = vb[s];
+    out(k, vb);
   }
+}
+
+template <int DIR>
+__device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __restrict__ W,
+                                     const double* __restrict__ Bn, const double* __restrict__ Fb,
+                                     double* __restrict__ S, double* __restrict__ Bnb) {
+  const int i = (int)(f % G.n), j = (int)(f / G.n);
+  double wlb[8], wrb[8];
+  size_t cell[4];
+  face_reverse<DIR>(G, i, j, W, Bn, Fb, wlb, wrb, cell);
+  constexpr int nc = DIR ? 6 : 5;
+  Bnb[f] += wlb[nc] + wrb[nc];
+  stencil_reverse<DIR>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
+    for (int s = 0; s < 4; s++) S[((size_t)s * 8 + k) * G.n2 + f] = vb[s];
+  });
 }
 
 template <int DIR>
@@ -346,12 +371,99 @@ __global__ __launch_bounds__(kT) void k_gather_prim_vjp(Grid G, const double* __
   for (int k = 0; k < 8; k++) Qb[k * G.n2 + c] += ub[k];
 }
 
+// Fused face reverse + stencil sum + cons->prim reverse over a tile of
+// TW x TH cells, one workgroup of TT lanes: the x faces the tile's cells
+// read (TH rows of TW + 3) are reversed one per lane and leave their
+// stencil contributions in LDS; after a barrier every cell sums its four;
+// the y faces (TH + 3 rows of TW) follow in the same LDS; then the cell
+// adds (dW/dQ)^T Wb to Qb.  Faces next to the tile edge are reversed by both
+// tiles that read them; the normal-B adjoint of a face is added only by the
+// tile that owns the face.  Deterministic: fixed summation order, no
+// atomics.
+template <int TW, int TH, int TT>
+__global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restrict__ W,
+                                                  const double* __restrict__ Bx, const double* __restrict__ By,
+                                                  const double* __restrict__ Fbx, const double* __restrict__ Fby,
+                                                  double* __restrict__ Bnbx, double* __restrict__ Bnby,
+                                                  double* __restrict__ Qb) {
+  static_assert(TH * (TW + 3) <= TT && (TH + 3) * TW <= TT, "one face per lane and direction");
+  __shared__ double S[8][4][TT];  // [variable][stencil position][face of the tile]
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
+  // this lane's cell
+  const int ci = tid % TW, cj = tid / TW;
+  const int i = i0 + ci, j = j0 + cj;
+  const bool has_cell = tid < TW * TH && i < G.n && j < G.n;
+  double wb[8];
+  // ---- x faces: tile row r, face column q -> face i0 - 1 + q
+  if (tid < TH * (TW + 3)) {
+    const int r = tid / (TW + 3), q = tid % (TW + 3);
+    const int fj = j0 + r, fi = wrap_any(i0 - 1 + q, G.n);
+    if (fj < G.n) {
+      double wlb[8], wrb[8];
+      size_t cell[4];
+      face_reverse<0>(G, fi, fj, W, Bx, Fbx, wlb, wrb, cell);
+      if (q >= 1 && q <= TW && i0 - 1 + q < G.n) Bnbx[(size_t)fj * G.n + fi] += wlb[5] + wrb[5];
+      stencil_reverse<0>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
+        for (int s = 0; s < 4; s++) S[k][s][tid] = vb[s];
+      });
+    }
+  }
+  __syncthreads();
+  if (has_cell) {
+    // this cell is stencil position s of the x face i + 2 - s: tile column ci + 3 - s
+    for (int k = 0; k < 8; k++) {
+      double a = 0.;
+      for (int s = 0; s < 4; s++) a += S[k][s][cj * (TW + 3) + ci + 3 - s];
+      wb[k] = a;
+    }
+  }
+  __syncthreads();
+  // ---- y faces: face row r -> face j0 - 1 + r, column q
+  if (tid < (TH + 3) * TW) {
+    const int r = tid / TW, q = tid % TW;
+    const int fj = wrap_any(j0 - 1 + r, G.n), fi = i0 + q;
+    if (fi < G.n) {
+      double wlb[8], wrb[8];
+      size_t cell[4];
+      face_reverse<1>(G, fi, fj, W, By, Fby, wlb, wrb, cell);
+      if (r >= 1 && r <= TH && j0 - 1 + r < G.n) Bnby[(size_t)fj * G.n + fi] += wlb[6] + wrb[6];
+      stencil_reverse<1>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
+        for (int s = 0; s < 4; s++) S[k][s][tid] = vb[s];
+      });
+    }
+  }
+  __syncthreads();
+  if (has_cell) {
+    const size_t c = (size_t)j * G.n + i;
+    double w[8], ub[8];
+    for (int k = 0; k < 8; k++) {
+      double a = wb[k];
+      for (int s = 0; s < 4; s++) a += S[k][s][(cj + 3 - s) * TW + ci];
+      wb[k] = a;
+      w[k] = W[k * G.n2 + c];
+      ub[k] = 0.;
+    }
+    prim_vjp_w(w, G.g, wb, ub);
+    for (int k = 0; k < 8; k++) Qb[k * G.n2 + c] += ub[k];
+  }
+}
+
+template <int TW, int TH, int TT>
+void launch_stage_vjp(const Grid& G, const double* W, const double* Bx, const double* By, const double* Fbx,
+                      const double* Fby, double* Bnbx, double* Bnby, double* Qb, hipStream_t s) {
+  const dim3 grid((unsigned)((G.n + TW - 1) / TW), (unsigned)((G.n + TH - 1) / TH));
+  k_stage_vjp<TW, TH, TT><<<grid, TT, 0, s>>>(G, W, Bx, By, Fbx, Fby, Bnbx, Bnby, Qb);
+}
+
 unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 }  // namespace
 
 struct amr_adj {
   Grid G;
+  int fused = 1;  // face reverse + gather in one tiled kernel: 1 16x13 tiles of 256 lanes, 2 32x13 tiles of
+                  // 512 lanes; 0: per-face kernels and a gather
   // forward scratch
   double *Fx, *Fy, *E, *Us, *Uo, *Bxo, *Byo, *tape;
   // adjoint work
@@ -392,28 +504,40 @@ int forward(amr_adj* a, const double* U, const double* Bx, const double* By, dou
   return 0;
 }
 
+// face reverse of one stage from its flux adjoints (a->Fbx, a->Fby): normal-B
+// adjoints into Bnb*, cell adjoints added to Qb
+void face_and_gather(amr_adj* a, const double* W, const double* Bx, const double* By, double* Bnbx, double* Bnby,
+                     double* Qb, hipStream_t s) {
+  const Grid& G = a->G;
+  if (a->fused == 1) {
+    launch_stage_vjp<16, 13, 256>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+  } else if (a->fused == 2) {
+    launch_stage_vjp<32, 13, 512>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+  } else {
+    const unsigned b1 = blocks(G.n2);
+    k_flux_vjp<0><<<b1, kT, 0, s>>>(G, W, Bx, a->Fbx, a->Sx, Bnbx);
+    k_flux_vjp<1><<<b1, kT, 0, s>>>(G, W, By, a->Fby, a->Sy, Bnby);
+    k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, W, a->Sx, a->Sy, Qb);
+  }
+}
+
 // reverse sweep of one step from its tape and input faces
 int reverse(amr_adj* a, const Tape& T, const double* Bx, const double* By, double dt, const double* Uob,
             const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb, hipStream_t s) {
   const Grid& G = a->G;
   const unsigned b1 = blocks(G.n2);
-  const dim3 b2(b1, 2);
   // ---- stage 2
   k_bsync_vjp<<<b1, kT, 0, s>>>(G, Uob, a->Qb2, T.pre2, T.Bxo, T.Byo, a->bb);
   k_faces_vjp<<<b1, kT, 0, s>>>(G, Bxob, Byob, a->bb, 2, a->Bbx, a->Bby, Bxb, Byb, a->Bsbx, a->Bsby);
   k_emf_b<<<b1, kT, 0, s>>>(G, a->Bbx, a->Bby, dt, 2, a->Eb);
   k_flux_b<<<b1, kT, 0, s>>>(G, a->Qb2, a->Eb, dt, 2, a->Fbx, a->Fby, Ub, a->Usb);
-  k_flux_vjp<0><<<b1, kT, 0, s>>>(G, T.W2, T.Bxs, a->Fbx, a->Sx, a->Bsbx);
-  k_flux_vjp<1><<<b1, kT, 0, s>>>(G, T.W2, T.Bys, a->Fby, a->Sy, a->Bsby);
-  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, T.W2, a->Sx, a->Sy, a->Usb);
+  face_and_gather(a, T.W2, T.Bxs, T.Bys, a->Bsbx, a->Bsby, a->Usb, s);
   // ---- stage 1
   k_bsync_vjp<<<b1, kT, 0, s>>>(G, a->Usb, a->Usb, T.pre1, T.Bxs, T.Bys, a->bb);
   k_faces_vjp<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, a->bb, 1, a->Bsbx, a->Bsby, Bxb, Byb, nullptr, nullptr);
   k_emf_b<<<b1, kT, 0, s>>>(G, a->Bsbx, a->Bsby, dt, 1, a->Eb);
   k_flux_b<<<b1, kT, 0, s>>>(G, a->Usb, a->Eb, dt, 1, a->Fbx, a->Fby, Ub, nullptr);
-  k_flux_vjp<0><<<b1, kT, 0, s>>>(G, T.W1, Bx, a->Fbx, a->Sx, Bxb);
-  k_flux_vjp<1><<<b1, kT, 0, s>>>(G, T.W1, By, a->Fby, a->Sy, Byb);
-  k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, T.W1, a->Sx, a->Sy, Ub);
+  face_and_gather(a, T.W1, Bx, By, Bxb, Byb, Ub, s);
   ADJ_CHK(hipGetLastError());
   return 0;
 }
@@ -483,6 +607,15 @@ int amr_adj_step_vjp(amr_adj* a, const double* U, const double* Bx, const double
   // recompute the step into the context's own tape
   if (int e = forward(a, U, Bx, By, dt, a->Uo, a->Bxo, a->Byo, T, s)) return e;
   return reverse(a, T, Bx, By, dt, Uob, Bxob, Byob, Ub, Bxb, Byb, s);
+}
+
+// test hook: 1 = fused tiled face reverse on 16x13 tiles (default), 2 = on
+// 32x13 tiles, 0 = per-face kernels and a separate gather (same derivative,
+// different summation order)
+int amr_debug_adj_set_fused(amr_adj* a, int mode) {
+  if (!a || mode < 0 || mode > 2) return AMR_ERR_CONFIG;
+  a->fused = mode;
+  return AMR_OK;
 }
 
 // host entry points of the pointwise reverse pieces (CPU tests)

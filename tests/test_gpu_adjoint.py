@@ -138,6 +138,31 @@ def test_tape_reverse_equals_recomputing_reverse():
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("n", [16, 45, 64])
+def test_tiled_reverse_equals_per_face_reverse(n):
+    """The tiled face reverse (stencil sums in LDS, faces at tile edges
+    reversed twice, grids narrower than a tile and not a multiple of it)
+    equals the per-face kernels + gather up to the summation order."""
+    import torch
+
+    from amr2d_mhd_amd import adjoint as AD
+    from amr2d_mhd_amd._lib import lib
+
+    mesh, (rho, v, p, Bx, By), dts = _setup(n, 2)
+    U = AD.cons_from_controls(rho, v, p, Bx, By, GAMMA).contiguous()
+    outs = mesh.step(U, Bx, By, dts[0])
+    gen = torch.Generator().manual_seed(11)
+    seeds = [torch.randn(t.shape, generator=gen, dtype=torch.float64).cuda() for t in outs]
+    res = {}
+    for mode in (0, 2, 1):  # per-face + gather, 32x13 tiles, 16x13 tiles (the default, set last)
+        assert lib().amr_debug_adj_set_fused(mesh.h, mode) == 0
+        res[mode] = mesh.step_vjp(U, Bx, By, dts[0], *seeds)
+    torch.cuda.synchronize()
+    for mode in (1, 2):
+        for a, b in zip(res[mode], res[0]):
+            assert torch.allclose(a, b, rtol=1e-13, atol=1e-15 * float(b.abs().max())), (mode, float((a - b).abs().max()))
+
+
 def test_adjoint_matches_fd_of_cpu_forward():
     """Directional derivative from the GPU adjoint against central finite
     differences of the numpy forward (the restated reference step)."""
