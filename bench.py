@@ -15,6 +15,8 @@ Also reported:
                 dominant kernel): algorithmic bytes per launch / mean launch
                 time from HIP events on the engine's stream during the timed
                 region, against the 8 TB/s HBM peak;
+  valu          the same kernel's executed FP64 FLOPs per launch (counter-
+                measured per zone) / mean launch time, against 78.6 TFLOP/s;
   cpu_baseline  the reference itself (compiled from its sources into
                 oracle/_ref with its OpenMP loops) on the first steps of the
                 same rotor run, on the host cores of this machine.
@@ -38,9 +40,16 @@ ROTOR_STEPS = 4763  # steps of the full rotor run to t = 0.295 without FMA contr
 STAGE_BYTES_PER_ZONE = 176.0
 # HBM bytes per leaf zone one stage launch actually moves, from rocprofv3 FETCH_SIZE
 # (x2.0, copy-calibrated) + WRITE_SIZE (x1.0) on this workload:
-# profiles/stage_traffic_rotor_lv5.md (272.7 read + 270.0 written)
-STAGE_TRAFFIC_BYTES_PER_ZONE = 542.7
+# profiles/stage_traffic_rotor_lv5.md (273.5 read + 270.0 written, tuned 768-lane variant)
+STAGE_TRAFFIC_BYTES_PER_ZONE = 543.5
 HBM_PEAK_GBS = 8000.0
+# FP64 FLOPs one stage launch executes per leaf zone: rocprofv3 counters
+# (SQ_INSTS_VALU_FLOPS_FP64 + SQ_INSTS_VALU_FLOPS_FP64_TRANS) x 64 lanes per
+# dispatch / (256 x leaves), median over the stage dispatches of this workload
+# (profiles/fp64_flops_rotor_lv5_adjoint_512.md); FP64 division and sqrt count
+# with their full instruction sequences, and the whole 20x20 tile is computed
+STAGE_FP64_FLOPS_PER_ZONE = 3835.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def cpu_model():
@@ -364,6 +373,12 @@ def main():
                      "kernel": "stage_kernel<20,2>", "mean_launch_us": stage_ms / stage_n * 1e3,
                      "launches": stage_n, "bytes_per_zone": STAGE_BYTES_PER_ZONE,
                      "zones_per_launch": zones_per_launch},
+        "valu": {"bound": "fp64", "kernel": "stage_kernel<20,2>",
+                 "achieved": STAGE_FP64_FLOPS_PER_ZONE * zones_per_launch / (stage_ms / stage_n * 1e-3) / 1e12,
+                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": STAGE_FP64_FLOPS_PER_ZONE * zones_per_launch / (stage_ms / stage_n * 1e-3) / 1e12
+                 / FP64_PEAK_TFLOPS,
+                 "flops_per_zone": STAGE_FP64_FLOPS_PER_ZONE},
         "cpu_baseline": None,
     }
     if args.ot_steps:
