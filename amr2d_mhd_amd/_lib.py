@@ -73,6 +73,7 @@ _SIGS = {
     "amr_recon_batch": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "amr_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_set_fill_cache": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_set_stage_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

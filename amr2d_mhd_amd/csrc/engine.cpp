@@ -133,6 +133,24 @@ struct PinnedVec {
   T* data() { return p; }
 };
 
+// A compiled ghost-fill program kept for reuse, with its device copy: AMR
+// runs refine and coarsen the same blocks back and forth, so trees recur
+// (the 5-level rotor changes its tree 1912 times but has 525 distinct trees;
+// 1315 changes return to the tree of two changes earlier).  The program is a
+// function of the key alone: lastActive, then per block its neighbour table
+// row, leaf flag and arena slot.
+struct FillProg {
+  std::vector<int32_t> key;
+  uint64_t hash = 0;
+  PinnedVec<FillOp> ops;
+  std::vector<FillGen> gen;
+  FillOp* d_ops = nullptr;
+  int cap_ops = 0;
+  FillGen* d_gen = nullptr;
+  int cap_gen = 0;
+  uint64_t used = 0;  // LRU stamp
+};
+
 // ------------------------------------------------------------- instance
 struct Instance {
   amr_sim_config cfg;
@@ -161,6 +179,10 @@ struct Instance {
   int cap_ops = 0;
   FillGen* d_gen = nullptr;
   int cap_gen = 0;
+  // the tree the current program was compiled for, and earlier programs
+  std::vector<int32_t> prog_key;
+  uint64_t prog_hash = 0;
+  std::vector<FillProg> prog_cache;
 };
 
 // --------------------------------------------------------------- context
@@ -168,7 +190,8 @@ struct Profile {
   double ms = 0;
   int64_t launches = 0;
 };
-enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_COUNT };
+enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_HOST_COMPILE,
+       PF_COUNT };
 
 
 }  // namespace
@@ -202,6 +225,9 @@ struct amr_ctx {
   // the device work of one step replayed as a HIP graph; re-captured when
   // the topology (work lists, program tables, buffer addresses) changes
   bool graph_on = true, graph_dirty = true, regrid_launched = false;
+  // compiled fill programs kept per instance besides the current one (LRU)
+  int prog_cache_max = 4;
+  uint64_t prog_clock = 0;
   // stage-kernel variant: fixed (0..kStageVariants-1) or chosen per leaf-count
   // bucket by the tuner (-1): the first launches in a bucket try every variant
   // kTuneTrials times, timed by the kernel's own device-clock span, then the
@@ -1185,6 +1211,74 @@ int upload_params(amr_ctx* c) {
   return 0;
 }
 
+// the tree a fill program depends on (FillProg::key)
+void program_key(const Instance& I, std::vector<int32_t>& key, uint64_t& hash) {
+  key.clear();
+  key.reserve(1 + (size_t)I.lastActive * 10);
+  key.push_back(I.lastActive);
+  for (int nb = 0; nb < I.lastActive; nb++) {
+    for (int s = 0; s < 8; s++) key.push_back(I.lp[nb][s]);
+    key.push_back(I.leafs[nb]);
+    key.push_back(nb < (int)I.gs.size() ? I.gs[nb] : -1);
+  }
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ key.size();
+  for (int32_t v : key) {
+    h ^= (uint32_t)v;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 32;
+  }
+  hash = h;
+}
+
+void swap_program(Instance& I, FillProg& P) {
+  std::swap(I.ops, P.ops);
+  std::swap(I.gen, P.gen);
+  std::swap(I.d_ops, P.d_ops);
+  std::swap(I.cap_ops, P.cap_ops);
+  std::swap(I.d_gen, P.d_gen);
+  std::swap(I.cap_gen, P.cap_gen);
+  std::swap(I.prog_key, P.key);
+  std::swap(I.prog_hash, P.hash);
+}
+
+// Makes the program of I's current tree current without compiling when that
+// tree is the one of the current program or of a cached one (the device copy
+// travels with it; the stream is idle here, so no launch still reads a
+// buffer that changes role).  Otherwise the current program is cached (the
+// least recently used entry gives up its buffers for the new compile), the
+// key is set and false is returned: compile into I.ops/I.gen and upload.
+bool reuse_program(amr_ctx* c, Instance& I) {
+  std::vector<int32_t> key;
+  uint64_t hash = 0;
+  program_key(I, key, hash);
+  if (!I.prog_key.empty() && hash == I.prog_hash && key == I.prog_key) return true;
+  if (c->prog_cache_max <= 0) {
+    I.prog_key.swap(key);
+    I.prog_hash = hash;
+    return false;
+  }
+  for (auto& P : I.prog_cache)
+    if (P.hash == hash && !P.key.empty() && P.key == key) {
+      swap_program(I, P);  // P now holds the previous program
+      P.used = ++c->prog_clock;
+      return true;
+    }
+  FillProg* slot = nullptr;
+  if ((int)I.prog_cache.size() < c->prog_cache_max) {
+    I.prog_cache.emplace_back();
+    slot = &I.prog_cache.back();
+  } else {
+    slot = &I.prog_cache[0];
+    for (auto& P : I.prog_cache)
+      if (P.used < slot->used) slot = &P;
+  }
+  swap_program(I, *slot);  // the previous program is cached, its buffers are reused
+  slot->used = ++c->prog_clock;
+  I.prog_key.swap(key);
+  I.prog_hash = hash;
+  return false;
+}
+
 // leaf work list and fill program of the running instances
 int rebuild_topology(amr_ctx* c) {
   HostTimed ht(c, PF_HOST_TOPO);
@@ -1206,7 +1300,16 @@ int rebuild_topology(amr_ctx* c) {
   if (c->nleaf)
     HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
                           c->stream));
-  // recompile the programs of instances whose tree changed, in parallel
+  // programs of instances whose tree changed: the current one if the tree is
+  // the same, a cached one if the tree recurs, else compiled (in parallel)
+  {
+    std::vector<int> todo;
+    for (int qi : dirty)
+      if (!reuse_program(c, c->inst[qi])) todo.push_back(qi);
+      else c->inst[qi].prog_dirty = false;
+    dirty.swap(todo);
+  }
+  const auto t_compile = std::chrono::steady_clock::now();
   int nthreads = (int)std::min<size_t>(dirty.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
   if (nthreads < 1) nthreads = 1;
   if ((int)c->scratch.size() < nthreads) c->scratch.resize(nthreads);
@@ -1227,6 +1330,11 @@ int rebuild_topology(amr_ctx* c) {
     for (int t = 1; t < nthreads; t++) th.emplace_back(work, t);
     work(0);
     for (auto& x : th) x.join();
+  }
+  if (!dirty.empty() && (c->prof_mask & (1u << PF_HOST_COMPILE))) {
+    c->prof[PF_HOST_COMPILE].ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_compile).count();
+    c->prof[PF_HOST_COMPILE].launches += (int64_t)dirty.size();
   }
   bool failed = false;
   for (size_t q = 0; q < dirty.size(); q++)
@@ -1720,6 +1828,10 @@ int amr_destroy(amr_ctx* c) {
   for (auto& I : c->inst) {
     hipFree(I.d_ops);
     hipFree(I.d_gen);
+    for (auto& P : I.prog_cache) {
+      hipFree(P.d_ops);
+      hipFree(P.d_gen);
+    }
   }
   hipFree(c->d_ip); hipFree(c->d_lam); hipFree(c->d_grad); hipFree(c->d_err); hipFree(c->d_rops);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
@@ -2007,6 +2119,20 @@ int amr_set_graph(amr_ctx* c, int on) {
   if (!c) return AMR_ERR_CONFIG;
   c->graph_on = on != 0;
   c->graph_dirty = true;
+  return AMR_OK;
+}
+
+int amr_set_fill_cache(amr_ctx* c, int entries) {
+  if (!c || entries < 0 || entries > 64) return AMR_ERR_CONFIG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->prog_cache_max = entries;
+  for (auto& I : c->inst) {
+    for (auto& P : I.prog_cache) {
+      hipFree(P.d_ops);
+      hipFree(P.d_gen);
+    }
+    I.prog_cache.clear();
+  }
   return AMR_OK;
 }
 

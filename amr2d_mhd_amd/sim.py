@@ -145,11 +145,16 @@ class Simulation:
         return ib
 
     # --------------------------------------------------------------- timing
-    PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait")
+    PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait",
+                        "host_compile")
 
     def set_graph(self, on=True):
         """Replay each step's device work as a HIP graph (default) or launch it directly."""
         _check(self.h, lib().amr_set_graph(self.h, 1 if on else 0))
+
+    def set_fill_cache(self, entries):
+        """Fill programs kept per instance for recurring trees (default 4; 0 = compile on every change)."""
+        _check(self.h, lib().amr_set_fill_cache(self.h, int(entries)))
 
     def set_stage_variant(self, variant):
         """Select the stage-kernel variant (0-4, identical results; -1 = tuned per leaf-count bucket)."""

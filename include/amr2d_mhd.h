@@ -149,6 +149,14 @@ int amr_set_stage_variant(amr_ctx *ctx, int variant);
  * microseconds per leaf block of each variant, us_per_leaf[b][5]. */
 int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per_leaf);
 
+/* Fill-program cache (no counterpart in the reference): after a tree change
+ * an instance reuses a program compiled earlier for the same tree (the key is
+ * the whole neighbour table, leaf flags and slot map) instead of compiling
+ * and uploading it again.  `entries` programs are kept per instance besides
+ * the current one (default 4; 0 disables the cache); changing it drops the
+ * cached ones. */
+int amr_set_fill_cache(amr_ctx *ctx, int entries);
+
 /* Kernel timing (no counterpart in the reference, which only keeps OpenMP
  * wall-clock accumulators, main.cpp:124-127): launches of the kernel families
  * selected by `mask` (bit 0 stage, 1 ghost fill, 2 CT, 3 CFL/criteria,
@@ -156,7 +164,8 @@ int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per
  * synchronisation (the stage kernel's events are attached to its dispatch);
  * bits 5-7 accumulate host wall time of the topology rebuild (work lists and
  * fill program), of the regrid decisions and of the per-step wait for the
- * device.  amr_profile_read returns the summed milliseconds and the number of
+ * device; bit 8 the host time of fill-program compiles, counting compiled
+ * programs (cache misses) as launches.  amr_profile_read returns the summed milliseconds and the number of
  * launches (or host scopes) of one family since the last amr_profile_enable. */
 int amr_profile_enable(amr_ctx *ctx, int mask);
 int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);

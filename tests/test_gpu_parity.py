@@ -197,6 +197,30 @@ def test_graph_replay_equals_direct_launches():
     assert g.info() == d.info()
 
 
+def test_fill_program_cache_equals_recompiling():
+    """Reusing cached fill programs for recurring trees equals compiling the
+    program at every tree change bit for bit, and the cache is hit."""
+    import amr2d_mhd_amd as A
+
+    cfg = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
+    a, b = A.Simulation(cfg), A.Simulation(cfg)
+    b.set_fill_cache(0)
+    for s in (a, b):
+        s.profile(("host_topology", "host_compile"))
+        s.init()
+    for k in range(900):
+        a.step()
+        b.step()
+        if k % 150 == 149:
+            diff = compare_states(gpu_state(a), gpu_state(b))
+            assert not diff, f"after step {k}: " + "; ".join(diff[:4])
+    assert a.info() == b.info()
+    pa, pb = a.profile_read(), b.profile_read()
+    rebuilds, compiled, compiled_plain = pa["host_topology"][1], pa["host_compile"][1], pb["host_compile"][1]
+    assert rebuilds == pb["host_topology"][1]
+    assert compiled < compiled_plain <= rebuilds, (compiled, compiled_plain, rebuilds)
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_stage_variant_bitwise(variant):
     """Every stage-kernel variant reproduces the oracle bit for bit."""
