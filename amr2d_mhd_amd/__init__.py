@@ -6,4 +6,4 @@ state stays in HBM; ``riemann_batch``/``recon_batch`` expose the pointwise
 kernels for parity checks.
 """
 from ._lib import AmrError, FIELDS, FIELD_VARS, FILL_KIND, SOLVERS, lib, make_config  # noqa: F401
-from .sim import Simulation, recon_batch, riemann_batch  # noqa: F401
+from .sim import Simulation, recon_batch, riemann_batch, run_group  # noqa: F401

@@ -142,12 +142,10 @@ struct PinnedVec {
 struct FillProg {
   std::vector<int32_t> key;
   uint64_t hash = 0;
-  PinnedVec<FillOp> ops;
-  std::vector<FillGen> gen;
-  FillOp* d_ops = nullptr;
-  int cap_ops = 0;
+  FillOp* d_ops = nullptr;  // device copy only: the host arrays are compile scratch
+  int cap_ops = 0, nops = 0;
   FillGen* d_gen = nullptr;
-  int cap_gen = 0;
+  int cap_gen = 0, ngen = 0;
   uint64_t used = 0;  // LRU stamp
 };
 
@@ -171,14 +169,15 @@ struct Instance {
   amr_diag err{};
   bool done = false;
   // ghost-fill program of this instance (global slot ids); recompiled only
-  // when its tree changed, kept in its own device segment
+  // when its tree changed, kept in its own device segment (nops ops, ngen
+  // explicit-operand entries); ops/gen are the host arrays of the last compile
   PinnedVec<FillOp> ops;
   std::vector<FillGen> gen;
   bool prog_dirty = true;
   FillOp* d_ops = nullptr;
-  int cap_ops = 0;
+  int cap_ops = 0, nops = 0;
   FillGen* d_gen = nullptr;
-  int cap_gen = 0;
+  int cap_gen = 0, ngen = 0;
   // the tree the current program was compiled for, and earlier programs
   std::vector<int32_t> prog_key;
   uint64_t prog_hash = 0;
@@ -227,6 +226,8 @@ struct amr_ctx {
   bool graph_on = true, graph_dirty = true, regrid_launched = false;
   // compiled fill programs kept per instance besides the current one (LRU)
   int prog_cache_max = 4;
+  // a step launched by step_begin and not yet completed by step_end
+  bool in_flight = false, step_trial = false;
   uint64_t prog_clock = 0;
   // stage-kernel variant: fixed (0..kStageVariants-1) or chosen per leaf-count
   // bucket by the tuner (-1): the first launches in a bucket try every variant
@@ -1231,12 +1232,12 @@ void program_key(const Instance& I, std::vector<int32_t>& key, uint64_t& hash) {
 }
 
 void swap_program(Instance& I, FillProg& P) {
-  std::swap(I.ops, P.ops);
-  std::swap(I.gen, P.gen);
   std::swap(I.d_ops, P.d_ops);
   std::swap(I.cap_ops, P.cap_ops);
+  std::swap(I.nops, P.nops);
   std::swap(I.d_gen, P.d_gen);
   std::swap(I.cap_gen, P.cap_gen);
+  std::swap(I.ngen, P.ngen);
   std::swap(I.prog_key, P.key);
   std::swap(I.prog_hash, P.hash);
 }
@@ -1245,8 +1246,9 @@ void swap_program(Instance& I, FillProg& P) {
 // tree is the one of the current program or of a cached one (the device copy
 // travels with it; the stream is idle here, so no launch still reads a
 // buffer that changes role).  Otherwise the current program is cached (the
-// least recently used entry gives up its buffers for the new compile), the
-// key is set and false is returned: compile into I.ops/I.gen and upload.
+// least recently used entry gives up its device buffers for the new
+// program), the key is set and false is returned: compile into I.ops/I.gen
+// and upload.
 bool reuse_program(amr_ctx* c, Instance& I) {
   std::vector<int32_t> key;
   uint64_t hash = 0;
@@ -1346,6 +1348,8 @@ int rebuild_topology(amr_ctx* c) {
   for (int qi : dirty) {
     Instance& I = c->inst[qi];
     const int n = (int)I.ops.size(), ng = (int)I.gen.size();
+    I.nops = n;
+    I.ngen = ng;
     if (int e = ensure(I.d_ops, I.cap_ops, std::max(1, n))) return e;
     if (int e = ensure(I.d_gen, I.cap_gen, std::max(1, ng))) return e;
     if (n) HIPCHK(hipMemcpyAsync(I.d_ops, I.ops.data(), sizeof(FillOp) * n, hipMemcpyHostToDevice, c->stream));
@@ -1362,7 +1366,7 @@ int rebuild_topology(amr_ctx* c) {
   for (auto& I : c->inst) {
     c->h_inst_ops.push_back(I.d_ops);
     c->h_inst_gen.push_back(I.d_gen);
-    total += I.done ? 0 : (int)(I.ops.size() / kFillChunk);
+    total += I.done ? 0 : I.nops / kFillChunk;
     c->h_prefix.push_back(total);
   }
   c->nchunks = total;
@@ -1672,26 +1676,39 @@ int capture_step(amr_ctx* c) {
 // Returns nonzero for a HIP failure, or when no instance is left running
 // and one failed during this call (its code); otherwise instance failures
 // are reported through amr_info().status.
-int step_all(amr_ctx* c, int* n_active) {
+// First half of a step: rebuild the work lists and fill programs if the
+// trees changed, then launch the step's device work (graph or direct) on the
+// context's stream and return without waiting.
+int step_begin(amr_ctx* c) {
   c->call_fail = 0;
-  if (!running(c)) {
-    if (n_active) *n_active = 0;
-    return c->diag.code;
-  }
+  c->in_flight = false;
+  if (!running(c)) return 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   fill_params(c);
   const bool device_timing = (c->prof_mask & 0x1fu) != 0;
-  const bool trial = tuning(c);
-  if (c->graph_on && !device_timing && !trial) {
+  c->step_trial = tuning(c);
+  if (c->graph_on && !device_timing && !c->step_trial) {
     if (c->graph_dirty || !c->gexec)
       if (int e = capture_step(c)) return e;
     HIPCHK(hipGraphLaunch(c->gexec, c->stream));
   } else {
     if (int e = enqueue_step(c)) return e;
   }
+  c->in_flight = true;
+  return 0;
+}
+
+// Second half: wait for the step, advance time and dt (main.cpp:231-238),
+// regrid (admesh) and retire finished instances.
+int step_end(amr_ctx* c, int* n_active) {
+  if (!c->in_flight) {
+    if (n_active) *n_active = running(c);
+    return running(c) ? 0 : c->diag.code;
+  }
+  c->in_flight = false;
   if (int e = wait_errors(c)) return e;
-  if (trial) tune_collect(c);
+  if (c->step_trial) tune_collect(c);
   int active = 0;
   for (size_t q = 0; q < c->inst.size(); q++) {
     Instance& I = c->inst[q];
@@ -1716,6 +1733,39 @@ int step_all(amr_ctx* c, int* n_active) {
   }
   if (n_active) *n_active = active;
   return (active == 0 && c->call_fail) ? c->call_fail : 0;
+}
+
+int step_all(amr_ctx* c, int* n_active) {
+  if (!running(c)) {
+    c->call_fail = 0;
+    if (n_active) *n_active = 0;
+    return c->diag.code;
+  }
+  if (int e = step_begin(c)) return e;
+  return step_end(c, n_active);
+}
+
+// leaf-cell count of each running instance (zone-updates of its next step)
+int64_t running_cells(amr_ctx* c, std::vector<int64_t>& leaves) {
+  const int64_t cells = (int64_t)c->nxi * c->nxi;
+  leaves.assign(c->inst.size(), 0);
+  int64_t total = 0;
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    const Instance& I = c->inst[q];
+    if (I.done) continue;
+    for (int nb = 0; nb < I.lastActive; nb++) leaves[q] += I.leafs[nb];
+    leaves[q] *= cells;
+    total += leaves[q];
+  }
+  return total;
+}
+
+// zone-updates of a completed step: the instances that did not fail in it
+int64_t completed_cells(amr_ctx* c, const std::vector<int64_t>& leaves) {
+  int64_t zu = 0;
+  for (size_t q = 0; q < c->inst.size(); q++)
+    if (!c->inst[q].status) zu += leaves[q];
+  return zu;
 }
 
 // end of a phase-level call: the first instance failure of the call, if any
@@ -1908,20 +1958,13 @@ int amr_step(amr_ctx* c, int* n_active) {
 int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
   if (!c) return AMR_ERR_CONFIG;
   int64_t n = 0, zu = 0;
-  const int64_t cells = (int64_t)c->nxi * c->nxi;
   int active = running(c);
-  std::vector<int64_t> leaves(c->inst.size());
+  std::vector<int64_t> leaves;
   while (active > 0 && (max_steps < 0 || n < max_steps)) {
-    for (size_t q = 0; q < c->inst.size(); q++) {
-      const Instance& I = c->inst[q];
-      leaves[q] = 0;
-      if (!I.done)
-        for (int nb = 0; nb < I.lastActive; nb++) leaves[q] += I.leafs[nb];
-    }
+    running_cells(c, leaves);
     const int e = step_all(c, &active);
     // a step counts for the instances that completed it
-    for (size_t q = 0; q < c->inst.size(); q++)
-      if (!c->inst[q].status) zu += leaves[q] * cells;
+    zu += completed_cells(c, leaves);
     if (e) {
       if (steps) *steps = n;
       if (zone_updates) *zone_updates = zu;
@@ -1932,6 +1975,62 @@ int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
   if (steps) *steps = n;
   if (zone_updates) *zone_updates = zu;
   return AMR_OK;
+}
+
+int amr_step_begin(amr_ctx* c) {
+  if (!c || c->in_flight) return AMR_ERR_CONFIG;
+  return step_begin(c);
+}
+
+int amr_step_end(amr_ctx* c, int* n_active) {
+  if (!c) return AMR_ERR_CONFIG;
+  return step_end(c, n_active);
+}
+
+int amr_run_group(int n_ctx, amr_ctx** ctxs, int max_steps, int64_t* steps, int64_t* zone_updates) {
+  if (n_ctx <= 0 || !ctxs) return AMR_ERR_CONFIG;
+  for (int k = 0; k < n_ctx; k++)
+    if (!ctxs[k] || ctxs[k]->in_flight) return AMR_ERR_CONFIG;
+  std::vector<std::vector<int64_t>> leaves(n_ctx);
+  std::vector<int64_t> done(n_ctx, 0);
+  int64_t zu = 0;
+  int err = 0;
+  auto launch = [&](int k) {
+    amr_ctx* c = ctxs[k];
+    if (!running(c) || (max_steps >= 0 && done[k] >= max_steps)) return 0;
+    running_cells(c, leaves[k]);
+    return step_begin(c);
+  };
+  // a context stops at its own end or error (like amr_run); the others go on
+  // unless the runtime itself failed
+  bool hip_failed = false;
+  auto note = [&](int e) {
+    if (e && !err) err = e;
+    if (e == AMR_ERR_HIP) hip_failed = true;
+  };
+  for (int k = 0; k < n_ctx && !hip_failed; k++) note(launch(k));
+  // round robin: while one context's host work (wait, regrid, rebuild,
+  // launch) runs, the steps already launched on the others keep the device busy
+  bool any = true;
+  while (any) {
+    any = false;
+    for (int k = 0; k < n_ctx; k++) {
+      amr_ctx* c = ctxs[k];
+      if (!c->in_flight) continue;
+      int active = 0;
+      const int e = step_end(c, &active);
+      zu += completed_cells(c, leaves[k]);
+      done[k]++;
+      note(e);
+      if (!e && !hip_failed) note(launch(k));
+      any = any || c->in_flight;
+    }
+  }
+  int64_t n = 0;
+  for (int k = 0; k < n_ctx; k++) n = std::max(n, done[k]);
+  if (steps) *steps = n;
+  if (zone_updates) *zone_updates = zu;
+  return err;
 }
 
 int amr_info(amr_ctx* c, int inst, amr_inst_info* o) {

@@ -183,6 +183,21 @@ class Simulation:
         return out
 
 
+def run_group(sims, max_steps=-1):
+    """Run several Simulations round robin (amr_run_group): while the host
+    completes one's step, the steps launched on the others keep the GPU busy.
+    Returns (largest step count, total zone-updates); on failure the raised
+    AmrError carries both."""
+    arr = (C.c_void_p * len(sims))(*[s.h for s in sims])
+    steps, zu = C.c_int64(0), C.c_int64(0)
+    rc = lib().amr_run_group(len(sims), arr, int(max_steps), C.byref(steps), C.byref(zu))
+    if rc:
+        e = AmrError(rc)
+        e.steps, e.zone_updates = steps.value, zu.value
+        raise e
+    return steps.value, zu.value
+
+
 def riemann_batch(method, direction, gamma, wL, wR):
     """riemannS on the GPU for torch cuda float64 tensors wL, wR of shape [n, 8]."""
     import torch

@@ -90,29 +90,44 @@ def cpu_baseline(steps, threads):
                        f"({zu} zone-updates in {tm['loop_seconds']:.2f} s), host CPU: {cpu_model()}")
 
 
-def ensemble_bench(steps, full, n_inst, plain=True, warmup=0):
+def ensemble_bench(steps, full, n_inst, plain=True, warmup=0, groups=2):
     """Batched ensemble of seeded parameterised rotors against the same
     instances run one after another, each alone in its own context (plain
-    launches).  Both legs time exactly `steps` steps of every instance from
-    its initial condition, so their zone-updates are identical; with `full`
-    the batch also runs every instance to t = 0.2."""
+    launches).  The batch is split into `groups` contexts of consecutive
+    instances run round robin (amr_run_group), so one group's host work
+    overlaps the other groups' device work.  Both legs time exactly `steps`
+    steps of every instance from its initial condition, so their
+    zone-updates are identical; with `full` the batch also runs every
+    instance to t = 0.2."""
     import torch
 
     import amr2d_mhd_amd as A
     from amr2d_mhd_amd.ensemble import SEED, ensemble_configs
 
     cfgs = ensemble_configs(n_inst)
-    batch = A.Simulation(cfgs)
-    batch.init()
+    groups = max(1, min(groups, len(cfgs)))
+    bounds = [round(len(cfgs) * k / groups) for k in range(groups + 1)]
+
+    def make_batch():
+        sims = [A.Simulation(cfgs[bounds[k]:bounds[k + 1]]) for k in range(groups)]
+        for b in sims:
+            b.init()
+        return sims
+
+    def statuses(sims):
+        return [b.info(q)["status"] for b in sims for q in range(b.n)]
+
+    batch = make_batch()
     if warmup:
-        batch.run(warmup)
+        A.run_group(batch, warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    nb, zb = batch.run(steps)
+    nb, zb = A.run_group(batch, steps)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
-    failed = [q for q in range(len(cfgs)) if batch.info(q)["status"]]
-    batch.close()
+    failed = [q for q, st in enumerate(statuses(batch)) if st]
+    for b in batch:
+        b.close()
     tp, zp = 0.0, 0
     for q, cfg in enumerate(cfgs if plain and not warmup else []):
         sim = A.Simulation(cfg)
@@ -135,22 +150,22 @@ def ensemble_bench(steps, full, n_inst, plain=True, warmup=0):
            "workload": "seeded rotor_p ensemble: rho_in U[5,20], v0 U[1,3], Bx U[0.5,2]/sqrt(4pi), p 5, "
                        "max level U{3..6}, MC-MUSCL, HLLC (even) / HLLD (odd), CT, RK2, CFL 0.5",
            "head_to_head_steps": steps, "batched_warmup_steps": warmup,
-           "batched": {"zone_updates": zb, "seconds": tb, "value": zb / tb},
+           "batched": {"zone_updates": zb, "seconds": tb, "value": zb / tb, "groups": groups},
            "plain_sequential": {"zone_updates": zp, "seconds": tp, "value": zp / tp if tp else None},
            "speedup": (zb / tb) / (zp / tp) if tp and zp else None}
     if full:
-        batch = A.Simulation(cfgs)
-        batch.init()
+        batch = make_batch()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        n, z = batch.run(-1)
+        n, z = A.run_group(batch, -1)
         torch.cuda.synchronize()
         tf = time.perf_counter() - t0
-        ts = [batch.info(q)["t"] for q in range(len(cfgs))]
+        ts = [b.info(q)["t"] for b in batch for q in range(b.n)]
         out["full_run"] = {"steps": n, "zone_updates": z, "seconds": tf, "value": z / tf,
-                           "aborted_instances": [q for q in range(len(cfgs)) if batch.info(q)["status"]],
+                           "aborted_instances": [q for q, st in enumerate(statuses(batch)) if st],
                            "t_min": min(ts), "t_max": max(ts)}
-        batch.close()
+        for b in batch:
+            b.close()
     return out
 
 
@@ -296,6 +311,8 @@ def main():
     ap.add_argument("--ens-full", action="store_true", help="also run the batched ensemble to t = 0.2")
     ap.add_argument("--ens-warmup", type=int, default=0,
                     help="untimed batched steps before the timed ones (skips the plain leg)")
+    ap.add_argument("--ens-groups", type=int, default=2,
+                    help="contexts the batch is split into, run round robin (1: one context)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="time only the batched ensemble (for profiling); prints its JSON object")
     ap.add_argument("--ot-steps", type=int, default=0,
@@ -321,6 +338,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.only_ensemble:
         print(json.dumps(ensemble_bench(args.ens_steps, args.ens_full, args.ensemble or 256, plain=False,
+                                        groups=args.ens_groups,
                                         warmup=args.ens_warmup)))
         return 0
     if args.copy_calibration:
@@ -386,7 +404,8 @@ def main():
     if args.adjoint:
         out["adjoint"] = adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd)
     if args.ensemble:
-        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup)
+        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup,
+                                         groups=args.ens_groups)
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)

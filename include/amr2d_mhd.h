@@ -89,6 +89,24 @@ int amr_step(amr_ctx *ctx, int *n_active);
  * number of steps and the leaf-cell updates performed in *zone_updates. */
 int amr_run(amr_ctx *ctx, int max_steps, int64_t *steps, int64_t *zone_updates);
 
+/* The two halves of amr_step (no counterpart in the reference): amr_step_begin
+ * rebuilds work lists and fill programs after tree changes and launches the
+ * step's device work on the context's stream without waiting; amr_step_end
+ * waits for it, advances t and dt, regrids and retires finished instances.
+ * A context's calls must alternate begin, end. */
+int amr_step_begin(amr_ctx *ctx);
+int amr_step_end(amr_ctx *ctx, int *n_active);
+
+/* amr_run over several contexts (e.g. an ensemble split into groups), each on
+ * its own stream, round robin: while the host completes one context's step
+ * and launches its next one, the steps already launched on the others keep
+ * the device busy.  Each context runs until its instances are done or fail,
+ * or max_steps (< 0: no limit) of its steps; *steps receives the largest
+ * step count, *zone_updates the total.  Returns the first error of any
+ * context (the others continue unless it is AMR_ERR_HIP).  Results of every
+ * instance are those of its own single run. */
+int amr_run_group(int n_ctx, amr_ctx **ctxs, int max_steps, int64_t *steps, int64_t *zone_updates);
+
 int amr_info(amr_ctx *ctx, int inst, amr_inst_info *info);
 int amr_last_error(amr_ctx *ctx, amr_diag *diag);
 

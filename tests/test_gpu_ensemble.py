@@ -63,6 +63,44 @@ def test_batch_equals_alone_bitwise():
         assert not d, f"instance {ids[q]}: " + "; ".join(d[:4])
 
 
+def test_grouped_run_equals_single_context_bitwise():
+    """The same instances split into round-robin groups (amr_run_group, one
+    stream per context, the steps of the groups overlapping on the GPU), and
+    stepped through amr_step_begin / amr_step_end, equal one context's run."""
+    import ctypes as C
+
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd._lib import lib
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    cfgs = [instance_config(i) for i in range(12)]
+    steps = 50
+    one = A.Simulation(cfgs)
+    one.init()
+    n1, z1 = one.run(steps)
+    groups = [A.Simulation(cfgs[0:5]), A.Simulation(cfgs[5:12])]
+    for g in groups:
+        g.init()
+    ng, zg = A.run_group(groups, steps)
+    assert (ng, zg) == (n1, z1)
+    split = A.Simulation(cfgs[5:12])
+    split.init()
+    active = C.c_int(0)
+    for _ in range(steps):
+        assert lib().amr_step_begin(split.h) == 0
+        assert lib().amr_step_begin(split.h) != 0  # a step is already in flight
+        assert lib().amr_step_end(split.h, C.byref(active)) == 0
+    for q in range(len(cfgs)):
+        g, gq = (groups[0], q) if q < 5 else (groups[1], q - 5)
+        assert one.info(q) == g.info(gq)
+        d = compare_states(gpu_state(one, q), gpu_state(g, gq))
+        assert not d, f"instance {q}: " + "; ".join(d[:4])
+        if q >= 5:
+            assert split.info(q - 5) == g.info(gq)
+            d = compare_states(gpu_state(split, q - 5), gpu_state(g, gq))
+            assert not d, f"instance {q} (begin/end): " + "; ".join(d[:4])
+
+
 def test_failing_instances_are_isolated():
     """At p = 1 the reference aborts many seeded rotors early (p <= 0 after the
     first refinements).  Each such instance must stop with the oracle's code at
