@@ -82,6 +82,8 @@ struct FillScratch {
   std::vector<uint8_t> ring;      // per cell of a tile: 1 in the ghost ring
   int ring_nt = 0;
   bool force_general = false;  // test hook: emit every restriction as kFillGeneral
+  bool interior_write = false;  // a fill wrote an interior element (never: rejected)
+  std::vector<int32_t> group_of, group_count;
 };
 
 // page-locked growable array, so uploads are true async DMA (falls back to
@@ -546,14 +548,20 @@ struct FillCompiler {
     if (S.cur.size() < need) S.cur.resize(need, -1);
     S.touched.clear();
     S.nodes.clear();
+    S.interior_write = false;
   }
   size_t addr(int nb, int i, int j) const { return (size_t)nb * N2 + j * NT + i; }
+  // fills write ghost-ring elements only: an interior element always holds
+  // its pre-fill value
   int64_t read(int nb, int i, int j) const {
-    const int64_t v = S.cur[addr(nb, i, j)];
-    return v >= 0 ? v : src_ref(nb, j * NT + i);
+    const int cell = j * NT + i;
+    if (!S.ring[cell]) return src_ref(nb, cell);
+    const int64_t v = S.cur[(size_t)nb * N2 + cell];
+    return v >= 0 ? v : src_ref(nb, cell);
   }
   void write(int nb, int i, int j, int64_t v) {
     const size_t a = addr(nb, i, j);
+    S.interior_write |= !S.ring[j * NT + i];
     if (S.cur[a] < 0) S.touched.push_back((nb << 16) | (j * NT + i));
     S.cur[a] = v;
   }
@@ -562,8 +570,12 @@ struct FillCompiler {
     RNode nd{};
     nd.nb = fn;
     nd.cell = (uint16_t)(j1 * NT + i1);
-    nd.simple = !S.force_general && S.cur[addr(fn, i1, j1)] < 0 && S.cur[addr(fn, i1, j1 + 1)] < 0 &&
-                S.cur[addr(fn, i1 + 1, j1)] < 0 && S.cur[addr(fn, i1 + 1, j1 + 1)] < 0;
+    const int c0 = j1 * NT + i1;
+    const uint8_t* ring = S.ring.data();
+    nd.simple = !S.force_general &&
+                ((!ring[c0] && !ring[c0 + 1] && !ring[c0 + NT] && !ring[c0 + NT + 1]) ||
+                 (S.cur[addr(fn, i1, j1)] < 0 && S.cur[addr(fn, i1, j1 + 1)] < 0 && S.cur[addr(fn, i1 + 1, j1)] < 0 &&
+                  S.cur[addr(fn, i1 + 1, j1 + 1)] < 0));
     if (!nd.simple) {
       nd.ref[0] = read(fn, i1, j1);
       nd.ref[1] = read(fn, i1, j1 + 1);
@@ -797,7 +809,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
   }
   FillCompiler fc(I, NT, ng, S);
   fc.run();
-  int status = 0;
+  int status = S.interior_write ? AMR_ERR_CONFIG : 0;
   // 1. ops in first-write order; a written element's cur[] entry becomes its
   //    op index (tagged) for the stale-read search below
   constexpr int64_t kOpTag = int64_t(1) << 61;
@@ -889,25 +901,52 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
     while (I.ops.size() % kFillChunk) I.ops.push_back(nop);
   };
   if (linked) {
-    for (int q : S.order) S.uf[q] = root(q);
-    std::sort(S.order.begin(), S.order.end(),
-              [&](int a, int b) { return S.uf[a] != S.uf[b] ? S.uf[a] < S.uf[b] : a < b; });
-    S.moved.clear();
+    // groups in the order of their first linked op, ops of a group in link
+    // order (a counting sort over the group index); op order inside a chunk
+    // does not matter, only that a group shares one chunk
+    const size_t no = S.order.size();
+    S.group_of.resize(no);
+    for (size_t t = 0; t < no; t++) S.group_of[t] = root(S.order[t]);
+    // number the groups; a root's union-find entry is not needed any more and
+    // holds -(group + 2) from here on
+    S.group_count.clear();
+    for (size_t t = 0; t < no; t++) {
+      const int r = S.group_of[t];
+      int g;
+      if (S.uf[r] >= 0) {
+        g = (int)S.group_count.size();
+        S.group_count.push_back(0);
+        S.uf[r] = -(g + 2);
+      } else {
+        g = -S.uf[r] - 2;
+      }
+      S.group_of[t] = g;
+      S.group_count[g] += 1;
+    }
+    const size_t ng_ = S.group_count.size();
+    std::vector<int32_t>& start = S.group_count;  // exclusive prefix sums in place
+    int32_t acc = 0;
+    for (size_t g = 0; g < ng_; g++) {
+      const int32_t cnt = start[g];
+      if (cnt > kFillChunk) return AMR_ERR_CONFIG;
+      start[g] = acc;
+      acc += cnt;
+    }
+    S.moved.resize(no);
     FillOp* p = I.ops.data();
-    for (int q : S.order) {
-      S.moved.push_back(p[q]);
+    for (size_t t = 0; t < no; t++) {
+      const int q = S.order[t];
+      S.moved[start[S.group_of[t]]++] = p[q];
       p[q] = nop;
     }
     pad_to_chunk();
-    size_t g = 0;
-    while (g < S.order.size()) {
-      size_t h = g;
-      while (h < S.order.size() && S.uf[S.order[h]] == S.uf[S.order[g]]) h++;
-      const size_t m = h - g;
-      if (m > (size_t)kFillChunk) return AMR_ERR_CONFIG;
+    // start[g] now holds the end of group g
+    size_t b = 0;
+    for (size_t g = 0; g < ng_; g++) {
+      const size_t e = (size_t)start[g], m = e - b;
       if (I.ops.size() % kFillChunk + m > (size_t)kFillChunk) pad_to_chunk();
-      for (size_t t = g; t < h; t++) I.ops.push_back(S.moved[t]);
-      g = h;
+      for (size_t t = b; t < e; t++) I.ops.push_back(S.moved[t]);
+      b = e;
     }
   }
   pad_to_chunk();
