@@ -67,7 +67,10 @@ struct RNode {
 
 // host scratch reused across compiles (cur[] is reset only where touched)
 struct FillScratch {
-  std::vector<int64_t> cur;       // symbolic value per element, -1 = pre-fill
+  // symbolic value per ghost-ring element (the only ones fills write),
+  // [block][ring index]: -1 = pre-fill value, >= 0 = the pre-fill value of
+  // element (nb << 16) | cell, <= -2 = restriction node -(v + 2)
+  std::vector<int32_t> cur;
   std::vector<int32_t> touched;   // written elements, (nb << 16) | cell, in first-write order
   std::vector<RNode> nodes;
   struct RingSrc {
@@ -80,6 +83,8 @@ struct FillScratch {
   std::vector<int32_t> order;
   std::vector<FillOp> moved;
   std::vector<uint8_t> ring;      // per cell of a tile: 1 in the ghost ring
+  std::vector<int32_t> ring_idx;  // per cell: index among the ring cells, -1 inside
+  int nring = 0;
   int ring_nt = 0;
   bool force_general = false;  // test hook: emit every restriction as kFillGeneral
   bool interior_write = false;  // a fill wrote an interior element (never: rejected)
@@ -544,38 +549,42 @@ struct FillCompiler {
   FillScratch& S;
 
   FillCompiler(const Instance& inst, int NT_, int ng_, FillScratch& s) : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), S(s) {
-    const size_t need = (size_t)I.lastActive * N2;
+    const size_t need = (size_t)I.lastActive * S.nring;
     if (S.cur.size() < need) S.cur.resize(need, -1);
     S.touched.clear();
     S.nodes.clear();
     S.interior_write = false;
   }
-  size_t addr(int nb, int i, int j) const { return (size_t)nb * N2 + j * NT + i; }
+  static int32_t lref(int nb, int cell) { return (nb << 16) | cell; }
+  // table slot of ring cell `cell` of block nb
+  size_t slot(int nb, int cell) const { return (size_t)nb * S.nring + S.ring_idx[cell]; }
   // fills write ghost-ring elements only: an interior element always holds
   // its pre-fill value
-  int64_t read(int nb, int i, int j) const {
+  int32_t read(int nb, int i, int j) const {
     const int cell = j * NT + i;
-    if (!S.ring[cell]) return src_ref(nb, cell);
-    const int64_t v = S.cur[(size_t)nb * N2 + cell];
-    return v >= 0 ? v : src_ref(nb, cell);
+    if (!S.ring[cell]) return lref(nb, cell);
+    const int32_t v = S.cur[slot(nb, cell)];
+    return v != -1 ? v : lref(nb, cell);
   }
-  void write(int nb, int i, int j, int64_t v) {
-    const size_t a = addr(nb, i, j);
-    S.interior_write |= !S.ring[j * NT + i];
-    if (S.cur[a] < 0) S.touched.push_back((nb << 16) | (j * NT + i));
+  void write(int nb, int i, int j, int32_t v) {
+    const int cell = j * NT + i;
+    if (!S.ring[cell]) {
+      S.interior_write = true;
+      return;
+    }
+    const size_t a = slot(nb, cell);
+    if (S.cur[a] == -1) S.touched.push_back((nb << 16) | cell);
     S.cur[a] = v;
   }
+  bool unwritten(int nb, int cell) const { return !S.ring[cell] || S.cur[slot(nb, cell)] == -1; }
   void copy(int dn, int di, int dj, int sn, int si, int sj) { write(dn, di, dj, read(sn, si, sj)); }
   void restrict_(int dn, int di, int dj, int fn, int i1, int j1) {
     RNode nd{};
     nd.nb = fn;
     nd.cell = (uint16_t)(j1 * NT + i1);
     const int c0 = j1 * NT + i1;
-    const uint8_t* ring = S.ring.data();
-    nd.simple = !S.force_general &&
-                ((!ring[c0] && !ring[c0 + 1] && !ring[c0 + NT] && !ring[c0 + NT + 1]) ||
-                 (S.cur[addr(fn, i1, j1)] < 0 && S.cur[addr(fn, i1, j1 + 1)] < 0 && S.cur[addr(fn, i1 + 1, j1)] < 0 &&
-                  S.cur[addr(fn, i1 + 1, j1 + 1)] < 0));
+    nd.simple = !S.force_general && unwritten(fn, c0) && unwritten(fn, c0 + NT) && unwritten(fn, c0 + 1) &&
+                unwritten(fn, c0 + NT + 1);
     if (!nd.simple) {
       nd.ref[0] = read(fn, i1, j1);
       nd.ref[1] = read(fn, i1, j1 + 1);
@@ -583,10 +592,10 @@ struct FillCompiler {
       nd.ref[3] = read(fn, i1 + 1, j1 + 1);
     }
     S.nodes.push_back(nd);
-    write(dn, di, dj, kNodeBit | (int64_t)(S.nodes.size() - 1));
+    write(dn, di, dj, -(int32_t)S.nodes.size() - 1);  // node index size-1 -> -(index + 2)
   }
   void finish() {
-    for (int32_t t : S.touched) S.cur[(size_t)(t >> 16) * N2 + (t & 0xffff)] = -1;
+    for (int32_t t : S.touched) S.cur[slot(t >> 16, t & 0xffff)] = -1;
   }
   // corner of an entry: the diagonal block `cb` found through the neighbour
   // table is copied when it is a leaf (with same-level indices even if it is
@@ -801,34 +810,41 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
   const int off[4] = {0, NT, 1, NT + 1};
   if (S.ring_nt != NT) {
     S.ring.assign(N2, 0);
+    S.ring_idx.assign(N2, -1);
+    S.nring = 0;
     for (int c = 0; c < N2; c++) {
       const int i = c % NT, j = c / NT;
       S.ring[c] = (i < ng || i >= NT - ng || j < ng || j >= NT - ng) ? 1 : 0;
+      if (S.ring[c]) S.ring_idx[c] = S.nring++;
     }
     S.ring_nt = NT;
+    S.cur.clear();
   }
+  if (I.lastActive > 32767) return AMR_ERR_CONFIG;  // block ids are 15-bit in the symbolic values
   FillCompiler fc(I, NT, ng, S);
   fc.run();
   int status = S.interior_write ? AMR_ERR_CONFIG : 0;
   // 1. ops in first-write order; a written element's cur[] entry becomes its
-  //    op index (tagged) for the stale-read search below
-  constexpr int64_t kOpTag = int64_t(1) << 61;
+  //    op index (>= 0; -1 where the value is unchanged) for the stale-read
+  //    search below
   S.ringsrc.clear();
   S.order.clear();
   const uint8_t* ring = S.ring.data();
+  const int32_t* ridx = S.ring_idx.data();
   const int32_t* gs = I.gs.data();
+  const int nring = S.nring;
   I.ops.reserve(S.touched.size() + 2 * kFillChunk);
   for (int32_t t : S.touched) {
     const int nb = t >> 16, cell = t & 0xffff;
-    const size_t a = (size_t)nb * N2 + cell;
-    const int64_t v = S.cur[a];
+    const size_t a = (size_t)nb * nring + ridx[cell];
+    const int32_t v = S.cur[a];
     FillOp op{};
     op.dst_gs = gs[nb];
     op.dst_cell = (uint16_t)cell;
     FillScratch::RingSrc rs;
     rs.nref = 0;
-    if (!(v & kNodeBit)) {
-      if (v == src_ref(nb, cell)) {  // value unchanged
+    if (v >= 0) {
+      if (v == t) {  // value unchanged
         S.cur[a] = -1;
         continue;
       }
@@ -837,21 +853,21 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
       op.src_cell = (uint16_t)(v & 0xffff);
       if (ring[op.src_cell]) { rs.nref = 1; rs.ref[0] = v; }
     } else {
-      const RNode& nd = S.nodes[v & ~kNodeBit];
+      const RNode& nd = S.nodes[-v - 2];
       if (nd.simple && !S.force_general) {
         op.op = kFillRestrict;
         op.src = gs[nd.nb];
         op.src_cell = nd.cell;
         if (ring[nd.cell] | ring[nd.cell + 1] | ring[nd.cell + NT] | ring[nd.cell + NT + 1]) {
           rs.nref = 4;
-          for (int q = 0; q < 4; q++) rs.ref[q] = src_ref(nd.nb, nd.cell + off[q]);
+          for (int q = 0; q < 4; q++) rs.ref[q] = (nd.nb << 16) | (nd.cell + off[q]);
         }
       } else {
         FillGen g{};
         rs.nref = 4;
         for (int q = 0; q < 4; q++) {
-          const int64_t r = nd.simple ? src_ref(nd.nb, nd.cell + off[q]) : nd.ref[q];
-          if (r & kNodeBit) status = AMR_ERR_CONFIG;
+          const int64_t r = nd.simple ? (int64_t)((nd.nb << 16) | (nd.cell + off[q])) : nd.ref[q];
+          if (r < 0) status = AMR_ERR_CONFIG;  // a restriction of a restriction
           rs.ref[q] = r;
           g.ref[q] = src_ref(gs[r >> 16], (int)(r & 0xffff));
         }
@@ -864,7 +880,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
       rs.op = (int32_t)I.ops.size();
       S.ringsrc.push_back(rs);
     }
-    S.cur[a] = kOpTag | (int64_t)I.ops.size();
+    S.cur[a] = (int32_t)I.ops.size();
     I.ops.push_back(op);
   }
   const int n = (int)I.ops.size();
@@ -879,14 +895,17 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
   for (const auto& rs : S.ringsrc)
     for (int t = 0; t < rs.nref; t++) {
       const int64_t ref = rs.ref[t];
-      const int64_t w = S.cur[(size_t)(ref >> 16) * N2 + (ref & 0xffff)];
+      if (ref < 0) continue;
+      const int rc = (int)(ref & 0xffff);
+      if (!ring[rc]) continue;
+      const int32_t w = S.cur[(size_t)(ref >> 16) * nring + ridx[rc]];
       const int q = rs.op;
-      if (w < 0 || !(w & kOpTag) || (int)(w & ~kOpTag) == q) continue;
+      if (w < 0 || w == q) continue;
       if (!linked) {
         S.uf.assign(n, -1);
         linked = true;
       }
-      const int wi = (int)(w & ~kOpTag);
+      const int wi = w;
       if (S.uf[q] < 0) { S.uf[q] = q; S.order.push_back(q); }
       if (S.uf[wi] < 0) { S.uf[wi] = wi; S.order.push_back(wi); }
       const int ra = root(q), rb = root(wi);
