@@ -81,7 +81,7 @@ _SIGS = {
     "amr_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
-    "amr_adj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    "amr_adj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.POINTER(C.c_void_p)]),
     "amr_adj_destroy": (C.c_int, [C.c_void_p]),
     "amr_adj_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]),
@@ -96,6 +96,9 @@ _SIGS = {
     "amr_debug_side_roe": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle_vjp": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_hlle": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "amr_debug_hlld_vjp": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "amr_debug_hlld": (None, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "amr_debug_recon_minmod_vjp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "amr_debug_prim_vjp": (None, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_host_fill": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -1,6 +1,6 @@
 """Discrete adjoint of the frozen-mesh step, behind a torch.autograd.Function.
 
-The forward is the reference's RK2 step (minmod MUSCL + HLLE, CT) on one
+The forward is the reference's RK2 step (minmod MUSCL + HLLE or HLLD, CT) on one
 uniform periodic N x N grid at a frozen dt sequence (``amr_adj_step``); the
 backward runs the hand-written reverse kernels (``amr_adj_vjp_tape``) over
 per-step tapes (the intermediates of one step, 24 N^2 doubles).  With
@@ -30,12 +30,12 @@ def _stream():
 
 
 class FrozenMesh:
-    """One uniform periodic N x N grid of cell size dx."""
+    """One uniform periodic N x N grid of cell size dx; flux "HLLE" or "HLLD"."""
 
-    def __init__(self, n, dx, gamma):
-        self.n, self.dx, self.gamma = int(n), float(dx), float(gamma)
+    def __init__(self, n, dx, gamma, flux="HLLE"):
+        self.n, self.dx, self.gamma, self.flux = int(n), float(dx), float(gamma), flux
         h = C.c_void_p()
-        rc = lib().amr_adj_create(self.n, self.dx, self.gamma, C.byref(h))
+        rc = lib().amr_adj_create(self.n, self.dx, self.gamma, flux.encode(), C.byref(h))
         if rc:
             raise AmrError(rc)
         self.h = h

@@ -74,6 +74,7 @@ __device__ inline void face_states(const Grid& G, const double* W, const double*
   }
 }
 
+template <int SOLVER>
 __global__ __launch_bounds__(kT) void k_flux(Grid G, const double* __restrict__ W, const double* __restrict__ Bx,
                                              const double* __restrict__ By, double* __restrict__ Fx,
                                              double* __restrict__ Fy) {
@@ -84,7 +85,12 @@ __global__ __launch_bounds__(kT) void k_flux(Grid G, const double* __restrict__ 
   double wl[8], wr[8], fl[8];
   size_t cell[4];
   face_states(G, W, dir ? By : Bx, dir, i, j, wl, wr, cell);
-  flux_hlle(wl, wr, G.g, fl);
+  if (SOLVER == kFluxHLLD) {
+    for (int k = 0; k < 8; k++) fl[k] = 0.;
+    flux_hlld(wl, wr, G.g, fl);
+  } else {
+    flux_hlle(wl, wr, G.g, fl);
+  }
   if (dir) swap_xy(fl);
   double* F = dir ? Fy : Fx;
   for (int k = 0; k < 8; k++) F[k * G.n2 + f] = fl[k];
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(kT) void k_flux_b(Grid G, const double* __restrict_
 // reverse of reconstruction + HLLE at one face (i, j) of direction DIR: the
 // adjoints (wlb, wrb) of its two edge states in the grid frame, the stencil
 // cells (i-2 .. i+1 along the normal) and the normal-B adjoint
-template <int DIR>
+template <int DIR, int SOLVER>
 __device__ inline void face_reverse(const Grid& G, int i, int j, const double* __restrict__ W,
                                     const double* __restrict__ Bn, const double* __restrict__ Fb, double* wlb,
                                     double* wrb, size_t* cell) {
@@ -292,7 +298,8 @@ __device__ inline void face_reverse(const Grid& G, int i, int j, const double* _
     wrb[k] = 0.;
   }
   if (DIR) swap_xy(fb);  // adjoint of the output swap
-  hlle_vjp(wl, wr, G.g, fb, wlb, wrb);
+  if (SOLVER == kFluxHLLD) hlld_vjp(wl, wr, G.g, fb, wlb, wrb);
+  else hlle_vjp(wl, wr, G.g, fb, wlb, wrb);
   if (DIR) {
     swap_xy(wlb);  // adjoint of the input swap
     swap_xy(wrb);
@@ -322,14 +329,14 @@ __device__ inline void stencil_reverse(const Grid& G, const double* __restrict__
   }
 }
 
-template <int DIR>
+template <int DIR, int SOLVER>
 __device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __restrict__ W,
                                      const double* __restrict__ Bn, const double* __restrict__ Fb,
                                      double* __restrict__ S, double* __restrict__ Bnb) {
   const int i = (int)(f % G.n), j = (int)(f / G.n);
   double wlb[8], wrb[8];
   size_t cell[4];
-  face_reverse<DIR>(G, i, j, W, Bn, Fb, wlb, wrb, cell);
+  face_reverse<DIR, SOLVER>(G, i, j, W, Bn, Fb, wlb, wrb, cell);
   constexpr int nc = DIR ? 6 : 5;
   Bnb[f] += wlb[nc] + wrb[nc];
   stencil_reverse<DIR>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
@@ -337,13 +344,13 @@ __device__ inline void flux_vjp_face(const Grid& G, size_t f, const double* __re
   });
 }
 
-template <int DIR>
+template <int DIR, int SOLVER>
 __global__ __launch_bounds__(kT) void k_flux_vjp(Grid G, const double* __restrict__ W, const double* __restrict__ Bn,
                                                  const double* __restrict__ Fb, double* __restrict__ S,
                                                  double* __restrict__ Bnb) {
   const size_t f = (size_t)blockIdx.x * kT + threadIdx.x;
   if (f >= G.n2) return;
-  flux_vjp_face<DIR>(G, f, W, Bn, Fb, S, Bnb);
+  flux_vjp_face<DIR, SOLVER>(G, f, W, Bn, Fb, S, Bnb);
 }
 
 // per-cell sum of the stencil contributions, then the cons->prim reverse
@@ -380,7 +387,7 @@ __global__ __launch_bounds__(kT) void k_gather_prim_vjp(Grid G, const double* __
 // tiles that read them; the normal-B adjoint of a face is added only by the
 // tile that owns the face.  Deterministic: fixed summation order, no
 // atomics.
-template <int TW, int TH, int TT>
+template <int TW, int TH, int TT, int SOLVER>
 __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restrict__ W,
                                                   const double* __restrict__ Bx, const double* __restrict__ By,
                                                   const double* __restrict__ Fbx, const double* __restrict__ Fby,
@@ -402,7 +409,7 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
     if (fj < G.n) {
       double wlb[8], wrb[8];
       size_t cell[4];
-      face_reverse<0>(G, fi, fj, W, Bx, Fbx, wlb, wrb, cell);
+      face_reverse<0, SOLVER>(G, fi, fj, W, Bx, Fbx, wlb, wrb, cell);
       if (q >= 1 && q <= TW && i0 - 1 + q < G.n) Bnbx[(size_t)fj * G.n + fi] += wlb[5] + wrb[5];
       stencil_reverse<0>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
         for (int s = 0; s < 4; s++) S[k][s][tid] = vb[s];
@@ -426,7 +433,7 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
     if (fi < G.n) {
       double wlb[8], wrb[8];
       size_t cell[4];
-      face_reverse<1>(G, fi, fj, W, By, Fby, wlb, wrb, cell);
+      face_reverse<1, SOLVER>(G, fi, fj, W, By, Fby, wlb, wrb, cell);
       if (r >= 1 && r <= TH && j0 - 1 + r < G.n) Bnby[(size_t)fj * G.n + fi] += wlb[6] + wrb[6];
       stencil_reverse<1>(G, W, cell, wlb, wrb, [&](int k, const double* vb) {
         for (int s = 0; s < 4; s++) S[k][s][tid] = vb[s];
@@ -449,11 +456,11 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
   }
 }
 
-template <int TW, int TH, int TT>
+template <int TW, int TH, int TT, int SOLVER>
 void launch_stage_vjp(const Grid& G, const double* W, const double* Bx, const double* By, const double* Fbx,
                       const double* Fby, double* Bnbx, double* Bnby, double* Qb, hipStream_t s) {
   const dim3 grid((unsigned)((G.n + TW - 1) / TW), (unsigned)((G.n + TH - 1) / TH));
-  k_stage_vjp<TW, TH, TT><<<grid, TT, 0, s>>>(G, W, Bx, By, Fbx, Fby, Bnbx, Bnby, Qb);
+  k_stage_vjp<TW, TH, TT, SOLVER><<<grid, TT, 0, s>>>(G, W, Bx, By, Fbx, Fby, Bnbx, Bnby, Qb);
 }
 
 unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -462,6 +469,7 @@ unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 struct amr_adj {
   Grid G;
+  int solver = kFluxHLLE;  // kFluxHLLE or kFluxHLLD
   int fused = 1;  // face reverse + gather in one tiled kernel: 1 16x13 tiles of 256 lanes, 2 32x13 tiles of
                   // 512 lanes; 0: per-face kernels and a gather
   // forward scratch
@@ -482,20 +490,25 @@ namespace {
     }                                                                                          \
   } while (0)
 
+void launch_flux(amr_adj* a, const double* W, const double* Bx, const double* By, hipStream_t s) {
+  const dim3 b2(blocks(a->G.n2), 2);
+  if (a->solver == kFluxHLLD) k_flux<kFluxHLLD><<<b2, kT, 0, s>>>(a->G, W, Bx, By, a->Fx, a->Fy);
+  else k_flux<kFluxHLLE><<<b2, kT, 0, s>>>(a->G, W, Bx, By, a->Fx, a->Fy);
+}
+
 // forward stage sequence, writing the step's tape
 int forward(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo, double* Bxo,
             double* Byo, const Tape& T, hipStream_t s) {
   const Grid& G = a->G;
   const unsigned b1 = blocks(G.n2);
-  const dim3 b2(b1, 2);
   k_prim<<<b1, kT, 0, s>>>(G, U, T.W1);
-  k_flux<<<b2, kT, 0, s>>>(G, T.W1, Bx, By, a->Fx, a->Fy);
+  launch_flux(a, T.W1, Bx, By, s);
   k_update<<<b1, kT, 0, s>>>(G, U, nullptr, a->Fx, a->Fy, dt, 1, a->Us);
   k_emf<<<b1, kT, 0, s>>>(G, a->Fx, a->Fy, a->E);
   k_faces<<<b1, kT, 0, s>>>(G, Bx, By, nullptr, nullptr, a->E, dt, 1, T.Bxs, T.Bys, nullptr, nullptr);
   k_bsync<<<b1, kT, 0, s>>>(G, a->Us, T.Bxs, T.Bys, T.pre1);
   k_prim<<<b1, kT, 0, s>>>(G, a->Us, T.W2);
-  k_flux<<<b2, kT, 0, s>>>(G, T.W2, T.Bxs, T.Bys, a->Fx, a->Fy);
+  launch_flux(a, T.W2, T.Bxs, T.Bys, s);
   k_update<<<b1, kT, 0, s>>>(G, U, a->Us, a->Fx, a->Fy, dt, 2, Uo);
   k_emf<<<b1, kT, 0, s>>>(G, a->Fx, a->Fy, a->E);
   k_faces<<<b1, kT, 0, s>>>(G, Bx, By, T.Bxs, T.Bys, a->E, dt, 2, Bxo, Byo, T.Bxo, T.Byo);
@@ -509,14 +522,22 @@ int forward(amr_adj* a, const double* U, const double* Bx, const double* By, dou
 void face_and_gather(amr_adj* a, const double* W, const double* Bx, const double* By, double* Bnbx, double* Bnby,
                      double* Qb, hipStream_t s) {
   const Grid& G = a->G;
+  const bool hlld = a->solver == kFluxHLLD;
   if (a->fused == 1) {
-    launch_stage_vjp<16, 13, 256>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+    if (hlld) launch_stage_vjp<16, 13, 256, kFluxHLLD>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+    else launch_stage_vjp<16, 13, 256, kFluxHLLE>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
   } else if (a->fused == 2) {
-    launch_stage_vjp<32, 13, 512>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+    if (hlld) launch_stage_vjp<32, 13, 512, kFluxHLLD>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
+    else launch_stage_vjp<32, 13, 512, kFluxHLLE>(G, W, Bx, By, a->Fbx, a->Fby, Bnbx, Bnby, Qb, s);
   } else {
     const unsigned b1 = blocks(G.n2);
-    k_flux_vjp<0><<<b1, kT, 0, s>>>(G, W, Bx, a->Fbx, a->Sx, Bnbx);
-    k_flux_vjp<1><<<b1, kT, 0, s>>>(G, W, By, a->Fby, a->Sy, Bnby);
+    if (hlld) {
+      k_flux_vjp<0, kFluxHLLD><<<b1, kT, 0, s>>>(G, W, Bx, a->Fbx, a->Sx, Bnbx);
+      k_flux_vjp<1, kFluxHLLD><<<b1, kT, 0, s>>>(G, W, By, a->Fby, a->Sy, Bnby);
+    } else {
+      k_flux_vjp<0, kFluxHLLE><<<b1, kT, 0, s>>>(G, W, Bx, a->Fbx, a->Sx, Bnbx);
+      k_flux_vjp<1, kFluxHLLE><<<b1, kT, 0, s>>>(G, W, By, a->Fby, a->Sy, Bnby);
+    }
     k_gather_prim_vjp<<<b1, kT, 0, s>>>(G, W, a->Sx, a->Sy, Qb);
   }
 }
@@ -546,9 +567,13 @@ int reverse(amr_adj* a, const Tape& T, const double* Bx, const double* By, doubl
 
 extern "C" {
 
-int amr_adj_create(int n, double dx, double gamma, amr_adj** out) {
+int amr_adj_create(int n, double dx, double gamma, const char* flux, amr_adj** out) {
   if (!out || n < 4 || dx <= 0) return AMR_ERR_CONFIG;
+  int solver = kFluxHLLE;
+  if (flux && std::strcmp(flux, "HLLD") == 0) solver = kFluxHLLD;
+  else if (flux && std::strcmp(flux, "HLLE") != 0) return AMR_ERR_CONFIG;
   amr_adj* a = new amr_adj();
+  a->solver = solver;
   a->G = Grid{n, dx, gamma, (size_t)n * n};
   const size_t n2 = a->G.n2;
   double** bufs8[] = {&a->Fx, &a->Fy, &a->Us, &a->Uo, &a->Qb2, &a->Usb, &a->Fbx, &a->Fby};
@@ -625,6 +650,18 @@ void amr_debug_hlle_vjp(const double* wL, const double* wR, double gamma, const 
 }
 
 void amr_debug_hlle(const double* wL, const double* wR, double gamma, double* f) { flux_hlle(wL, wR, gamma, f); }
+
+// HLLD reverse on the host; f receives the flux its own forward computed
+void amr_debug_hlld_vjp(const double* wL, const double* wR, double gamma, const double* fb, double* wLb, double* wRb,
+                        double* f) {
+  for (int k = 0; k < 8; k++) wLb[k] = wRb[k] = 0.;
+  hlld_vjp(wL, wR, gamma, fb, wLb, wRb, f);
+}
+
+void amr_debug_hlld(const double* wL, const double* wR, double gamma, double* f) {
+  for (int k = 0; k < 8; k++) f[k] = 0.;
+  flux_hlld(wL, wR, gamma, f);
+}
 
 // forward values of the reverse's own forward pieces next to those of the
 // engine's: out[0..7] = side_fwd (cf, e, H) of both sides and roe_fwd (lmin,

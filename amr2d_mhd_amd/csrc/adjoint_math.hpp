@@ -239,6 +239,62 @@ __host__ __device__ inline void side_bwd(const double* w, double g, double e, do
   fast_bwd(w, g, f, iw0, cfb, wb);
 }
 
+// side reverse with general adjoints of q[8] and F[8], plus adjoints of e
+// and pt used outside q and F (HLLD star states), H and the fast speed
+__host__ __device__ inline void side_bwd_vec(const double* w, double g, double e, double H, const FastT& f,
+                                             const double* qb, const double* Fb, double Hb, double eb_x, double ptb_x,
+                                             double cfb, double* wb) {
+  const double iw0 = 1. / w[0];
+  const double vn = w[1], Bn = w[5];
+  const double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
+  const double mv = w[0] * vn;
+  const double vb_dot = w[1] * w[5] + w[2] * w[6] + w[3] * w[7];
+  double mvb = Fb[0] + Fb[1] * w[1] + Fb[2] * w[2] + Fb[3] * w[3] + Fb[4] * H;
+  double vnb = Fb[5] * w[5] + Fb[6] * w[6] + Fb[7] * w[7];
+  const double Bnb = -(Fb[1] * w[5] + Fb[2] * w[6] + Fb[3] * w[7] + Fb[4] * vb_dot + Fb[5] * w[1] + Fb[6] * w[2] +
+                       Fb[7] * w[3]);
+  const double ptb = Fb[1] + ptb_x;
+  Hb += Fb[4] * mv;
+  const double vdb = -Fb[4] * Bn;
+  wb[1] += Fb[1] * mv + vdb * w[5] - Fb[5] * Bn;
+  wb[2] += Fb[2] * mv + vdb * w[6] - Fb[6] * Bn;
+  wb[3] += Fb[3] * mv + vdb * w[7] - Fb[7] * Bn;
+  wb[5] += -Fb[1] * Bn + vdb * w[1] + Fb[5] * vn;
+  wb[6] += -Fb[2] * Bn + vdb * w[2] + Fb[6] * vn;
+  wb[7] += -Fb[3] * Bn + vdb * w[3] + Fb[7] * vn;
+  wb[0] += mvb * vn;
+  vnb += mvb * w[0];
+  wb[4] += ptb;
+  double b2b = 0.5 * ptb;
+  wb[0] += qb[0] + qb[1] * w[1] + qb[2] * w[2] + qb[3] * w[3];
+  wb[1] += qb[1] * w[0];
+  wb[2] += qb[2] * w[0];
+  wb[3] += qb[3] * w[0];
+  double eb = qb[4] + eb_x;
+  wb[5] += qb[5];
+  wb[6] += qb[6];
+  wb[7] += qb[7];
+  eb += Hb * iw0;
+  wb[4] += Hb * iw0;
+  b2b += 0.5 * Hb * iw0;
+  wb[0] -= Hb * H * iw0;
+  const double e1b = eb * w[0];
+  wb[0] += eb * (e * iw0);
+  wb[4] += e1b * iw0 / (g - 1);
+  wb[0] -= e1b * (w[4] * iw0 / (g - 1)) * iw0;
+  wb[1] += e1b * w[1];
+  wb[2] += e1b * w[2];
+  wb[3] += e1b * w[3];
+  b2b += e1b * 0.5 * iw0;
+  wb[0] -= e1b * (0.5 * b2 * iw0) * iw0;
+  wb[5] += b2b * 2. * w[5];
+  wb[6] += b2b * 2. * w[6];
+  wb[7] += b2b * 2. * w[7];
+  wb[1] += vnb;
+  wb[5] += Bnb;
+  fast_bwd(w, g, f, iw0, cfb, wb);
+}
+
 // Roe-average bounds (roe_bounds) with the intermediates of the reverse
 struct RoeT {
   double sl, sr, nrm, v[3], B[3], RT, d, d6, d7, X, Y, h, btsq, cY, vax2, a2raw, a2, ct2, tdif, sq, cfast;
@@ -414,6 +470,367 @@ __host__ __device__ inline void hlle_vjp(const double* wL, const double* wR, dou
   // FL_b = SR Nb, qL_b = -SL SR Nb; FR_b = -SL Nb, qR_b = SL SR Nb
   side_bwd(wL, g, eL, HL, fL, Nb, SR, -SL * SR, HLb, cfLb, wLb);
   side_bwd(wR, g, eR, HR, fR, Nb, -SL, SL * SR, HRb, cfRb, wRb);
+}
+
+// ------------------------------------------------------------- HLLD flux
+// star state (hlld_star) reverse: (wsb, esb) -> adjoints of the side's w, S,
+// SM, pts, e and pt (accumulated); ws/es are the forward's outputs
+__host__ __device__ inline void hlld_star_bwd(const double* w, double S, double SM, double pts, double e, double pt,
+                                              const double* ws, double es, double* wsb, double esb, double* wb,
+                                              double& Sb, double& SMb, double& ptsb, double& eb, double& ptb) {
+  const double a = S - w[1], b = S - SM;
+  double ab = 0., bb = 0.;
+  // es = (a e - pt w1 + pts SM + w5 (vdotb(w) - vdotb(ws))) / b
+  const double Nb = esb / b;
+  bb -= esb * es / b;
+  ab += Nb * e;
+  eb += Nb * a;
+  ptb -= Nb * w[1];
+  wb[1] -= Nb * pt;
+  ptsb += Nb * SM;
+  SMb += Nb * pts;
+  wb[5] += Nb * (vdotb(w) - vdotb(ws));
+  const double vw = Nb * w[5], vs = -Nb * w[5];
+  wb[1] += vw * w[5];
+  wb[5] += vw * w[1];
+  wb[2] += vw * w[6];
+  wb[6] += vw * w[2];
+  wb[3] += vw * w[7];
+  wb[7] += vw * w[3];
+  wsb[1] += vs * ws[5];
+  wsb[5] += vs * ws[1];
+  wsb[2] += vs * ws[6];
+  wsb[6] += vs * ws[2];
+  wsb[3] += vs * ws[7];
+  wsb[7] += vs * ws[3];
+  const double den = w[0] * (S - w[1]) * (S - SM) - (w[5] * w[5]);
+  if (den != 0) {
+    const double sv = (SM - w[1]) / den;
+    const double sb = (w[0] * ((S - w[1]) * (S - w[1])) - (w[5] * w[5])) / den;
+    double denb = 0., svb = 0., sbb = 0.;
+    wb[7] += wsb[7] * sb;
+    sbb += wsb[7] * w[7];
+    wb[6] += wsb[6] * sb;
+    sbb += wsb[6] * w[6];
+    const double Nsb = sbb / den;
+    denb -= sbb * sb / den;
+    wb[0] += Nsb * (a * a);
+    ab += Nsb * w[0] * 2. * a;
+    wb[5] -= Nsb * 2. * w[5];
+    wb[3] += wsb[3];
+    wb[5] -= wsb[3] * w[7] * sv;
+    wb[7] -= wsb[3] * w[5] * sv;
+    svb -= wsb[3] * w[5] * w[7];
+    wb[2] += wsb[2];
+    wb[5] -= wsb[2] * w[6] * sv;
+    wb[6] -= wsb[2] * w[5] * sv;
+    svb -= wsb[2] * w[5] * w[6];
+    SMb += svb / den;
+    wb[1] -= svb / den;
+    denb -= svb * sv / den;
+    wb[0] += denb * a * b;
+    ab += denb * w[0] * b;
+    bb += denb * w[0] * a;
+    wb[5] -= denb * 2. * w[5];
+  } else {
+    wb[2] += wsb[2];
+    wb[3] += wsb[3];
+  }
+  wb[5] += wsb[5];
+  SMb += wsb[1];
+  // ws0 = w0 a / b
+  wb[0] += wsb[0] * a / b;
+  ab += wsb[0] * w[0] / b;
+  bb -= wsb[0] * ws[0] / b;
+  Sb += ab + bb;
+  wb[1] -= ab;
+  SMb -= bb;
+}
+
+// star_cons reverse: ub -> (wb, eb) of the star state
+__host__ __device__ inline void star_cons_bwd(const double* w, const double* ub, double* wb, double& eb) {
+  wb[0] += ub[0] + ub[1] * w[1] + ub[2] * w[2] + ub[3] * w[3];
+  wb[1] += ub[1] * w[0];
+  wb[2] += ub[2] * w[0];
+  wb[3] += ub[3] * w[0];
+  eb += ub[4];
+  wb[5] += ub[5];
+  wb[6] += ub[6];
+  wb[7] += ub[7];
+}
+
+// HLLD flux (flux_hlld, Miyoshi & Kusano 2005 with the reference's branch
+// conditions) reverse: fb -> wLb, wRb (accumulated).  The forward values are
+// recomputed with the forward's own functions; f receives the flux.
+__host__ __device__ inline void hlld_vjp(const double* wL, const double* wR, double g, const double* fb, double* wLb,
+                                         double* wRb, double* f = nullptr) {
+  // sides: fast speed, energy, enthalpy and total pressure as side_state
+  // computes them; q and F per component on demand (side_qF)
+  struct SideV {
+    double e, H, pt, cf;
+  } L, R;
+  FastT fL, fR;
+  side_fwd(wL, g, fL, L.e, L.H);
+  side_fwd(wR, g, fR, R.e, R.H);
+  L.cf = fL.cf;
+  R.cf = fR.cf;
+  L.pt = wL[4] + 0.5 * (wL[5] * wL[5] + wL[6] * wL[6] + wL[7] * wL[7]);
+  R.pt = wR[4] + 0.5 * (wR[5] * wR[5] + wR[6] * wR[6] + wR[7] * wR[7]);
+  const double vL = wL[1], vR = wR[1];
+  const double cmax = std_max(L.cf, R.cf);
+  const double SL = std_min(vL, vR) - cmax;
+  const double SR = std_max(vL, vR) + cmax;
+  const double den = (SR - vR) * wR[0] - (SL - vL) * wL[0];
+  const double SM = ((SR - vR) * wR[0] * vR - (SL - vL) * wL[0] * vL - R.pt + L.pt) / den;
+  const double pts = ((SR - vR) * wR[0] * L.pt - (SL - vL) * wL[0] * R.pt +
+                      wL[0] * wR[0] * (SR - vR) * (SL - vL) * (vR - vL)) / den;
+  double wLs[8], wRs[8], eLs, eRs;
+  hlld_star(wL, SL, SM, pts, L.e, L.pt, wLs, eLs);
+  hlld_star(wR, SR, SM, pts, R.e, R.pt, wRs, eRs);
+  const double rl = sqrt(wLs[0]), rr = sqrt(wRs[0]), rlr = sqrt(wLs[0] * wRs[0]);
+  const double sg = sign0(wL[5]), sgR = sign0(wR[5]);
+  double wss[8];
+  wss[0] = wLs[0];
+  wss[1] = SM;
+  wss[2] = (rl * wLs[2] + rr * wRs[2] + (wRs[6] - wLs[6]) * sg) / (rl + rr);
+  wss[3] = (rl * wLs[3] + rr * wRs[3] + (wRs[7] - wLs[7]) * sg) / (rl + rr);
+  wss[4] = 0.;
+  wss[5] = wLs[5];
+  wss[6] = (rl * wRs[6] + rr * wLs[6] + rlr * (wRs[2] - wLs[2]) * sg) / (rl + rr);
+  wss[7] = (rl * wRs[7] + rr * wLs[7] + rlr * (wRs[3] - wLs[3]) * sg) / (rl + rr);
+  const double dss = vdotb(wss);
+  const double eLss = eLs - rl * (vdotb(wLs) - dss) * sg;
+  const double eRss = eRs + rr * (vdotb(wRs) - dss) * sgR;
+  const double SLs = SM - fabs(wL[5]) / rl;
+  const double SRs = SM + fabs(wR[5]) / rr;
+  double qLs[8], qRs[8], qLss[8], qRss[8];
+  star_cons(wLs, eLs, qLs);
+  star_cons(wRs, eRs, qRs);
+  star_cons(wss, eLss, qLss);
+  star_cons(wss, eRss, qRss);
+  int region = 0;
+  if (SL > 0) region = 1;
+  else if (SL <= 0. && SLs >= 0.) region = 2;
+  else if (SLs <= 0. && SM >= 0.) region = 3;
+  else if (SM <= 0. && SRs >= 0.) region = 4;
+  else if (SRs <= 0. && SR >= 0.) region = 5;
+  else if (SR < 0.) region = 6;
+  if (f) {
+    for (int k = 0; k < 8; k++) {
+      double Lq, LF, Rq, RF;
+      side_qF(wL, L.e, L.H, k, Lq, LF);
+      side_qF(wR, R.e, R.H, k, Rq, RF);
+      switch (region) {
+        case 1: f[k] = LF; break;
+        case 2: f[k] = LF + SL * qLs[k] - SL * Lq; break;
+        case 3: f[k] = LF + SLs * qLss[k] - (SLs - SL) * qLs[k] - SL * Lq; break;
+        case 4: f[k] = RF + SRs * qRss[k] - (SRs - SR) * qRs[k] - SR * Rq; break;
+        case 5: f[k] = RF + SR * qRs[k] - SR * Rq; break;
+        case 6: f[k] = RF; break;
+        default: f[k] = 0.; break;
+      }
+    }
+  }
+  // ---- reverse
+  double LFb[8] = {0}, RFb[8] = {0}, Lqb[8] = {0}, Rqb[8] = {0};
+  double qLsb[8] = {0}, qRsb[8] = {0}, qssb[8] = {0};
+  double SLb = 0., SRb = 0., SMb = 0., SLsb = 0., SRsb = 0., eLssb = 0., eRssb = 0.;
+  for (int k = 0; k < 8; k++) {
+    const double b = fb[k];
+    double Lqk, LFk, Rqk, RFk;
+    side_qF(wL, L.e, L.H, k, Lqk, LFk);
+    side_qF(wR, R.e, R.H, k, Rqk, RFk);
+    (void)LFk;
+    (void)RFk;
+    switch (region) {
+      case 1: LFb[k] += b; break;
+      case 2:
+        LFb[k] += b;
+        qLsb[k] += SL * b;
+        Lqb[k] -= SL * b;
+        SLb += b * (qLs[k] - Lqk);
+        break;
+      case 3:
+        LFb[k] += b;
+        qssb[k] += SLs * b;
+        qLsb[k] -= (SLs - SL) * b;
+        Lqb[k] -= SL * b;
+        SLsb += b * (qLss[k] - qLs[k]);
+        SLb += b * (qLs[k] - Lqk);
+        break;
+      case 4:
+        RFb[k] += b;
+        qssb[k] += SRs * b;
+        qRsb[k] -= (SRs - SR) * b;
+        Rqb[k] -= SR * b;
+        SRsb += b * (qRss[k] - qRs[k]);
+        SRb += b * (qRs[k] - Rqk);
+        break;
+      case 5:
+        RFb[k] += b;
+        qRsb[k] += SR * b;
+        Rqb[k] -= SR * b;
+        SRb += b * (qRs[k] - Rqk);
+        break;
+      case 6: RFb[k] += b; break;
+      default: break;
+    }
+  }
+  double wLsb[8] = {0}, wRsb[8] = {0}, wssb[8] = {0};
+  double eLsb = 0., eRsb = 0., rlb = 0., rrb = 0., rlrb = 0.;
+  // q** = star_cons(w**, e_L** or e_R**)
+  if (region == 3) star_cons_bwd(wss, qssb, wssb, eLssb);
+  if (region == 4) star_cons_bwd(wss, qssb, wssb, eRssb);
+  star_cons_bwd(wLs, qLsb, wLsb, eLsb);
+  star_cons_bwd(wRs, qRsb, wRsb, eRsb);
+  // S_L* = SM - |BxL| / rl, S_R* = SM + |BxR| / rr
+  SMb += SLsb + SRsb;
+  wLb[5] -= SLsb * sign0(wL[5]) / rl;
+  rlb += SLsb * fabs(wL[5]) / (rl * rl);
+  wRb[5] += SRsb * sign0(wR[5]) / rr;
+  rrb -= SRsb * fabs(wR[5]) / (rr * rr);
+  // e_L** = e_L* - rl (vdotb(wLs) - dss) sg; e_R** = e_R* + rr (vdotb(wRs) - dss) sgR
+  double dssb = 0., vLsb = 0., vRsb = 0.;
+  eLsb += eLssb;
+  rlb -= eLssb * (vdotb(wLs) - dss) * sg;
+  vLsb -= eLssb * rl * sg;
+  dssb += eLssb * rl * sg;
+  eRsb += eRssb;
+  rrb += eRssb * (vdotb(wRs) - dss) * sgR;
+  vRsb += eRssb * rr * sgR;
+  dssb -= eRssb * rr * sgR;
+  wLsb[1] += vLsb * wLs[5];
+  wLsb[5] += vLsb * wLs[1];
+  wLsb[2] += vLsb * wLs[6];
+  wLsb[6] += vLsb * wLs[2];
+  wLsb[3] += vLsb * wLs[7];
+  wLsb[7] += vLsb * wLs[3];
+  wRsb[1] += vRsb * wRs[5];
+  wRsb[5] += vRsb * wRs[1];
+  wRsb[2] += vRsb * wRs[6];
+  wRsb[6] += vRsb * wRs[2];
+  wRsb[3] += vRsb * wRs[7];
+  wRsb[7] += vRsb * wRs[3];
+  // dss = vdotb(w**)
+  wssb[1] += dssb * wss[5];
+  wssb[5] += dssb * wss[1];
+  wssb[2] += dssb * wss[6];
+  wssb[6] += dssb * wss[2];
+  wssb[3] += dssb * wss[7];
+  wssb[7] += dssb * wss[3];
+  // w**
+  const double D = rl + rr;
+  double Db = 0.;
+  wLsb[0] += wssb[0];
+  SMb += wssb[1];
+  wLsb[5] += wssb[5];
+  {
+    const double N = wssb[2] / D;
+    Db -= wssb[2] * wss[2] / D;
+    rlb += N * wLs[2];
+    wLsb[2] += N * rl;
+    rrb += N * wRs[2];
+    wRsb[2] += N * rr;
+    wRsb[6] += N * sg;
+    wLsb[6] -= N * sg;
+  }
+  {
+    const double N = wssb[3] / D;
+    Db -= wssb[3] * wss[3] / D;
+    rlb += N * wLs[3];
+    wLsb[3] += N * rl;
+    rrb += N * wRs[3];
+    wRsb[3] += N * rr;
+    wRsb[7] += N * sg;
+    wLsb[7] -= N * sg;
+  }
+  {
+    const double N = wssb[6] / D;
+    Db -= wssb[6] * wss[6] / D;
+    rlb += N * wRs[6];
+    wRsb[6] += N * rl;
+    rrb += N * wLs[6];
+    wLsb[6] += N * rr;
+    rlrb += N * (wRs[2] - wLs[2]) * sg;
+    wRsb[2] += N * rlr * sg;
+    wLsb[2] -= N * rlr * sg;
+  }
+  {
+    const double N = wssb[7] / D;
+    Db -= wssb[7] * wss[7] / D;
+    rlb += N * wRs[7];
+    wRsb[7] += N * rl;
+    rrb += N * wLs[7];
+    wLsb[7] += N * rr;
+    rlrb += N * (wRs[3] - wLs[3]) * sg;
+    wRsb[3] += N * rlr * sg;
+    wLsb[3] -= N * rlr * sg;
+  }
+  rlb += Db;
+  rrb += Db;
+  // rl = sqrt(rhoL*), rr = sqrt(rhoR*), rlr = sqrt(rhoL* rhoR*)
+  if (rl > 0.) wLsb[0] += rlb * 0.5 / rl;
+  if (rr > 0.) wRsb[0] += rrb * 0.5 / rr;
+  if (rlr > 0.) {
+    const double pb = rlrb * 0.5 / rlr;
+    wLsb[0] += pb * wRs[0];
+    wRsb[0] += pb * wLs[0];
+  }
+  // star states
+  double ptsb = 0., eLb = 0., eRb = 0., ptLb = 0., ptRb = 0.;
+  hlld_star_bwd(wL, SL, SM, pts, L.e, L.pt, wLs, eLs, wLsb, eLsb, wLb, SLb, SMb, ptsb, eLb, ptLb);
+  hlld_star_bwd(wR, SR, SM, pts, R.e, R.pt, wRs, eRs, wRsb, eRsb, wRb, SRb, SMb, ptsb, eRb, ptRb);
+  // SM, pts over den
+  const double dR = SR - vR, dL = SL - vL;
+  double dRb = 0., dLb = 0., denb = 0., vLb = 0., vRb = 0., wL0b = 0., wR0b = 0.;
+  {
+    const double Ns = SMb / den, Np = ptsb / den;
+    denb -= SMb * SM / den + ptsb * pts / den;
+    dRb += Ns * wR[0] * vR;
+    wR0b += Ns * dR * vR;
+    vRb += Ns * dR * wR[0];
+    dLb -= Ns * wL[0] * vL;
+    wL0b -= Ns * dL * vL;
+    vLb -= Ns * dL * wL[0];
+    ptRb -= Ns;
+    ptLb += Ns;
+    dRb += Np * wR[0] * L.pt;
+    wR0b += Np * dR * L.pt;
+    ptLb += Np * dR * wR[0];
+    dLb -= Np * wL[0] * R.pt;
+    wL0b -= Np * dL * R.pt;
+    ptRb -= Np * dL * wL[0];
+    const double dv = vR - vL, t = wL[0] * wR[0] * dR * dL;
+    wL0b += Np * wR[0] * dR * dL * dv;
+    wR0b += Np * wL[0] * dR * dL * dv;
+    dRb += Np * wL[0] * wR[0] * dL * dv;
+    dLb += Np * wL[0] * wR[0] * dR * dv;
+    vRb += Np * t;
+    vLb -= Np * t;
+  }
+  dRb += denb * wR[0];
+  wR0b += denb * dR;
+  dLb -= denb * wL[0];
+  wL0b -= denb * dL;
+  SRb += dRb;
+  vRb -= dRb;
+  SLb += dLb;
+  vLb -= dLb;
+  // SL = min(vL, vR) - cmax, SR = max(vL, vR) + cmax, cmax = max(cfL, cfR)
+  double cmaxb = -SLb + SRb;
+  if (vR < vL) vRb += SLb;
+  else vLb += SLb;
+  if (vL < vR) vRb += SRb;
+  else vLb += SRb;
+  double cfLb = 0., cfRb = 0.;
+  if (L.cf < R.cf) cfRb += cmaxb;
+  else cfLb += cmaxb;
+  wLb[0] += wL0b;
+  wRb[0] += wR0b;
+  wLb[1] += vLb;
+  wRb[1] += vRb;
+  side_bwd_vec(wL, g, L.e, L.H, fL, Lqb, LFb, 0., eLb, ptLb, cfLb, wLb);
+  side_bwd_vec(wR, g, R.e, R.H, fR, Rqb, RFb, 0., eRb, ptRb, cfRb, wRb);
 }
 
 }  // namespace amr

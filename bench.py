@@ -196,7 +196,7 @@ def ot_bench(steps, warmup):
             "unit": "zone-updates/s", "t_reached": info["t"], "leaves_at_end": info["nleaf"]}
 
 
-def adjoint_bench(n, steps, every, fd_dirs):
+def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):
     """Frozen-mesh adjoint (build-defined workload): Orszag-Tang controls on a
     uniform periodic n x n grid (smoothly modulated and seeded-noise density,
     pressure and velocity, so no limiter evaluation starts on a kink),
@@ -214,7 +214,7 @@ def adjoint_bench(n, steps, every, fd_dirs):
     from amr2d_mhd_amd import adjoint as AD
 
     g = 5.0 / 3.0
-    mesh = AD.FrozenMesh(n, 1.0 / n, g)
+    mesh = AD.FrozenMesh(n, 1.0 / n, g, flux)
     ctrl = AD.orszag_tang_controls(n, modulation=0.05, noise=1e-3)
     U = AD.cons_from_controls(*ctrl, g).contiguous()
     dts, Uc, Bxc, Byc = [], U, ctrl[3], ctrl[4]
@@ -265,7 +265,7 @@ def adjoint_bench(n, steps, every, fd_dirs):
     step_tape_ms = per_call(lambda: mesh.step(U, ctrl[3], ctrl[4], dts[0], tape))
     vjp_ms = per_call(lambda: mesh.vjp(tape, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy))
     step_vjp_ms = per_call(lambda: mesh.step_vjp(U, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy))
-    out = {"workload": f"frozen uniform periodic {n}x{n}, Orszag-Tang controls, minmod-MUSCL + HLLE, CT, RK2, "
+    out = {"workload": f"frozen uniform periodic {n}x{n}, Orszag-Tang controls, minmod-MUSCL + {flux}, CT, RK2, "
                        f"{steps} steps, checkpoints every {every}",
            "J": float(J), "forward_ms_per_step": tf / steps * 1e3,
            "backward_ms_per_step": tb / steps * 1e3,
@@ -323,6 +323,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-fd", type=int, default=0, help="finite-difference check directions")
+    ap.add_argument("--adj-flux", default="HLLE", choices=["HLLE", "HLLD"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -402,7 +403,7 @@ def main():
     if args.ot_steps:
         out["orszag_tang"] = ot_bench(args.ot_steps, args.ot_warmup)
     if args.adjoint:
-        out["adjoint"] = adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd)
+        out["adjoint"] = adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd, args.adj_flux)
     if args.ensemble:
         out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup,
                                          groups=args.ens_groups)

@@ -193,8 +193,9 @@ int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
  * size dx, DEVICE pointers: U[8][N][N] (row-major, i fastest), Bx[N][N] (x-face
  * between cells i-1 and i), By[N][N] (y-face between rows j-1 and j).
  * amr_adj_step is one RK2 step of the reference (main.cpp:128-247) on that
- * grid with the minmod-MUSCL + HLLE configuration and CT, at the given frozen
- * dt; on a single-level mesh it equals the engine bit for bit.
+ * grid with minmod MUSCL, the `flux` solver ("HLLE" or "HLLD"; NULL = HLLE)
+ * and CT, at the given frozen dt; on a single-level mesh it equals the engine
+ * bit for bit.
  * amr_adj_step_tape is the same step that also writes the step's tape (the
  * intermediates its reverse needs, amr_adj_tape_doubles(adj) doubles) to
  * `tape` (a DEVICE buffer; NULL: the context's own one).  amr_adj_vjp_tape
@@ -205,7 +206,7 @@ int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
  * tape, then runs the reverse sweep.  Outputs must not alias inputs; calls on
  * one context are stream-ordered and share its work buffers. */
 typedef struct amr_adj amr_adj;
-int amr_adj_create(int n, double dx, double gamma, amr_adj **out);
+int amr_adj_create(int n, double dx, double gamma, const char *flux, amr_adj **out);
 int amr_adj_destroy(amr_adj *adj);
 int64_t amr_adj_tape_doubles(const amr_adj *adj);
 int amr_adj_step(amr_adj *adj, const double *U, const double *Bx, const double *By, double dt, double *Uo,

@@ -5,7 +5,8 @@ the reference's step (main.cpp:128-247) for the minmod-MUSCL + HLLE
 configuration: cons->prim (meshblock.cpp:119-179), minmod edge states
 (celledges.cpp:9-64, slopelimiter.cpp:6-54) with the normal B taken from the
 faces (MUSCL2D.cpp:44-53), HLLE (riemannS.cpp:118-189 with misc.cpp:71-140,
-including the RoeAvg normaliser sqrt(rhoL + sqrt(rhoR))), the RK2 stage
+including the RoeAvg normaliser sqrt(rhoL + sqrt(rhoR))) or HLLD
+(riemannS.cpp:397-549, misc.cpp:64-68 sign with sign(0) = 0), the RK2 stage
 update (main.cpp:147-157, 179-189), and CT (CT2D.cpp:15-123: corner EMF, face
 update with the reciprocal, cell sync with the energy fix).  Every elementwise
 expression keeps the reference's operand order, so results are bitwise those
@@ -77,7 +78,7 @@ def side(w, g):
     F[5] = vn * w[5] - Bn * w[1]
     F[6] = vn * w[6] - Bn * w[2]
     F[7] = vn * w[7] - Bn * w[3]
-    return q, F, H, cf
+    return q, F, H, cf, e, pt
 
 
 def roe_bounds(wL, wR, g, HL, HR):
@@ -106,8 +107,8 @@ def roe_bounds(wL, wR, g, HL, HR):
 
 
 def hlle(wL, wR, g):
-    qL, FL, HL, cfL = side(wL, g)
-    qR, FR, HR, cfR = side(wR, g)
+    qL, FL, HL, cfL = side(wL, g)[:4]
+    qR, FR, HR, cfR = side(wR, g)[:4]
     lmin, lmax = roe_bounds(wL, wR, g, HL, HR)
     xl = wL[1] - cfL
     il = np.where(lmin < xl, lmin, xl)       # std::min
@@ -118,20 +119,106 @@ def hlle(wL, wR, g):
     return (SR * FL - SL * FR + SL * SR * (qR - qL)) / (SR - SL)
 
 
-def fluxes(W, Bx, By, g):
+def sign0(x):
+    return np.where(x > 0, 1.0, np.where(x < 0, -1.0, 0.0))
+
+
+def vdotb(w):
+    s = w[1] * w[5]
+    s = s + w[2] * w[6]
+    return s + w[3] * w[7]
+
+
+def hlld_star(w, S, SM, pts, e, pt):
+    """Star state of one side (riemannS.cpp:467-497); ws[4] stays 0."""
+    ws = np.empty_like(w)
+    ws[0] = w[0] * (S - w[1]) / (S - SM)
+    ws[1] = SM
+    ws[4] = 0.
+    ws[5] = w[5]
+    den = w[0] * (S - w[1]) * (S - SM) - (w[5] * w[5])
+    nz = den != 0
+    dd = np.where(nz, den, 1.0)
+    sv = (SM - w[1]) / dd
+    ws[2] = np.where(nz, w[2] - w[5] * w[6] * sv, w[2])
+    ws[3] = np.where(nz, w[3] - w[5] * w[7] * sv, w[3])
+    sb = (w[0] * ((S - w[1]) * (S - w[1])) - (w[5] * w[5])) / dd
+    ws[6] = np.where(nz, w[6] * sb, 0.)
+    ws[7] = np.where(nz, w[7] * sb, 0.)
+    es = (((S - w[1]) * e - pt * w[1] + pts * SM + w[5] * (vdotb(w) - vdotb(ws))) / (S - SM))
+    return ws, es
+
+
+def star_cons(w, e):
+    u = w.copy()
+    u[1] = w[0] * w[1]
+    u[2] = w[0] * w[2]
+    u[3] = w[0] * w[3]
+    u[4] = e
+    return u
+
+
+def hlld(wL, wR, g):
+    """HLLD (riemannS.cpp:397-549, Miyoshi & Kusano 2005) with the
+    reference's wave-speed estimate and region conditions."""
+    qL, FL, HL, cfL, eL, ptL = side(wL, g)
+    qR, FR, HR, cfR, eR, ptR = side(wR, g)
+    vL, vR = wL[1], wR[1]
+    cmax = np.where(cfL < cfR, cfR, cfL)            # std::max
+    SL = np.where(vR < vL, vR, vL) - cmax           # std::min
+    SR = np.where(vL < vR, vR, vL) + cmax
+    den = (SR - vR) * wR[0] - (SL - vL) * wL[0]
+    SM = ((SR - vR) * wR[0] * vR - (SL - vL) * wL[0] * vL - ptR + ptL) / den
+    pts = ((SR - vR) * wR[0] * ptL - (SL - vL) * wL[0] * ptR +
+           wL[0] * wR[0] * (SR - vR) * (SL - vL) * (vR - vL)) / den
+    wLs, eLs = hlld_star(wL, SL, SM, pts, eL, ptL)
+    wRs, eRs = hlld_star(wR, SR, SM, pts, eR, ptR)
+    rl, rr, rlr = np.sqrt(wLs[0]), np.sqrt(wRs[0]), np.sqrt(wLs[0] * wRs[0])
+    sg = sign0(wL[5])
+    wss = np.empty_like(wL)
+    wss[0] = wLs[0]
+    wss[1] = SM
+    wss[2] = (rl * wLs[2] + rr * wRs[2] + (wRs[6] - wLs[6]) * sg) / (rl + rr)
+    wss[3] = (rl * wLs[3] + rr * wRs[3] + (wRs[7] - wLs[7]) * sg) / (rl + rr)
+    wss[4] = 0.
+    wss[5] = wLs[5]
+    wss[6] = (rl * wRs[6] + rr * wLs[6] + rlr * (wRs[2] - wLs[2]) * sg) / (rl + rr)
+    wss[7] = (rl * wRs[7] + rr * wLs[7] + rlr * (wRs[3] - wLs[3]) * sg) / (rl + rr)
+    dss = vdotb(wss)
+    eLss = eLs - rl * (vdotb(wLs) - dss) * sg
+    eRss = eRs + rr * (vdotb(wRs) - dss) * sign0(wR[5])
+    SLs = SM - np.abs(wL[5]) / rl
+    SRs = SM + np.abs(wR[5]) / rr
+    qLs, qRs = star_cons(wLs, eLs), star_cons(wRs, eRs)
+    qLss, qRss = star_cons(wss, eLss), star_cons(wss, eRss)
+    c1 = SL > 0
+    c2 = ~c1 & (SL <= 0.) & (SLs >= 0.)
+    c3 = ~c1 & ~c2 & (SLs <= 0.) & (SM >= 0.)
+    c4 = ~c1 & ~c2 & ~c3 & (SM <= 0.) & (SRs >= 0.)
+    c5 = ~c1 & ~c2 & ~c3 & ~c4 & (SRs <= 0.) & (SR >= 0.)
+    c6 = ~c1 & ~c2 & ~c3 & ~c4 & ~c5 & (SR < 0.)
+    return np.select([c1, c2, c3, c4, c5, c6],
+                     [FL, FL + SL * qLs - SL * qL, FL + SLs * qLss - (SLs - SL) * qLs - SL * qL,
+                      FR + SRs * qRss - (SRs - SR) * qRs - SR * qR, FR + SR * qRs - SR * qR, FR], 0.)
+
+
+def fluxes(W, Bx, By, g, flux="HLLE"):
     """x-face fluxes Fx[8][j][i] (face between i-1 and i) and y-face Fy."""
+    riemann = hlld if flux == "HLLD" else hlle
     sh = lambda A, o, ax: np.roll(A, -o, axis=ax)  # A[.. idx + o ..]
     # x faces: left cell i-1, right cell i; stencil i-2 .. i+1 (axis 2)
     wl, wr = recon(sh(W, -2, 2), sh(W, -1, 2), W, sh(W, 1, 2))
     wl[5] = Bx
     wr[5] = Bx
-    Fx = hlle(wl, wr, g)
+    with np.errstate(all="ignore"):  # unused branches of the region selection
+        Fx = riemann(wl, wr, g)
     # y faces: swap x/y components, then swap the flux back
     wl, wr = recon(sh(W, -2, 1), sh(W, -1, 1), W, sh(W, 1, 1))
     wl[6] = By
     wr[6] = By
     sw = [0, 2, 1, 3, 4, 6, 5, 7]
-    Fy = hlle(wl[sw], wr[sw], g)[sw]
+    with np.errstate(all="ignore"):
+        Fy = riemann(wl[sw], wr[sw], g)[sw]
     return Fx, Fy
 
 
@@ -153,10 +240,10 @@ def bsync(U, Bx, By):
     U[6] = by
 
 
-def step(U, Bx, By, dt, dx, g):
+def step(U, Bx, By, dt, dx, g, flux="HLLE"):
     """One RK2 step with CT; returns new (U, Bx, By)."""
     # stage 1
-    Fx, Fy = fluxes(prim(U, g), Bx, By, g)
+    Fx, Fy = fluxes(prim(U, g), Bx, By, g, flux)
     Us = U - dt * residual(Fx, Fy, dx)
     E = emf(Fx, Fy)
     rx = (1. / dx) * (np.roll(E, -1, axis=0) - E)
@@ -165,7 +252,7 @@ def step(U, Bx, By, dt, dx, g):
     Bys = By - dt * ry
     bsync(Us, Bxs, Bys)
     # stage 2
-    Fx, Fy = fluxes(prim(Us, g), Bxs, Bys, g)
+    Fx, Fy = fluxes(prim(Us, g), Bxs, Bys, g, flux)
     Un = 0.5 * (U + Us - dt * residual(Fx, Fy, dx))
     E = emf(Fx, Fy)
     rx = (1. / dx) * (np.roll(E, -1, axis=0) - E)

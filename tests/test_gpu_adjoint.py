@@ -15,13 +15,13 @@ pytestmark = pytest.mark.gpu
 GAMMA = 5.0 / 3.0
 
 
-def _setup(n, steps, vscale=1.0):
+def _setup(n, steps, vscale=1.0, flux="HLLE"):
     import torch
 
     from amr2d_mhd_amd import adjoint as AD
 
     rho, v, p, Bx, By = AD.orszag_tang_controls(n, vscale=vscale, modulation=0.05, noise=1e-3)
-    mesh = AD.FrozenMesh(n, 1.0 / n, GAMMA)
+    mesh = AD.FrozenMesh(n, 1.0 / n, GAMMA, flux)
     U = AD.cons_from_controls(rho, v, p, Bx, By, GAMMA).contiguous()
     dts, Uc, Bxc, Byc = [], U, Bx, By
     for _ in range(steps):
@@ -40,10 +40,11 @@ def _objective(ctrl, mesh, dts, target, every):
     return ((UT[0] - target) ** 2).sum()
 
 
-def test_gpu_step_equals_numpy_bitwise():
+@pytest.mark.parametrize("flux", ["HLLE", "HLLD"])
+def test_gpu_step_equals_numpy_bitwise(flux):
     import torch
 
-    mesh, (rho, v, p, Bx, By), dts = _setup(32, 4)
+    mesh, (rho, v, p, Bx, By), dts = _setup(32, 4, flux=flux)
     from amr2d_mhd_amd import adjoint as AD
 
     U = AD.cons_from_controls(rho, v, p, Bx, By, GAMMA).contiguous()
@@ -51,18 +52,18 @@ def test_gpu_step_equals_numpy_bitwise():
     Ug, Bxg, Byg = U, Bx, By
     for dt in dts:
         Ug, Bxg, Byg = mesh.step(Ug, Bxg, Byg, dt)
-        Un, Bxn, Byn = US.step(Un, Bxn, Byn, dt, 1.0 / 32, GAMMA)
+        Un, Bxn, Byn = US.step(Un, Bxn, Byn, dt, 1.0 / 32, GAMMA, flux)
         torch.cuda.synchronize()
         assert np.array_equal(Ug.cpu().numpy(), Un)
         assert np.array_equal(Bxg.cpu().numpy(), Bxn) and np.array_equal(Byg.cpu().numpy(), Byn)
 
 
-def _grad_and_target(n, steps, every):
+def _grad_and_target(n, steps, every, flux="HLLE"):
     import torch
 
     from amr2d_mhd_amd import adjoint as AD
 
-    mesh, ctrl, dts = _setup(n, steps)
+    mesh, ctrl, dts = _setup(n, steps, flux=flux)
     # target: the same set-up with the initial velocity scaled by 1.05
     rt, vt, pt, Bxt, Byt = AD.orszag_tang_controls(n, vscale=1.05, modulation=0.05, noise=1e-3)
     with torch.no_grad():
@@ -75,7 +76,8 @@ def _grad_and_target(n, steps, every):
     return mesh, ctrl, dts, target, [t.grad.clone() for t in leaves], float(J)
 
 
-def test_adjoint_matches_central_fd():
+@pytest.mark.parametrize("flux", ["HLLE", "HLLD"])
+def test_adjoint_matches_central_fd(flux):
     """Away from limiter kinks: generic (seeded-noise) initial data and a
     step small enough that no limiter ratio crosses a switch point (at 1e-6
     a few of the ~1e5 minmod ratios per stage sit close enough to r = 1 to
@@ -83,7 +85,7 @@ def test_adjoint_matches_central_fd():
     import torch
 
     n, steps, every = 64, 30, 10
-    mesh, ctrl, dts, target, grads, J0 = _grad_and_target(n, steps, every)
+    mesh, ctrl, dts, target, grads, J0 = _grad_and_target(n, steps, every, flux)
     assert J0 > 0
     gen = torch.Generator(device="cpu").manual_seed(20261202)
     eps = 1e-8
@@ -138,8 +140,8 @@ def test_tape_reverse_equals_recomputing_reverse():
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("n", [16, 45, 64])
-def test_tiled_reverse_equals_per_face_reverse(n):
+@pytest.mark.parametrize("n,flux", [(16, "HLLE"), (45, "HLLE"), (64, "HLLE"), (45, "HLLD")])
+def test_tiled_reverse_equals_per_face_reverse(n, flux):
     """The tiled face reverse (stencil sums in LDS, faces at tile edges
     reversed twice, grids narrower than a tile and not a multiple of it)
     equals the per-face kernels + gather up to the summation order."""
@@ -148,7 +150,7 @@ def test_tiled_reverse_equals_per_face_reverse(n):
     from amr2d_mhd_amd import adjoint as AD
     from amr2d_mhd_amd._lib import lib
 
-    mesh, (rho, v, p, Bx, By), dts = _setup(n, 2)
+    mesh, (rho, v, p, Bx, By), dts = _setup(n, 2, flux=flux)
     U = AD.cons_from_controls(rho, v, p, Bx, By, GAMMA).contiguous()
     outs = mesh.step(U, Bx, By, dts[0])
     gen = torch.Generator().manual_seed(11)
@@ -195,7 +197,8 @@ def test_adjoint_matches_fd_of_cpu_forward():
         assert abs(fd - ad) < 1e-5 * abs(fd), (fd, ad)
 
 
-def test_adjoint_equals_autograd_of_restatement():
+@pytest.mark.parametrize("flux", ["HLLE", "HLLD"])
+def test_adjoint_equals_autograd_of_restatement(flux):
     """The hand-written adjoint of one step equals torch autograd through the
     torch restatement of the step, component by component."""
     import torch
@@ -205,7 +208,7 @@ def test_adjoint_equals_autograd_of_restatement():
 
     n = 16
     rho, v, p, Bx, By = AD.orszag_tang_controls(n, modulation=0.05, noise=1e-3)
-    mesh = AD.FrozenMesh(n, 1.0 / n, GAMMA)
+    mesh = AD.FrozenMesh(n, 1.0 / n, GAMMA, flux)
     U = AD.cons_from_controls(rho, v, p, Bx, By, GAMMA).contiguous()
     dt = AD.cfl_dt(U, GAMMA, 1.0 / n, 0.4)
     for _ in range(2):
@@ -214,7 +217,7 @@ def test_adjoint_equals_autograd_of_restatement():
     seeds = [torch.randn(t.shape, generator=gen, dtype=torch.float64) for t in (U, Bx, By)]
     Ub, Bxb, Byb = mesh.step_vjp(U, Bx, By, dt, *[s.cuda() for s in seeds])
     Ut, Bxt, Byt = [t.detach().cpu().clone().requires_grad_(True) for t in (U, Bx, By)]
-    outs = UA.step(Ut, Bxt, Byt, dt, 1.0 / n, GAMMA)
+    outs = UA.step(Ut, Bxt, Byt, dt, 1.0 / n, GAMMA, flux)
     sum((o * s).sum() for o, s in zip(outs, seeds)).backward()
     for got, ref in ((Ub, Ut.grad), (Bxb, Bxt.grad), (Byb, Byt.grad)):
         assert torch.allclose(got.cpu(), ref, rtol=1e-12, atol=1e-13 * float(ref.abs().max()))
