@@ -197,7 +197,7 @@ struct Profile {
   int64_t launches = 0;
 };
 enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_HOST_COMPILE,
-       PF_COUNT };
+       PF_GRAPH, PF_COUNT };
 
 
 }  // namespace
@@ -230,7 +230,11 @@ struct amr_ctx {
   PinnedVec<InstParams> h_ip;  // pinned: the step graph copies from it
   // the device work of one step replayed as a HIP graph; re-captured when
   // the topology (work lists, program tables, buffer addresses) changes
-  bool graph_on = true, graph_dirty = true, regrid_launched = false;
+  bool graph_on = false, graph_dirty = true, regrid_launched = false;
+  // steps launched since the last topology rebuild: a graph is captured only
+  // for a topology that has already lasted a step (most trees of a regridding
+  // run last one or two steps, and a capture used once costs more than it saves)
+  int steps_same_topo = 0;
   // compiled fill programs kept per instance besides the current one (LRU)
   int prog_cache_max = 4;
   // a step launched by step_begin and not yet completed by step_end
@@ -252,6 +256,10 @@ struct amr_ctx {
   int span_variant[2] = {-1, -1};
   double wall_khz = 0;
   hipGraphExec_t gexec = nullptr;
+  // per-leaf CFL bounds and refinement indicators, [lam(nleaf) | grad(nleaf)]
+  // in one device and one pinned host buffer (one download per step); the
+  // four pointers are views set at each topology rebuild
+  double *d_red = nullptr, *h_red = nullptr;
   double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
   int cap_red = 0;
   DevError* d_err = nullptr;  // one record per instance
@@ -1441,18 +1449,19 @@ int rebuild_topology(amr_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_inst_gen, c->h_inst_gen.data(), sizeof(FillGen*) * ni, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_prefix, c->h_prefix.data(), sizeof(int32_t) * (ni + 1), hipMemcpyHostToDevice, c->stream));
   const int old_cap = c->cap_red;
-  if (int e = ensure(c->d_lam, c->cap_red, std::max(1, c->nleaf))) return e;
-  if (c->cap_red != old_cap || !c->d_grad) {
-    // reduction outputs (device + pinned host) follow the leaf capacity
-    if (c->d_grad) hipFree(c->d_grad);
-    if (c->h_lam) hipHostFree(c->h_lam);
-    if (c->h_grad) hipHostFree(c->h_grad);
-    HIPCHK(hipMalloc((void**)&c->d_grad, sizeof(double) * c->cap_red));
-    HIPCHK(hipHostMalloc((void**)&c->h_lam, sizeof(double) * c->cap_red));
-    HIPCHK(hipHostMalloc((void**)&c->h_grad, sizeof(double) * c->cap_red));
+  if (int e = ensure(c->d_red, c->cap_red, 2 * std::max(1, c->nleaf))) return e;
+  if (c->cap_red != old_cap || !c->h_red) {
+    // the pinned host copy follows the device capacity
+    if (c->h_red) hipHostFree(c->h_red);
+    HIPCHK(hipHostMalloc((void**)&c->h_red, sizeof(double) * c->cap_red));
   }
+  c->d_lam = c->d_red;
+  c->d_grad = c->d_red + c->nleaf;
+  c->h_lam = c->h_red;
+  c->h_grad = c->h_red + c->nleaf;
   c->topo_dirty = false;
   c->graph_dirty = true;
+  c->steps_same_topo = 0;
   return 0;
 }
 
@@ -1620,10 +1629,8 @@ int cfl_and_indicators(amr_ctx* c, const double* dW) {
     launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
                         c->stream);
   }
-  if (c->nleaf) {
-    HIPCHK(hipMemcpyAsync(c->h_lam, c->d_lam, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-  }
+  if (c->nleaf)
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * 2 * c->nleaf, hipMemcpyDeviceToHost, c->stream));
   return sync_errors(c);
 }
 
@@ -1687,10 +1694,8 @@ int enqueue_step(amr_ctx* c) {
     launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
                         c->stream);
   }
-  if (c->nleaf) {
-    HIPCHK(hipMemcpyAsync(c->h_lam, c->d_lam, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-  }
+  if (c->nleaf)
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * 2 * c->nleaf, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
   if (trial)
     HIPCHK(hipMemcpyAsync(c->h_span, c->d_span, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
@@ -1746,13 +1751,15 @@ int step_begin(amr_ctx* c) {
   fill_params(c);
   const bool device_timing = (c->prof_mask & 0x1fu) != 0;
   c->step_trial = tuning(c);
-  if (c->graph_on && !device_timing && !c->step_trial) {
+  if (c->graph_on && !device_timing && !c->step_trial && c->steps_same_topo > 0) {
     if (c->graph_dirty || !c->gexec)
       if (int e = capture_step(c)) return e;
     HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    if (c->prof_mask & (1u << PF_GRAPH)) c->prof[PF_GRAPH].launches += 1;
   } else {
     if (int e = enqueue_step(c)) return e;
   }
+  c->steps_same_topo++;
   c->in_flight = true;
   return 0;
 }
@@ -1941,10 +1948,9 @@ int amr_destroy(amr_ctx* c) {
       hipFree(P.d_gen);
     }
   }
-  hipFree(c->d_ip); hipFree(c->d_lam); hipFree(c->d_grad); hipFree(c->d_err); hipFree(c->d_rops);
+  hipFree(c->d_ip); hipFree(c->d_red); hipFree(c->d_err); hipFree(c->d_rops);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
-  if (c->h_lam) hipHostFree(c->h_lam);
-  if (c->h_grad) hipHostFree(c->h_grad);
+  if (c->h_red) hipHostFree(c->h_red);
   if (c->h_err) hipHostFree(c->h_err);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);

@@ -146,10 +146,10 @@ class Simulation:
 
     # --------------------------------------------------------------- timing
     PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait",
-                        "host_compile")
+                        "host_compile", "graph")
 
     def set_graph(self, on=True):
-        """Replay each step's device work as a HIP graph (default) or launch it directly."""
+        """Replay each step's device work as a HIP graph, or launch it directly (the default)."""
         _check(self.h, lib().amr_set_graph(self.h, 1 if on else 0))
 
     def set_fill_cache(self, entries):

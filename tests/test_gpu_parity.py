@@ -179,13 +179,16 @@ def test_pressure_error_reported():
 
 
 def test_graph_replay_equals_direct_launches():
-    """The step replayed as a HIP graph (re-captured at every tree change)
-    equals direct launches bit for bit, replay after replay."""
+    """The step replayed as a HIP graph (captured once a topology has lasted
+    a step, re-captured after every tree change) equals direct launches bit
+    for bit, replay after replay."""
     import amr2d_mhd_amd as A
 
     cfg = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
     g, d = A.Simulation(cfg), A.Simulation(cfg)
+    g.set_graph(True)
     d.set_graph(False)
+    g.profile(("graph",))
     g.init()
     d.init()
     for s in range(120):
@@ -195,6 +198,7 @@ def test_graph_replay_equals_direct_launches():
             diff = compare_states(gpu_state(g), gpu_state(d))
             assert not diff, f"after step {s}: " + "; ".join(diff[:4])
     assert g.info() == d.info()
+    assert g.profile_read()["graph"][1] > 20  # replays happened
 
 
 def test_fill_program_cache_equals_recompiling():
