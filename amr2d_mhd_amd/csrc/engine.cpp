@@ -18,6 +18,10 @@
 #include <cstring>
 #include <functional>
 #include <thread>
+#include <atomic>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,32 +69,6 @@ struct RNode {
   int64_t ref[4];
 };
 
-// host scratch reused across compiles (cur[] is reset only where touched)
-struct FillScratch {
-  // symbolic value per ghost-ring element (the only ones fills write),
-  // [block][ring index]: -1 = pre-fill value, >= 0 = the pre-fill value of
-  // element (nb << 16) | cell, <= -2 = restriction node -(v + 2)
-  std::vector<int32_t> cur;
-  std::vector<int32_t> touched;   // written elements, (nb << 16) | cell, in first-write order
-  std::vector<RNode> nodes;
-  struct RingSrc {
-    int32_t op;
-    int32_t nref;
-    int64_t ref[4];  // block-local operands
-  };
-  std::vector<RingSrc> ringsrc;   // ops with an operand in a ghost ring
-  std::vector<int32_t> uf;        // union-find over ops linked by stale reads (-1: not linked)
-  std::vector<int32_t> order;
-  std::vector<FillOp> moved;
-  std::vector<uint8_t> ring;      // per cell of a tile: 1 in the ghost ring
-  std::vector<int32_t> ring_idx;  // per cell: index among the ring cells, -1 inside
-  int nring = 0;
-  int ring_nt = 0;
-  bool force_general = false;  // test hook: emit every restriction as kFillGeneral
-  bool interior_write = false;  // a fill wrote an interior element (never: rejected)
-  std::vector<int32_t> group_of, group_count;
-};
-
 // page-locked growable array, so uploads are true async DMA (falls back to
 // ordinary memory where no device is present, e.g. for the host test hooks)
 template <class T>
@@ -136,8 +114,161 @@ struct PinnedVec {
     if (n == cap) reserve(n + 1);
     p[n++] = v;
   }
+  // size m, new elements left unset
+  void resize_uninit(size_t m) {
+    reserve(m);
+    n = m;
+  }
+  void swap(PinnedVec& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(cap, o.cap);
+    std::swap(pinned, o.pinned);
+  }
   size_t size() const { return n; }
   T* data() { return p; }
+};
+
+// an op with an operand in a ghost ring (a possible stale read)
+struct RingSrc {
+  int32_t op;
+  int32_t nref;
+  int64_t ref[4];  // block-local operands
+};
+
+// Persistent helper threads for the ranges of one compile: run(R, f) calls
+// f(r) for every r in [0, R) (R < 256) on the helpers and the caller and
+// returns when all have finished.  The work counter carries its generation,
+// so a helper that wakes late never takes an index of a later call.  Helpers
+// and caller spin briefly before they block: the phases of one compile
+// follow each other within microseconds.
+class RangePool {
+ public:
+  explicit RangePool(int helpers) {
+    for (int i = 0; i < helpers; i++) th_.emplace_back([this] { loop(); });
+  }
+  RangePool(const RangePool&) = delete;
+  RangePool& operator=(const RangePool&) = delete;
+  ~RangePool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_.store(true, std::memory_order_release);
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  int helpers() const { return (int)th_.size(); }
+  template <class F>
+  void run(int R, const F& f) {
+    const std::function<void(int)> job = [&f](int r) { f(r); };
+    uint64_t g;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      g = gen_.load(std::memory_order_relaxed) + 1;
+      job_ = &job;
+      pending_.store(R, std::memory_order_relaxed);
+      state_.store((g << 16) | ((uint64_t)R << 8), std::memory_order_release);
+      gen_.store(g, std::memory_order_release);
+    }
+    cv_.notify_all();
+    work(g);
+    if (spin([this] { return pending_.load(std::memory_order_acquire) == 0; })) return;
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return pending_.load(std::memory_order_acquire) == 0; });
+  }
+
+ private:
+  void work(uint64_t g) {
+    uint64_t v = state_.load(std::memory_order_acquire);
+    for (;;) {
+      if ((v >> 16) != g) return;
+      const int R = (int)((v >> 8) & 0xff), r = (int)(v & 0xff);
+      if (r >= R) return;
+      if (!state_.compare_exchange_weak(v, v + 1, std::memory_order_acq_rel, std::memory_order_acquire)) continue;
+      (*job_)(r);
+      if (pending_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> lk(m_);
+        done_.notify_all();
+      }
+      v = state_.load(std::memory_order_acquire);
+    }
+  }
+  // true once ready() holds, false after ~200 us without
+  template <class P>
+  static bool spin(const P& ready) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 1;; i++) {
+      if (ready()) return true;
+      __builtin_ia32_pause();
+      if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) return false;
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      auto fresh = [&] { return stop_.load(std::memory_order_acquire) || gen_.load(std::memory_order_acquire) != seen; };
+      if (!spin(fresh)) {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, fresh);
+      }
+      if (stop_.load(std::memory_order_acquire)) return;
+      seen = gen_.load(std::memory_order_acquire);
+      work(seen);
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  std::atomic<uint64_t> state_{0};  // generation << 16 | R << 8 | next index
+  std::atomic<int> pending_{0};
+  std::atomic<uint64_t> gen_{0};
+  std::atomic<bool> stop_{false};
+  const std::function<void(int)>* job_ = nullptr;
+};
+
+// Symbolic fill state of one contiguous range of the schedule.  A compile
+// splits the schedule into ranges executed independently (in parallel on
+// large trees); each range sees the pre-range state as "pre-fill" values,
+// and the merge substitutes what earlier ranges wrote.
+struct FillWorker {
+  // symbolic value per ghost-ring element (the only ones fills write),
+  // [block][ring index]: -1 = not written in this range, >= 0 = the value
+  // element (nb << 16) | cell held when the range began, <= -2 = restriction
+  // node -(v + 2) of this range.  Reset only where touched.
+  std::vector<int32_t> cur;
+  std::vector<int32_t> touched;  // elements written in this range, in first-write order
+  std::vector<RNode> nodes;
+  bool interior_write = false;
+  // elements first written (over all ranges) in this range, with their final
+  // values in pre-fill terms; nkeep of them change
+  std::vector<int32_t> first, fval;
+  int nkeep = 0;
+  // emission of this range's ops
+  std::vector<FillGen> gen;
+  std::vector<int32_t> gen_ops;  // ops that index gen
+  std::vector<RingSrc> ringsrc;  // ops with an operand in a ghost ring
+  bool nested = false;           // a restriction of a restriction (rejected)
+  std::vector<std::array<int32_t, 2>> edges;  // (reader op, writer op) of stale reads
+  std::vector<int32_t> linked, gid;           // linked ops of this range's op segment, their groups
+};
+
+// host scratch reused across compiles (tables reset only where touched)
+struct FillScratch {
+  std::vector<FillWorker> w;
+  std::vector<uint16_t> wmask;   // per ring element: bit r set when range r writes it
+  std::vector<RNode> rnodes;     // restriction nodes of all ranges, in pre-fill terms
+  std::vector<int32_t> nodeoff;  // first rnodes index of each range
+  std::vector<int32_t> seq;      // schedule entries in execution order
+  std::vector<int32_t> uf;       // union-find over ops linked by stale reads
+  std::vector<FillOp> moved;
+  std::vector<uint8_t> ring;      // per cell of a tile: 1 in the ghost ring
+  std::vector<int32_t> ring_idx;  // per cell: index among the ring cells, -1 inside
+  int nring = 0;
+  int ring_nt = 0;
+  bool force_general = false;  // test hook: emit every restriction as kFillGeneral
+  int force_ranges = 0;        // test hook: number of schedule ranges (0: by tree size)
+  std::vector<int32_t> group_count, group_pos;
+  std::unique_ptr<RangePool> pool;  // helpers of a compile split over threads
 };
 
 // A compiled ghost-fill program kept for reuse, with its device copy: AMR
@@ -554,14 +685,14 @@ void build_schedule(Instance& I) {
 struct FillCompiler {
   const Instance& I;
   int NT, ng, N2;
-  FillScratch& S;
+  const FillScratch& S;
+  FillWorker& W;
 
-  FillCompiler(const Instance& inst, int NT_, int ng_, FillScratch& s) : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), S(s) {
-    const size_t need = (size_t)I.lastActive * S.nring;
-    if (S.cur.size() < need) S.cur.resize(need, -1);
-    S.touched.clear();
-    S.nodes.clear();
-    S.interior_write = false;
+  FillCompiler(const Instance& inst, int NT_, int ng_, const FillScratch& s, FillWorker& w)
+      : I(inst), NT(NT_), ng(ng_), N2(NT_ * NT_), S(s), W(w) {
+    W.touched.clear();
+    W.nodes.clear();
+    W.interior_write = false;
   }
   static int32_t lref(int nb, int cell) { return (nb << 16) | cell; }
   // table slot of ring cell `cell` of block nb
@@ -571,20 +702,20 @@ struct FillCompiler {
   int32_t read(int nb, int i, int j) const {
     const int cell = j * NT + i;
     if (!S.ring[cell]) return lref(nb, cell);
-    const int32_t v = S.cur[slot(nb, cell)];
+    const int32_t v = W.cur[slot(nb, cell)];
     return v != -1 ? v : lref(nb, cell);
   }
   void write(int nb, int i, int j, int32_t v) {
     const int cell = j * NT + i;
     if (!S.ring[cell]) {
-      S.interior_write = true;
+      W.interior_write = true;
       return;
     }
     const size_t a = slot(nb, cell);
-    if (S.cur[a] == -1) S.touched.push_back((nb << 16) | cell);
-    S.cur[a] = v;
+    if (W.cur[a] == -1) W.touched.push_back((nb << 16) | cell);
+    W.cur[a] = v;
   }
-  bool unwritten(int nb, int cell) const { return !S.ring[cell] || S.cur[slot(nb, cell)] == -1; }
+  bool unwritten(int nb, int cell) const { return !S.ring[cell] || W.cur[slot(nb, cell)] == -1; }
   void copy(int dn, int di, int dj, int sn, int si, int sj) { write(dn, di, dj, read(sn, si, sj)); }
   void restrict_(int dn, int di, int dj, int fn, int i1, int j1) {
     RNode nd{};
@@ -599,11 +730,8 @@ struct FillCompiler {
       nd.ref[2] = read(fn, i1 + 1, j1);
       nd.ref[3] = read(fn, i1 + 1, j1 + 1);
     }
-    S.nodes.push_back(nd);
-    write(dn, di, dj, -(int32_t)S.nodes.size() - 1);  // node index size-1 -> -(index + 2)
-  }
-  void finish() {
-    for (int32_t t : S.touched) S.cur[slot(t >> 16, t & 0xffff)] = -1;
+    W.nodes.push_back(nd);
+    write(dn, di, dj, -(int32_t)W.nodes.size() - 1);  // node index size-1 -> -(index + 2)
   }
   // corner of an entry: the diagonal block `cb` found through the neighbour
   // table is copied when it is a leaf (with same-level indices even if it is
@@ -632,10 +760,15 @@ struct FillCompiler {
       }
   }
 
-  void run() {
+  // entries seq[k0, k1) of the schedule in execution order
+  void run(int k0, int k1) {
+    for (int k = k0; k < k1; k++) entry(I.sched[S.seq[k]]);
+  }
+
+  void entry(const std::array<int32_t, 3>& e) {
     const int lo = ng, hi = NT - 1 - ng, lob = lo - 1, hib = hi + 1, h2 = (NT - 1) / 2, top = hib;
     auto id = [](int i, int j, int& a, int& b) { a = i; b = j; };
-    for (const auto& e : I.sched) {
+    {
       const int nbs = e[0], n1 = e[1];
       switch (e[2]) {
         case 1:   // left neighbour, same level (boundary.cpp:17-35)
@@ -769,14 +902,8 @@ struct FillCompiler {
             }
           break;
         }
-        default: break;
-      }
-    }
-    // extboundary.cpp:5-61: mirror of the first ng interior rows/columns,
-    // applied to every field kind alike, after all internal entries
-    for (const auto& e : I.sched) {
-      const int nbs = e[0];
-      switch (e[2]) {
+        // extboundary.cpp:5-61: mirror of the first ng interior rows/columns,
+        // applied to every field kind alike, after all internal entries
         case -1:
           for (int g = 0; g < ng; g++)
             for (int j = 0; j < NT; j++) copy(nbs, g, j, nbs, lo + ng - 1 - g, j);
@@ -799,6 +926,22 @@ struct FillCompiler {
   }
 };
 
+// f(r) for r in [0, R) on the scratch's helper threads and the caller
+template <class F>
+void for_ranges(FillScratch& S, int R, int nthreads, const F& f) {
+  if (R == 1 || nthreads <= 1) {
+    for (int r = 0; r < R; r++) f(r);
+    return;
+  }
+  if (!S.pool || S.pool->helpers() != nthreads - 1) S.pool.reset(new RangePool(nthreads - 1));
+  S.pool->run(R, f);
+}
+
+// ring elements (blocks x ring cells) from which a compile is split over
+// threads: below, the helpers' wake-ups and the extra merge pass cost more
+// than they save
+constexpr size_t kSplitSlots = size_t(1) << 20;
+
 // Program of one instance: 16-byte destination ops plus the explicit-operand
 // table of restrictions that read an element already written in the same
 // fill.  Nested restrictions (a restriction reading another restriction's
@@ -806,12 +949,23 @@ struct FillCompiler {
 // restriction reads are interior cells or, for nghosts 4, ghost cells that
 // only the same entry's later injection writes -- and are rejected.
 //
+// Parallel symbolic execution: the schedule (internal entries, then the
+// external ones) is cut into R contiguous ranges run independently, each on
+// its own table, treating the state at its start as "pre-fill".  Range r's
+// reference to the start value of a ring element that an earlier range wrote
+// is then replaced by that range's value (recursively, through the last
+// earlier writer); a restriction whose operands an earlier range wrote loses
+// its `simple` form the same way.  The final value of an element is the one
+// of the last range that writes it, and ops are laid out in first-write order
+// over the whole schedule -- so the program is the one a single sequential
+// pass produces, whatever R is.
+//
 // Chunking: an op may read (as a pre-fill value) an element that another op
 // writes -- the reference's stale corner reads.  Such readers and writers
 // are linked into groups that are laid out inside one kFillChunk chunk, so
 // the single-pass kernel (read all, barrier, write all per chunk) keeps the
 // sequential semantics.  The segment is padded with kFillNop to whole chunks.
-int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
+int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads = 1) {
   I.ops.clear();
   I.gen.clear();
   const int N2 = NT * NT;
@@ -826,157 +980,349 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S) {
       if (S.ring[c]) S.ring_idx[c] = S.nring++;
     }
     S.ring_nt = NT;
-    S.cur.clear();
+    for (auto& w : S.w) w.cur.clear();
   }
   if (I.lastActive > 32767) return AMR_ERR_CONFIG;  // block ids are 15-bit in the symbolic values
-  FillCompiler fc(I, NT, ng, S);
-  fc.run();
-  int status = S.interior_write ? AMR_ERR_CONFIG : 0;
-  // 1. ops in first-write order; a written element's cur[] entry becomes its
-  //    op index (>= 0; -1 where the value is unchanged) for the stale-read
-  //    search below
-  S.ringsrc.clear();
-  S.order.clear();
   const uint8_t* ring = S.ring.data();
   const int32_t* ridx = S.ring_idx.data();
   const int32_t* gs = I.gs.data();
   const int nring = S.nring;
-  I.ops.reserve(S.touched.size() + 2 * kFillChunk);
-  for (int32_t t : S.touched) {
-    const int nb = t >> 16, cell = t & 0xffff;
-    const size_t a = (size_t)nb * nring + ridx[cell];
-    const int32_t v = S.cur[a];
-    FillOp op{};
-    op.dst_gs = gs[nb];
-    op.dst_cell = (uint16_t)cell;
-    FillScratch::RingSrc rs;
-    rs.nref = 0;
-    if (v >= 0) {
-      if (v == t) {  // value unchanged
-        S.cur[a] = -1;
+  // table slot of element t = (nb << 16) | cell
+  auto slot = [&](int32_t t) { return (size_t)(t >> 16) * nring + ridx[t & 0xffff]; };
+  // schedule in execution order: internal entries, then extboundary's
+  S.seq.clear();
+  for (size_t e = 0; e < I.sched.size(); e++)
+    if (I.sched[e][2] > 0) S.seq.push_back((int32_t)e);
+  for (size_t e = 0; e < I.sched.size(); e++)
+    if (I.sched[e][2] < 0) S.seq.push_back((int32_t)e);
+  const int nseq = (int)S.seq.size();
+  const size_t nslots = (size_t)I.lastActive * nring;
+  int R = 1;
+  if (S.force_ranges > 0) R = S.force_ranges;
+  else if (nthreads > 1 && nslots >= kSplitSlots) R = nthreads;
+  R = std::max(1, std::min({R, 16, std::max(1, nseq)}));
+  if ((int)S.w.size() < R) S.w.resize(R);
+  if (R > 1 && S.wmask.size() < nslots) S.wmask.resize(nslots, 0);
+  for (int r = 0; r < R; r++)
+    if (S.w[r].cur.size() < nslots) S.w[r].cur.resize(nslots, -1);
+
+  // 1. every range on its own table; with several, wmask records which
+  //    ranges write what
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    FillCompiler fc(I, NT, ng, S, W);
+    fc.run((int)((int64_t)nseq * r / R), (int)((int64_t)nseq * (r + 1) / R));
+    if (R > 1) {
+      const uint16_t bit = (uint16_t)(1u << r);
+      for (int32_t t : W.touched) __atomic_fetch_or(&S.wmask[slot(t)], bit, __ATOMIC_RELAXED);
+    }
+  });
+  S.nodeoff.assign(R + 1, 0);
+  for (int r = 0; r < R; r++) S.nodeoff[r + 1] = S.nodeoff[r] + (int32_t)S.w[r].nodes.size();
+
+  // value v of range r in pre-fill terms: a start value of a ring element an
+  // earlier range wrote becomes that range's value; nodes get global indices
+  auto subst = [&](int r, int32_t v) -> int32_t {
+    for (;;) {
+      if (v <= -2) return -(2 + S.nodeoff[r] + (-v - 2));
+      const int cell = v & 0xffff;
+      if (!ring[cell]) return v;
+      const size_t a = (size_t)(v >> 16) * nring + ridx[cell];
+      const uint32_t m = S.wmask[a] & ((1u << r) - 1u);
+      if (!m) return v;
+      r = 31 - __builtin_clz(m);
+      v = S.w[r].cur[a];
+    }
+  };
+
+  // 2. several ranges: per range, its nodes in pre-fill terms and the final
+  //    values of the elements it writes first (one range: its own table)
+  if (R > 1) {
+    S.rnodes.resize(S.nodeoff[R]);
+    for_ranges(S, R, nthreads, [&](int r) {
+      FillWorker& W = S.w[r];
+      const uint32_t before = (1u << r) - 1u;
+      for (size_t k = 0; k < W.nodes.size(); k++) {
+        const RNode& nd = W.nodes[k];
+        RNode o = nd;
+        if (nd.simple) {
+          bool still = true;
+          for (int q = 0; q < 4 && r > 0; q++) {
+            const int c = nd.cell + off[q];
+            if (ring[c] && (S.wmask[(size_t)nd.nb * nring + ridx[c]] & before)) still = false;
+          }
+          o.simple = still;
+          if (!still)
+            for (int q = 0; q < 4; q++) o.ref[q] = subst(r, FillCompiler::lref(nd.nb, nd.cell + off[q]));
+        } else {
+          for (int q = 0; q < 4; q++) o.ref[q] = subst(r, (int32_t)nd.ref[q]);
+        }
+        S.rnodes[S.nodeoff[r] + k] = o;
+      }
+      W.first.clear();
+      W.fval.clear();
+      W.nkeep = 0;
+      for (int32_t t : W.touched) {
+        const size_t a = slot(t);
+        const uint32_t m = S.wmask[a];
+        if ((int)__builtin_ctz(m) != r) continue;
+        const int rl = 31 - __builtin_clz(m);
+        const int32_t v = subst(rl, S.w[rl].cur[a]);
+        W.first.push_back(t);
+        W.fval.push_back(v);
+        W.nkeep += v != t;  // v == t: the element keeps its own value, no op
+      }
+    });
+  }
+  const RNode* nodes = R > 1 ? S.rnodes.data() : S.w[0].nodes.data();
+  bool interior_write = false;
+  std::vector<int32_t> opoff(R + 1, 0);
+  for (int r = 0; r < R; r++) {
+    interior_write |= S.w[r].interior_write;
+    opoff[r + 1] = opoff[r] + (R > 1 ? S.w[r].nkeep : (int)S.w[0].touched.size());
+  }
+  I.ops.reserve((size_t)opoff[R] + kFillChunk);
+  I.ops.resize_uninit(opoff[R]);
+
+  // 3. ops in first-write order; the first-writing range's table entry of a
+  //    written element becomes its op index (-1: no op) for the stale-read
+  //    search below
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    W.gen.clear();
+    W.gen_ops.clear();
+    W.ringsrc.clear();
+    W.nested = false;
+    // locals, so that the stores below cannot be taken to alias them
+    const uint8_t* __restrict rg = ring;
+    const int32_t* __restrict ri = ridx;
+    const int32_t* __restrict gsl = gs;
+    const RNode* __restrict nds = nodes;
+    int32_t* __restrict cur = W.cur.data();
+    const int32_t* __restrict tl = R > 1 ? W.first.data() : W.touched.data();
+    const int32_t* __restrict vl = R > 1 ? W.fval.data() : nullptr;
+    FillOp* __restrict ops = I.ops.data();
+    const int nr = nring, nt = NT;
+    const size_t m = R > 1 ? W.first.size() : W.touched.size();
+    int32_t o = opoff[r];
+    for (size_t i = 0; i < m; i++) {
+      const int32_t t = tl[i];
+      const size_t a = (size_t)(t >> 16) * nr + ri[t & 0xffff];
+      const int32_t v = vl ? vl[i] : cur[a];
+      if (v == t) {  // the element keeps its own value: no op
+        cur[a] = -1;
         continue;
       }
-      op.op = kFillCopy;
-      op.src = gs[v >> 16];
-      op.src_cell = (uint16_t)(v & 0xffff);
-      if (ring[op.src_cell]) { rs.nref = 1; rs.ref[0] = v; }
-    } else {
-      const RNode& nd = S.nodes[-v - 2];
-      if (nd.simple && !S.force_general) {
-        op.op = kFillRestrict;
-        op.src = gs[nd.nb];
-        op.src_cell = nd.cell;
-        if (ring[nd.cell] | ring[nd.cell + 1] | ring[nd.cell + NT] | ring[nd.cell + NT + 1]) {
-          rs.nref = 4;
-          for (int q = 0; q < 4; q++) rs.ref[q] = (nd.nb << 16) | (nd.cell + off[q]);
-        }
+      FillOp op{};
+      op.dst_gs = gsl[t >> 16];
+      op.dst_cell = (uint16_t)(t & 0xffff);
+      RingSrc rs;
+      rs.nref = 0;
+      if (v >= 0) {
+        op.op = kFillCopy;
+        op.src = gsl[v >> 16];
+        op.src_cell = (uint16_t)(v & 0xffff);
+        if (rg[op.src_cell]) { rs.nref = 1; rs.ref[0] = v; }
       } else {
-        FillGen g{};
-        rs.nref = 4;
-        for (int q = 0; q < 4; q++) {
-          const int64_t r = nd.simple ? (int64_t)((nd.nb << 16) | (nd.cell + off[q])) : nd.ref[q];
-          if (r < 0) status = AMR_ERR_CONFIG;  // a restriction of a restriction
-          rs.ref[q] = r;
-          g.ref[q] = src_ref(gs[r >> 16], (int)(r & 0xffff));
+        const RNode& nd = nds[-v - 2];
+        if (nd.simple) {
+          op.op = kFillRestrict;
+          op.src = gsl[nd.nb];
+          op.src_cell = nd.cell;
+          if (rg[nd.cell] | rg[nd.cell + 1] | rg[nd.cell + nt] | rg[nd.cell + nt + 1]) {
+            rs.nref = 4;
+            for (int q = 0; q < 4; q++) rs.ref[q] = (nd.nb << 16) | (nd.cell + off[q]);
+          }
+        } else {
+          FillGen g{};
+          rs.nref = 4;
+          for (int q = 0; q < 4; q++) {
+            const int64_t ref = nd.ref[q];
+            rs.ref[q] = ref;
+            if (ref < 0) {  // a restriction of a restriction
+              W.nested = true;
+              continue;
+            }
+            g.ref[q] = src_ref(gsl[ref >> 16], (int)(ref & 0xffff));
+          }
+          op.op = kFillGeneral;
+          op.src = (int32_t)W.gen.size();
+          W.gen_ops.push_back(o);
+          W.gen.push_back(g);
         }
-        op.op = kFillGeneral;
-        op.src = (int32_t)I.gen.size();
-        I.gen.push_back(g);
       }
+      if (rs.nref) {
+        rs.op = o;
+        W.ringsrc.push_back(rs);
+      }
+      cur[a] = o;  // from here on: the op writing the element
+      ops[o++] = op;
     }
-    if (rs.nref) {
-      rs.op = (int32_t)I.ops.size();
-      S.ringsrc.push_back(rs);
-    }
-    S.cur[a] = (int32_t)I.ops.size();
-    I.ops.push_back(op);
+    if (R == 1) opoff[1] = o;
+  });
+  const int n = opoff[R];
+  I.ops.resize_uninit(n);
+  int status = interior_write ? AMR_ERR_CONFIG : 0;
+  for (int r = 0; r < R; r++) {
+    FillWorker& W = S.w[r];
+    if (W.nested) status = AMR_ERR_CONFIG;
+    const int32_t base = (int32_t)I.gen.size();
+    for (int32_t q : W.gen_ops) I.ops.data()[q].src += base;
+    I.gen.insert(I.gen.end(), W.gen.begin(), W.gen.end());
   }
-  const int n = (int)I.ops.size();
-  // 2. link every op that reads (pre-fill) an element another op writes with
-  //    that writer; fills write ghost-ring elements only, so only operands in
-  //    a ring can be written ones
-  bool linked = false;
-  auto root = [&](int x) {
-    while (S.uf[x] != x) x = S.uf[x] = S.uf[S.uf[x]];
-    return x;
-  };
-  for (const auto& rs : S.ringsrc)
-    for (int t = 0; t < rs.nref; t++) {
-      const int64_t ref = rs.ref[t];
-      if (ref < 0) continue;
-      const int rc = (int)(ref & 0xffff);
-      if (!ring[rc]) continue;
-      const int32_t w = S.cur[(size_t)(ref >> 16) * nring + ridx[rc]];
-      const int q = rs.op;
-      if (w < 0 || w == q) continue;
-      if (!linked) {
-        S.uf.assign(n, -1);
-        linked = true;
+
+  // 4. stale reads: an op reading (pre-fill) an element another op writes is
+  //    linked with that writer; fills write ghost-ring elements only, so only
+  //    operands in a ring can be written ones
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    W.edges.clear();
+    for (const auto& rs : W.ringsrc)
+      for (int t = 0; t < rs.nref; t++) {
+        const int64_t ref = rs.ref[t];
+        if (ref < 0) continue;
+        const int rc = (int)(ref & 0xffff);
+        if (!ring[rc]) continue;
+        const size_t a = (size_t)(ref >> 16) * nring + ridx[rc];
+        int32_t w = -1;
+        if (R == 1) {
+          w = W.cur[a];
+        } else if (const uint32_t m = S.wmask[a]) {
+          w = S.w[__builtin_ctz(m)].cur[a];
+        }
+        if (w < 0 || w == rs.op) continue;
+        W.edges.push_back({rs.op, w});
       }
-      const int wi = w;
-      if (S.uf[q] < 0) { S.uf[q] = q; S.order.push_back(q); }
-      if (S.uf[wi] < 0) { S.uf[wi] = wi; S.order.push_back(wi); }
-      const int ra = root(q), rb = root(wi);
-      if (ra != rb) S.uf[std::max(ra, rb)] = std::min(ra, rb);
+  });
+  size_t nedges = 0;
+  for (int r = 0; r < R; r++) nedges += S.w[r].edges.size();
+  // groups = connected components (lock-free union-find, -1: not linked; a
+  // root is linked under a smaller one, so every component's root is its
+  // smallest op); the tables go back to "unwritten" where this compile
+  // touched them
+  if (nedges) S.uf.assign(n, -1);
+  int32_t* uf = S.uf.data();
+  auto find = [uf](int32_t x) {
+    for (;;) {
+      int32_t p = __atomic_load_n(&uf[x], __ATOMIC_ACQUIRE);
+      if (p == x) return x;
+      const int32_t gp = __atomic_load_n(&uf[p], __ATOMIC_ACQUIRE);
+      if (gp != p) __atomic_compare_exchange_n(&uf[x], &p, gp, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE);
+      x = gp;
     }
-  fc.finish();
-  // 3. linked groups move behind the other ops, each inside one chunk; their
-  //    old positions become padding
+  };
+  if (R == 1) {  // one thread: plain union-find, the same components and roots
+    FillWorker& W = S.w[0];
+    auto find1 = [uf](int32_t x) {
+      while (uf[x] != x) x = uf[x] = uf[uf[x]];
+      return x;
+    };
+    for (const auto& e : W.edges) {
+      if (uf[e[0]] == -1) uf[e[0]] = e[0];
+      if (uf[e[1]] == -1) uf[e[1]] = e[1];
+      const int32_t a = find1(e[0]), b = find1(e[1]);
+      if (a != b) uf[std::max(a, b)] = std::min(a, b);
+    }
+    for (int32_t t : W.touched) W.cur[slot(t)] = -1;
+  } else {
+    for_ranges(S, R, nthreads, [&](int r) {
+      FillWorker& W = S.w[r];
+      for (const auto& e : W.edges) {
+        for (int k = 0; k < 2; k++) {
+          int32_t unset = -1;
+          __atomic_compare_exchange_n(&uf[e[k]], &unset, e[k], false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE);
+        }
+        int32_t a = e[0], b = e[1];
+        for (;;) {
+          a = find(a);
+          b = find(b);
+          if (a == b) break;
+          if (a < b) std::swap(a, b);
+          int32_t expect = a;
+          if (__atomic_compare_exchange_n(&uf[a], &expect, b, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) break;
+        }
+      }
+      for (int32_t t : W.touched) {
+        const size_t a = slot(t);
+        W.cur[a] = -1;
+        __atomic_store_n(&S.wmask[a], (uint16_t)0, __ATOMIC_RELAXED);
+      }
+    });
+  }
+
+  // 5. linked ops move behind the others, grouped -- groups in the order of
+  //    their smallest op, ops of a group in op order -- each group inside one
+  //    chunk; their old positions become padding; whole chunks
   FillOp nop{};
   nop.op = kFillNop;
   auto pad_to_chunk = [&]() {
     while (I.ops.size() % kFillChunk) I.ops.push_back(nop);
   };
-  if (linked) {
-    // groups in the order of their first linked op, ops of a group in link
-    // order (a counting sort over the group index); op order inside a chunk
-    // does not matter, only that a group shares one chunk
-    const size_t no = S.order.size();
-    S.group_of.resize(no);
-    for (size_t t = 0; t < no; t++) S.group_of[t] = root(S.order[t]);
-    // number the groups; a root's union-find entry is not needed any more and
-    // holds -(group + 2) from here on
-    S.group_count.clear();
-    for (size_t t = 0; t < no; t++) {
-      const int r = S.group_of[t];
-      int g;
-      if (S.uf[r] >= 0) {
-        g = (int)S.group_count.size();
-        S.group_count.push_back(0);
-        S.uf[r] = -(g + 2);
-      } else {
-        g = -S.uf[r] - 2;
-      }
-      S.group_of[t] = g;
-      S.group_count[g] += 1;
-    }
-    const size_t ng_ = S.group_count.size();
-    std::vector<int32_t>& start = S.group_count;  // exclusive prefix sums in place
-    int32_t acc = 0;
-    for (size_t g = 0; g < ng_; g++) {
-      const int32_t cnt = start[g];
-      if (cnt > kFillChunk) return AMR_ERR_CONFIG;
-      start[g] = acc;
-      acc += cnt;
-    }
-    S.moved.resize(no);
-    FillOp* p = I.ops.data();
-    for (size_t t = 0; t < no; t++) {
-      const int q = S.order[t];
-      S.moved[start[S.group_of[t]]++] = p[q];
-      p[q] = nop;
-    }
+  if (!nedges) {
     pad_to_chunk();
-    // start[g] now holds the end of group g
-    size_t b = 0;
-    for (size_t g = 0; g < ng_; g++) {
-      const size_t e = (size_t)start[g], m = e - b;
-      if (I.ops.size() % kFillChunk + m > (size_t)kFillChunk) pad_to_chunk();
-      for (size_t t = b; t < e; t++) I.ops.push_back(S.moved[t]);
-      b = e;
-    }
+    return status;
   }
-  pad_to_chunk();
+  // linked ops of every range's op segment with their roots (all unions are
+  // done: plain finds); roots numbered in op order, their union-find entry
+  // holding -(group + 2) from then on
+  std::vector<int32_t> gbase(R + 1, 0);
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    W.linked.clear();
+    W.gid.clear();
+    int32_t nroots = 0;
+    for (int32_t q = opoff[r]; q < opoff[r + 1]; q++)
+      if (uf[q] != -1) {
+        int32_t x = q;
+        while (uf[x] != x) x = uf[x];
+        W.linked.push_back(q);
+        W.gid.push_back(x);
+        nroots += x == q;
+      }
+    gbase[r + 1] = nroots;
+  });
+  for (int r = 0; r < R; r++) gbase[r + 1] += gbase[r];
+  const int32_t ngroups = gbase[R];
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    int32_t g = gbase[r];
+    for (size_t i = 0; i < W.linked.size(); i++)
+      if (W.gid[i] == W.linked[i]) uf[W.linked[i]] = -(g++ + 2);
+  });
+  for_ranges(S, R, nthreads, [&](int r) {
+    FillWorker& W = S.w[r];
+    for (size_t i = 0; i < W.linked.size(); i++) W.gid[i] = -uf[W.gid[i]] - 2;
+  });
+  // group sizes, then each group's place behind the unlinked ops (a group
+  // that does not fit the rest of a chunk starts the next one)
+  S.group_count.assign(ngroups, 0);
+  for (int r = 0; r < R; r++)
+    for (int32_t g : S.w[r].gid) S.group_count[g] += 1;
+  S.group_pos.resize(ngroups);
+  const size_t tail = ((size_t)n + kFillChunk - 1) / kFillChunk * kFillChunk;
+  size_t pos = tail;
+  for (int32_t g = 0; g < ngroups; g++) {
+    const size_t m = (size_t)S.group_count[g];
+    if (m > (size_t)kFillChunk) return AMR_ERR_CONFIG;
+    if (pos % kFillChunk + m > (size_t)kFillChunk) pos = (pos + kFillChunk - 1) / kFillChunk * kFillChunk;
+    S.group_pos[g] = (int32_t)pos;
+    pos += m;
+  }
+  const size_t total = (pos + kFillChunk - 1) / kFillChunk * kFillChunk;
+  // destinations, ops of a group in op order
+  for (int r = 0; r < R; r++)
+    for (int32_t& g : S.w[r].gid) g = S.group_pos[g]++;
+  I.ops.reserve(total);
+  I.ops.resize_uninit(total);
+  FillOp* p = I.ops.data();
+  for (size_t e = (size_t)n; e < total; e++) p[e] = nop;
+  for_ranges(S, R, nthreads, [&](int r) {
+    const FillWorker& W = S.w[r];
+    for (size_t i = 0; i < W.linked.size(); i++) {
+      p[W.gid[i]] = p[W.linked[i]];
+      p[W.linked[i]] = nop;
+    }
+  });
   return status;
 }
 
@@ -1378,15 +1724,19 @@ int rebuild_topology(amr_ctx* c) {
     dirty.swap(todo);
   }
   const auto t_compile = std::chrono::steady_clock::now();
-  int nthreads = (int)std::min<size_t>(dirty.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  int nthreads = (int)std::min<size_t>(dirty.size(), hw);
   if (nthreads < 1) nthreads = 1;
   if ((int)c->scratch.size() < nthreads) c->scratch.resize(nthreads);
   std::vector<int> status(dirty.size(), 0);
+  // several instances: one thread each; a single one: its schedule split
+  // over the threads (large trees only)
+  const int inner = dirty.size() == 1 ? hw : 1;
   auto work = [&](int t) {
     for (size_t q = t; q < dirty.size(); q += nthreads) {
       Instance& I = c->inst[dirty[q]];
       build_schedule(I);
-      status[q] = compile_instance(I, c->NT, c->ng, c->scratch[t]);
+      status[q] = compile_instance(I, c->NT, c->ng, c->scratch[t], inner);
     }
   };
   if (nthreads == 1 || dirty.size() < 4) {
@@ -2442,11 +2792,43 @@ int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* i
   return m;
 }
 
+// Fill program of the given tree compiled with its schedule cut into
+// `ranges` ranges on up to `nthreads` threads (ranges 0: chosen by tree size
+// as the engine does): up to `cap` ops to `ops`, up to `gcap` explicit-operand
+// entries to `gen` and their count to *ngen; returns the op count (negative
+// on error).  Any split must give the single-range program.  reuse != 0
+// compiles with one scratch kept across calls (as an engine thread does),
+// else with a fresh one.
+int amr_debug_compile_ranges(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
+                             int ranges, int nthreads, int reuse, int cap, void* ops, int gcap, void* gen,
+                             int* ngen) {
+  static FillScratch kept;
+  Instance I;
+  I.lastActive = n;
+  grow_tree(I, n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
+    I.icoord[nb] = {icoord[2 * nb], icoord[2 * nb + 1]};
+    I.leafs[nb] = (uint8_t)leafs[nb];
+    I.gs.push_back(nb);
+  }
+  build_schedule(I);
+  FillScratch fresh;
+  FillScratch& S = reuse ? kept : fresh;
+  S.force_ranges = ranges;
+  if (compile_instance(I, NT, ng, S, nthreads)) return -1;
+  const int m = (int)I.ops.size();
+  if (ops) std::memcpy(ops, I.ops.data(), sizeof(FillOp) * std::min(m, cap));
+  if (gen) std::memcpy(gen, I.gen.data(), sizeof(FillGen) * std::min<size_t>(I.gen.size(), (size_t)gcap));
+  if (ngen) *ngen = (int)I.gen.size();
+  return m;
+}
+
 // Host cost of the schedule + fill-program compile for the given tree:
-// mean nanoseconds over `reps` compiles that reuse one scratch (as the
-// engine does).
+// mean nanoseconds over `reps` compiles on up to `nthreads` threads that
+// reuse one scratch (as the engine does).
 double amr_debug_compile_ns(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
-                            int reps) {
+                            int reps, int nthreads) {
   Instance I;
   I.lastActive = n;
   grow_tree(I, n);
@@ -2458,11 +2840,11 @@ double amr_debug_compile_ns(int NT, int ng, int n, const int32_t* lp, const int3
   }
   FillScratch S;
   build_schedule(I);
-  if (compile_instance(I, NT, ng, S)) return -1;
+  if (compile_instance(I, NT, ng, S, nthreads)) return -1;
   const auto t0 = std::chrono::steady_clock::now();
   for (int r = 0; r < reps; r++) {
     build_schedule(I);
-    compile_instance(I, NT, ng, S);
+    compile_instance(I, NT, ng, S, nthreads);
   }
   return std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / std::max(1, reps);
 }

@@ -75,6 +75,46 @@ def test_fill_program_compiler_matches_reference(tmp_path, ng, lv, seed, pr, pc,
         assert np.array_equal(Q, post[fld]), fld
 
 
+def _compile(lp, ic, lf, NT, ng, ranges, threads, reuse):
+    from amr2d_mhd_amd._lib import lib
+
+    L, ngen = lib(), C.c_int(0)
+    args = (NT, ng, len(lf), lp.ctypes.data, ic.ctypes.data, lf.ctypes.data, ranges, threads, reuse)
+    m = L.amr_debug_compile_ranges(*args, 0, None, 0, None, C.byref(ngen))
+    assert m > 0
+    ops = np.zeros((m, 16), np.uint8)
+    gen = np.zeros((max(1, ngen.value), 32), np.uint8)
+    assert L.amr_debug_compile_ranges(*args, m, ops.ctypes.data, ngen.value, gen.ctypes.data, C.byref(ngen)) == m
+    return ops, gen[:ngen.value]
+
+
+@need_ref
+@pytest.mark.parametrize("ng,lv,seed,pr,pc,rounds", [(2, 6, 7, 0.3, 0.3, 8), (4, 5, 3, 0.4, 0.25, 6),
+                                                     (2, 5, 99, 0.25, 0.5, 10), (4, 5, 12, 0.35, 0.3, 8)])
+def test_split_compile_equals_single_range(tmp_path, ng, lv, seed, pr, pc, rounds):
+    """The fill program compiled with the schedule cut into ranges -- each run
+    on its own table, later ranges' reads of what earlier ones wrote
+    substituted afterwards, several threads, tables reused across compiles
+    of different splits -- is the single-range program op for op (and the
+    single-range one equals the reference's fills, test above)."""
+    rc, err = refrun.run(str(tmp_path), ng, lv, ic="BWx", lim="MC", flux="RUSA", randtree=(seed, pr, pc, rounds))
+    assert rc == 0, err
+    pre = refrun.read_state(str(tmp_path / "state_rt_pre.bin"))
+    lp, ic, lf = [np.ascontiguousarray(pre[k], dtype=np.int32) for k in ("lp", "icoord", "leafs")]
+    NT = pre["NT"]
+    ref_ops, ref_gen = _compile(lp, ic, lf, NT, ng, 1, 1, 0)
+    linked = 0
+    for ranges, threads, reuse in [(2, 1, 1), (3, 2, 1), (7, 4, 1), (16, 8, 1), (1, 1, 1), (5, 5, 0), (16, 16, 1)]:
+        ops, gen = _compile(lp, ic, lf, NT, ng, ranges, threads, reuse)
+        assert np.array_equal(ops, ref_ops), (ranges, threads, reuse)
+        assert np.array_equal(gen, ref_gen), (ranges, threads, reuse)
+    # the trees exercise the stale-read grouping (ops behind the first
+    # padded chunk boundary after the plain ops)
+    kinds = ref_ops.view(np.int16).reshape(-1, 8)[:, 3]
+    assert (kinds == 3).sum() > 0  # padding exists
+    assert len(ref_ops) % 256 == 0
+
+
 def test_schedule_matches_oracle():
     """locate_bounds order of the product equals the oracle's on evolving trees."""
     from amr2d_mhd_amd._lib import lib
