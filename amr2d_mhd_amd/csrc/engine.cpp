@@ -1200,8 +1200,10 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
   // root is linked under a smaller one, so every component's root is its
   // smallest op); the tables go back to "unwritten" where this compile
   // touched them
-  if (nedges) S.uf.assign(n, -1);
+  if ((int)S.uf.size() < n) S.uf.resize(n);
   int32_t* uf = S.uf.data();
+  if (nedges)
+    for_ranges(S, R, nthreads, [&](int r) { std::fill(uf + opoff[r], uf + opoff[r + 1], -1); });
   auto find = [uf](int32_t x) {
     for (;;) {
       int32_t p = __atomic_load_n(&uf[x], __ATOMIC_ACQUIRE);
@@ -1315,7 +1317,10 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
   I.ops.reserve(total);
   I.ops.resize_uninit(total);
   FillOp* p = I.ops.data();
-  for (size_t e = (size_t)n; e < total; e++) p[e] = nop;
+  for_ranges(S, R, nthreads, [&](int r) {
+    // padding of this range's share of the tail, then its linked ops
+    std::fill(p + n + (total - n) * r / R, p + n + (total - n) * (r + 1) / R, nop);
+  });
   for_ranges(S, R, nthreads, [&](int r) {
     const FillWorker& W = S.w[r];
     for (size_t i = 0; i < W.linked.size(); i++) {

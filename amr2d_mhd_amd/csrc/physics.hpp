@@ -230,11 +230,43 @@ __host__ __device__ inline void star_cons(const double* w, double e, double* u) 
   u[7] = w[7];
 }
 
-// riemannS.cpp:397-549 (Miyoshi & Kusano 2005 HLLD)
+// scalars of side_state only (the conserved vector and flux are formed where
+// a flux region needs them, with the same expressions)
+struct SideScalars {
+  double e, H, pt, cf;
+};
+
+__host__ __device__ inline void side_scalars(const double* w, double g, SideScalars& s) {
+  s.cf = fast_speed(w, g, 5);
+  const double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
+  double e = (w[4] / (w[0] * (g - 1))) + 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) + 0.5 * b2 / w[0];
+  e = e * w[0];
+  s.e = e;
+  s.H = (e + w[4] + 0.5 * b2) / w[0];
+  s.pt = w[4] + 0.5 * b2;
+}
+
+// side_state's x-flux from its scalars
+__host__ __device__ inline void side_flux(const double* w, const SideScalars& s, double* F) {
+  const double vn = w[1], Bn = w[5];
+  const double mv = w[0] * vn;
+  F[0] = mv;
+  F[1] = mv * w[1] + s.pt - Bn * w[5];
+  F[2] = mv * w[2] - Bn * w[6];
+  F[3] = mv * w[3] - Bn * w[7];
+  F[4] = mv * s.H - Bn * (w[1] * w[5] + w[2] * w[6] + w[3] * w[7]);
+  F[5] = vn * w[5] - Bn * w[1];
+  F[6] = vn * w[6] - Bn * w[2];
+  F[7] = vn * w[7] - Bn * w[3];
+}
+
+// riemannS.cpp:397-549 (Miyoshi & Kusano 2005 HLLD).  Only the selected
+// region's side flux and conserved vectors are formed: fewer live registers
+// in the stage kernel, the same operations per value.
 __host__ __device__ inline void flux_hlld(const double* wL, const double* wR, double g, double* f) {
-  Side L, R;
-  side_state(wL, g, L);
-  side_state(wR, g, R);
+  SideScalars L, R;
+  side_scalars(wL, g, L);
+  side_scalars(wR, g, R);
   const double vL = wL[1], vR = wR[1];
   const double cmax = std_max(L.cf, R.cf);
   const double SL = std_min(vL, vR) - cmax;
@@ -262,31 +294,37 @@ __host__ __device__ inline void flux_hlld(const double* wL, const double* wR, do
   const double eRss = eRs + rr * (vdotb(wRs) - dss) * sign0(wR[5]);
   const double SLs = SM - fabs(wL[5]) / rl;
   const double SRs = SM + fabs(wR[5]) / rr;
-  double qs[8], qss[8];
+  double F[8], q[8], qs[8], qss[8];
   if (SL > 0) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = L.F[k];
+    side_flux(wL, L, f);
   } else if (SL <= 0. && SLs >= 0.) {
+    side_flux(wL, L, F);
+    star_cons(wL, L.e, q);
     star_cons(wLs, eLs, qs);
 #pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = L.F[k] + SL * qs[k] - SL * L.q[k];
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SL * qs[k] - SL * q[k];
   } else if (SLs <= 0. && SM >= 0.) {
+    side_flux(wL, L, F);
+    star_cons(wL, L.e, q);
     star_cons(wLs, eLs, qs);
     star_cons(wss, eLss, qss);
 #pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = L.F[k] + SLs * qss[k] - (SLs - SL) * qs[k] - SL * L.q[k];
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SLs * qss[k] - (SLs - SL) * qs[k] - SL * q[k];
   } else if (SM <= 0. && SRs >= 0.) {
+    side_flux(wR, R, F);
+    star_cons(wR, R.e, q);
     star_cons(wRs, eRs, qs);
     star_cons(wss, eRss, qss);
 #pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = R.F[k] + SRs * qss[k] - (SRs - SR) * qs[k] - SR * R.q[k];
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SRs * qss[k] - (SRs - SR) * qs[k] - SR * q[k];
   } else if (SRs <= 0. && SR >= 0.) {
+    side_flux(wR, R, F);
+    star_cons(wR, R.e, q);
     star_cons(wRs, eRs, qs);
 #pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = R.F[k] + SR * qs[k] - SR * R.q[k];
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SR * qs[k] - SR * q[k];
   } else if (SR < 0.) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = R.F[k];
+    side_flux(wR, R, f);
   }
 }
 

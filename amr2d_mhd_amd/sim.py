@@ -11,6 +11,9 @@ import numpy as np
 
 from ._lib import FIELD_VARS, FIELDS, FILL_KIND, SOLVERS, AmrError, Diag, InstInfo, lib, make_config
 
+# stage-kernel variants of the library (kStageVariants in csrc/kernels.hpp)
+STAGE_VARIANTS = 5
+
 
 def _check(ctx, rc):
     if rc != 0:
@@ -157,13 +160,14 @@ class Simulation:
         _check(self.h, lib().amr_set_fill_cache(self.h, int(entries)))
 
     def set_stage_variant(self, variant):
-        """Select the stage-kernel variant (0-4, identical results; -1 = tuned per leaf-count bucket)."""
+        """Select the stage-kernel variant (0 to STAGE_VARIANTS - 1, identical results; -1 = tuned per
+        leaf-count bucket)."""
         _check(self.h, lib().amr_set_stage_variant(self.h, int(variant)))
 
     def stage_tuning(self, nbuckets=24):
         """Tuner decisions: {bucket: (chosen variant, [mean us per leaf of each variant])}."""
         chosen = np.zeros(nbuckets, np.int32)
-        us = np.zeros((nbuckets, 5))
+        us = np.zeros((nbuckets, STAGE_VARIANTS))
         _check(self.h, lib().amr_stage_tuning(self.h, nbuckets, chosen.ctypes.data, us.ctypes.data))
         return {b: (int(chosen[b]), us[b].tolist()) for b in range(nbuckets) if chosen[b] >= 0 or us[b].any()}
 
