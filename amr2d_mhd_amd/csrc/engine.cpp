@@ -381,7 +381,9 @@ struct amr_ctx {
     int n[kStageVariants] = {0};
     int chosen = -1;
   };
-  TuneBucket tune[24];
+  TuneBucket tune_by_spec[2][24];  // per stage-kernel specialisation
+  int stage_spec = kStageGeneric;   // of the running instances (set at each topology rebuild)
+  TuneBucket* tune = tune_by_spec[0];
   unsigned long long* d_span = nullptr;  // [stage 1 start, end, stage 2 start, end]
   unsigned long long* h_span = nullptr;
   int span_variant[2] = {-1, -1};
@@ -1715,6 +1717,15 @@ int rebuild_topology(amr_ctx* c) {
   }
   c->leaf_inst_begin[c->inst.size()] = (int)c->h_leaves.size();
   c->nleaf = (int)c->h_leaves.size();
+  {
+    // the specialised stage kernel serves launches whose instances all use
+    // HLLD with a MUSCL-family limiter
+    bool hlld_muscl = true;
+    for (const Instance& I : c->inst)
+      if (!I.done && !(I.solver == kFluxHLLD && I.lim < kLimWENO3)) hlld_muscl = false;
+    c->stage_spec = hlld_muscl ? kStageHLLDMuscl : kStageGeneric;
+    c->tune = c->tune_by_spec[c->stage_spec];
+  }
   if (int e = ensure(c->d_leaves, c->cap_leaves, std::max(1, c->nleaf))) return e;
   if (c->nleaf)
     HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
@@ -1876,8 +1887,8 @@ void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
     // the trial counts as pending until the step's results come back
     c->tune[tune_bucket(c->nleaf)].n[v] += 1;
   }
-  launch_stage(c->NT, c->ng, v, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream, tm.start(),
-               tm.stop(), span);
+  launch_stage(c->NT, c->ng, v, c->stage_spec, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream,
+               tm.start(), tm.stop(), span);
 }
 
 // fold the spans of this step's trial launches into the tuner

@@ -60,8 +60,10 @@ __device__ inline void stamp_end(unsigned long long* span) { atomicMax(span + 1,
 // Variants (same arithmetic, same results): TPB lanes per leaf, and the face
 // fluxes kept in LDS for the RK update (FLDS) or read back from the global
 // ff/gg the kernel has just written (smaller LDS footprint, more resident
-// workgroups per CU).
-template <int NT, int NG, int TPB, bool FLDS, int WPE>
+// workgroups per CU).  SPEC = kStageHLLDMuscl fixes the solver to HLLD and the
+// reconstruction to the MUSCL family at compile time (used when every leaf of
+// the launch has them): the other solvers' and WENO's registers drop out.
+template <int NT, int NG, int TPB, bool FLDS, int WPE, int SPEC = kStageGeneric>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
     FieldPtrs F, const LeafItem* __restrict__ leaves, const InstParams* __restrict__ ip, int step, DevError* err,
     unsigned long long* span) {
@@ -76,7 +78,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const LeafItem L = leaves[blockIdx.x];
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
-  const int lim = P.limiter, solver = P.solver;
+  constexpr bool muscl_only = SPEC == kStageHLLDMuscl;
+  const int lim = P.limiter, solver = SPEC == kStageHLLDMuscl ? (int)kFluxHLLD : P.solver;
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
   const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
@@ -130,10 +133,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const double* Wk = sW + k * N2;
-      if (lim == kLimWENO5) {
+      if (!muscl_only && lim == kLimWENO5) {
         wl[k] = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
         wr[k] = weno5_edge(Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], Wk[c + 3 * s], false);
-      } else if (lim == kLimWENO3) {
+      } else if (!muscl_only && lim == kLimWENO3) {
         wl[k] = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
         wr[k] = weno3_edge(Wk[c], Wk[c + s], Wk[c + 2 * s], false);
       } else {
@@ -651,34 +654,45 @@ bool tile_supported(int NT, int ng) { return (NT == 20 && ng == 2) || (NT == 24 
     else if (NT == 12 && NG == 2) { constexpr int T = 12, G = 2; (void)G; CALL; }   \
   } while (0)
 
-template <int T, int G, int TPB, bool FLDS, int WPE = 1>
+template <int T, int G, int TPB, bool FLDS, int WPE, int SPEC>
 void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
                           DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
                           unsigned long long* span) {
   if (ev_start && ev_stop) {
     // events attached to the dispatch itself: they time the kernel, not the launch
-    hipExtLaunchKernelGGL((stage_kernel<T, G, TPB, FLDS, WPE>), dim3(nleaf), dim3(TPB), 0, s, ev_start, ev_stop, 0,
-                          F, leaves, ip, step, err, span);
+    hipExtLaunchKernelGGL((stage_kernel<T, G, TPB, FLDS, WPE, SPEC>), dim3(nleaf), dim3(TPB), 0, s, ev_start,
+                          ev_stop, 0, F, leaves, ip, step, err, span);
     return;
   }
-  stage_kernel<T, G, TPB, FLDS, WPE><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span);
+  stage_kernel<T, G, TPB, FLDS, WPE, SPEC><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span);
 }
 
-void launch_stage(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
-                  const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
-                  hipEvent_t ev_stop, unsigned long long* span) {
-  if (nleaf <= 0) return;
-#define AMR_STAGE(TPB, FLDS, WPE)                                                                         \
-  AMR_DISPATCH_TILE(NT, ng, (launch_stage_variant<T, G, TPB, FLDS, WPE>(F, leaves, nleaf, ip, step, err, s, \
-                                                                        ev_start, ev_stop, span)))
+template <int SPEC>
+void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
+                       const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
+                       hipEvent_t ev_stop, unsigned long long* span) {
+#define AMR_STAGE(TPB, FLDS, WPE)                                                                     \
+  AMR_DISPATCH_TILE(NT, ng, (launch_stage_variant<T, G, TPB, FLDS, WPE, SPEC>(F, leaves, nleaf, ip, step, err, \
+                                                                              s, ev_start, ev_stop, span)))
   switch (variant) {
     case 1: AMR_STAGE(512, true, 1); break;
     case 2: AMR_STAGE(768, true, 1); break;
     case 3: AMR_STAGE(256, false, 4); break;
     case 4: AMR_STAGE(512, false, 4); break;
+    case 5: AMR_STAGE(512, true, 4); break;
     default: AMR_STAGE(256, true, 1); break;
   }
 #undef AMR_STAGE
+}
+
+void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
+                  const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
+                  hipEvent_t ev_stop, unsigned long long* span) {
+  if (nleaf <= 0) return;
+  if (spec == kStageHLLDMuscl)
+    launch_stage_spec<kStageHLLDMuscl>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
+  else
+    launch_stage_spec<kStageGeneric>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
 }
 
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s) {

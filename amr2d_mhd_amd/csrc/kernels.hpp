@@ -37,8 +37,11 @@ bool tile_supported(int NT, int ng);
 // The fewer-lane variants suit large leaf counts (throughput), the wide ones
 // small leaf counts (per-block latency).  A non-null `span` receives the
 // launch's device wall-clock span (see stamp_start/stamp_end).
-constexpr int kStageVariants = 5;
-void launch_stage(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
+constexpr int kStageVariants = 6;
+// stage-kernel specialisations: generic (solver and limiter read per leaf), or
+// HLLD with a MUSCL-family limiter fixed at compile time
+enum { kStageGeneric = 0, kStageHLLDMuscl = 1 };
+void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
                   hipEvent_t ev_stop = nullptr, unsigned long long* span = nullptr);
 void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s);

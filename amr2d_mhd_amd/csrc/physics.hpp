@@ -260,9 +260,12 @@ __host__ __device__ inline void side_flux(const double* w, const SideScalars& s,
   F[7] = vn * w[7] - Bn * w[3];
 }
 
-// riemannS.cpp:397-549 (Miyoshi & Kusano 2005 HLLD).  Only the selected
-// region's side flux and conserved vectors are formed: fewer live registers
-// in the stage kernel, the same operations per value.
+// riemannS.cpp:397-549 (Miyoshi & Kusano 2005 HLLD).  The flux region is
+// decided from the wave speeds and the star densities alone (the first
+// component of each star state, riemannS.cpp:467); the star states, the
+// double-star state, and the side flux and conserved vectors are formed only
+// in the region that uses them -- fewer live registers in the stage kernel,
+// the same operations per value.
 __host__ __device__ inline void flux_hlld(const double* wL, const double* wR, double g, double* f) {
   SideScalars L, R;
   side_scalars(wL, g, L);
@@ -275,12 +278,55 @@ __host__ __device__ inline void flux_hlld(const double* wL, const double* wR, do
   const double SM = ((SR - vR) * wR[0] * vR - (SL - vL) * wL[0] * vL - R.pt + L.pt) / den;
   const double pts = ((SR - vR) * wR[0] * L.pt - (SL - vL) * wL[0] * R.pt +
                       wL[0] * wR[0] * (SR - vR) * (SL - vL) * (vR - vL)) / den;
+  const double rhoLs = wL[0] * (SL - wL[1]) / (SL - SM);  // = hlld_star's ws[0]
+  const double rhoRs = wR[0] * (SR - wR[1]) / (SR - SM);
+  const double rl = sqrt(rhoLs), rr = sqrt(rhoRs);
+  const double SLs = SM - fabs(wL[5]) / rl;
+  const double SRs = SM + fabs(wR[5]) / rr;
+  int region = 0;  // 0: no branch taken (NaN speeds), f keeps its zeros
+  if (SL > 0) region = 1;
+  else if (SL <= 0. && SLs >= 0.) region = 2;
+  else if (SLs <= 0. && SM >= 0.) region = 3;
+  else if (SM <= 0. && SRs >= 0.) region = 4;
+  else if (SRs <= 0. && SR >= 0.) region = 5;
+  else if (SR < 0.) region = 6;
+  if (region == 1) {
+    side_flux(wL, L, f);
+    return;
+  }
+  if (region == 6) {
+    side_flux(wR, R, f);
+    return;
+  }
+  if (region == 0) return;
+  double F[8], q[8], qs[8];
+  if (region == 2) {
+    double wLs[8], eLs;
+    hlld_star(wL, SL, SM, pts, L.e, L.pt, wLs, eLs);
+    side_flux(wL, L, F);
+    star_cons(wL, L.e, q);
+    star_cons(wLs, eLs, qs);
+#pragma unroll
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SL * qs[k] - SL * q[k];
+    return;
+  }
+  if (region == 5) {
+    double wRs[8], eRs;
+    hlld_star(wR, SR, SM, pts, R.e, R.pt, wRs, eRs);
+    side_flux(wR, R, F);
+    star_cons(wR, R.e, q);
+    star_cons(wRs, eRs, qs);
+#pragma unroll
+    for (int k = 0; k < 8; k++) f[k] = F[k] + SR * qs[k] - SR * q[k];
+    return;
+  }
+  // regions 3 and 4: the double-star state
   double wLs[8], wRs[8], eLs, eRs;
   hlld_star(wL, SL, SM, pts, L.e, L.pt, wLs, eLs);
   hlld_star(wR, SR, SM, pts, R.e, R.pt, wRs, eRs);
-  const double rl = sqrt(wLs[0]), rr = sqrt(wRs[0]), rlr = sqrt(wLs[0] * wRs[0]);
+  const double rlr = sqrt(wLs[0] * wRs[0]);
   const double sg = sign0(wL[5]);
-  double wss[8];
+  double wss[8], qss[8];
   wss[0] = wLs[0];
   wss[1] = SM;
   wss[2] = (rl * wLs[2] + rr * wRs[2] + (wRs[6] - wLs[6]) * sg) / (rl + rr);
@@ -290,41 +336,22 @@ __host__ __device__ inline void flux_hlld(const double* wL, const double* wR, do
   wss[6] = (rl * wRs[6] + rr * wLs[6] + rlr * (wRs[2] - wLs[2]) * sg) / (rl + rr);
   wss[7] = (rl * wRs[7] + rr * wLs[7] + rlr * (wRs[3] - wLs[3]) * sg) / (rl + rr);
   const double dss = vdotb(wss);
-  const double eLss = eLs - rl * (vdotb(wLs) - dss) * sg;
-  const double eRss = eRs + rr * (vdotb(wRs) - dss) * sign0(wR[5]);
-  const double SLs = SM - fabs(wL[5]) / rl;
-  const double SRs = SM + fabs(wR[5]) / rr;
-  double F[8], q[8], qs[8], qss[8];
-  if (SL > 0) {
-    side_flux(wL, L, f);
-  } else if (SL <= 0. && SLs >= 0.) {
-    side_flux(wL, L, F);
-    star_cons(wL, L.e, q);
-    star_cons(wLs, eLs, qs);
-#pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = F[k] + SL * qs[k] - SL * q[k];
-  } else if (SLs <= 0. && SM >= 0.) {
+  if (region == 3) {
+    const double eLss = eLs - rl * (vdotb(wLs) - dss) * sg;
     side_flux(wL, L, F);
     star_cons(wL, L.e, q);
     star_cons(wLs, eLs, qs);
     star_cons(wss, eLss, qss);
 #pragma unroll
     for (int k = 0; k < 8; k++) f[k] = F[k] + SLs * qss[k] - (SLs - SL) * qs[k] - SL * q[k];
-  } else if (SM <= 0. && SRs >= 0.) {
+  } else {
+    const double eRss = eRs + rr * (vdotb(wRs) - dss) * sign0(wR[5]);
     side_flux(wR, R, F);
     star_cons(wR, R.e, q);
     star_cons(wRs, eRs, qs);
     star_cons(wss, eRss, qss);
 #pragma unroll
     for (int k = 0; k < 8; k++) f[k] = F[k] + SRs * qss[k] - (SRs - SR) * qs[k] - SR * q[k];
-  } else if (SRs <= 0. && SR >= 0.) {
-    side_flux(wR, R, F);
-    star_cons(wR, R.e, q);
-    star_cons(wRs, eRs, qs);
-#pragma unroll
-    for (int k = 0; k < 8; k++) f[k] = F[k] + SR * qs[k] - SR * q[k];
-  } else if (SR < 0.) {
-    side_flux(wR, R, f);
   }
 }
 

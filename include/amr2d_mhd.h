@@ -161,13 +161,18 @@ int amr_set_graph(amr_ctx *ctx, int on);
  * identical results): lanes per leaf block and where the face fluxes live
  * for the RK update -- 0: 256 lanes, fluxes in LDS; 1: 512 lanes, LDS;
  * 2: 768 lanes, LDS; 3: 256 lanes, fluxes read back from ff/gg, 4 waves per
- * SIMD; 4: 512 lanes, read back, 4 waves per SIMD.  -1 (the default) tunes
- * per leaf-count bucket (bucket b holds 2^b <= nleaf < 2^(b+1)): the first
- * stage launches in a bucket try every variant twice, timed by the kernel's
- * device-clock span, and the fastest per leaf block is kept. */
+ * SIMD; 4: 512 lanes, read back, 4 waves per SIMD; 5: 512 lanes, LDS, 4 waves
+ * per SIMD.  Each variant exists generic (solver and limiter read per
+ * instance) and specialised to HLLD with a MUSCL-family limiter, which
+ * launches use when every running instance has that pair.  -1 (the default)
+ * tunes per specialisation and leaf-count bucket (bucket b holds
+ * 2^b <= nleaf < 2^(b+1)): the first stage launches in a bucket try every
+ * variant twice, timed by the kernel's device-clock span, and the fastest
+ * per leaf block is kept. */
 int amr_set_stage_variant(amr_ctx *ctx, int variant);
-/* Tuner state: chosen[b] (-1 = not decided) and the mean measured
- * microseconds per leaf block of each variant, us_per_leaf[b][5]. */
+/* Tuner state of the specialisation in use: chosen[b] (-1 = not decided) and
+ * the mean measured microseconds per leaf block of each variant,
+ * us_per_leaf[b][6]. */
 int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per_leaf);
 
 /* Fill-program cache (no counterpart in the reference): after a tree change

@@ -225,7 +225,7 @@ def test_fill_program_cache_equals_recompiling():
     assert compiled < compiled_plain <= rebuilds, (compiled, compiled_plain, rebuilds)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_stage_variant_bitwise(variant):
     """Every stage-kernel variant reproduces the oracle bit for bit."""
     import amr2d_mhd_amd as A
