@@ -374,10 +374,11 @@ struct amr_ctx {
   // stage-kernel variant: fixed (0..kStageVariants-1) or chosen per leaf-count
   // bucket by the tuner (-1): the first launches in a bucket try every variant
   // kTuneTrials times, timed by the kernel's own device-clock span, then the
-  // fastest per leaf is kept for that bucket
+  // one with the fastest trial per leaf is kept for that bucket
   int stage_variant = -1;
   struct TuneBucket {
-    double us_per_leaf[kStageVariants] = {0};
+    double us_per_leaf[kStageVariants] = {0};  // sum over the trials (reported as a mean)
+    double us_min[kStageVariants] = {0};       // fastest trial (decides: noise only adds time)
     int n[kStageVariants] = {0};
     int chosen = -1;
   };
@@ -1847,7 +1848,7 @@ void run_fill(amr_ctx* c, const FillSet& s) {
   launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->nchunks, s, c->stream);
 }
 
-constexpr int kTuneTrials = 2;
+constexpr int kTuneTrials = 3;
 
 int tune_bucket(int nleaf) {
   int b = 0;
@@ -1899,14 +1900,18 @@ void tune_collect(amr_ctx* c) {
     if (v < 0) continue;
     c->span_variant[q] = -1;
     const unsigned long long t0 = ~c->h_span[2 * q], t1 = c->h_span[2 * q + 1];
-    if (t1 > t0 && c->wall_khz > 0) B.us_per_leaf[v] += (double)(t1 - t0) / (c->wall_khz * 1e-3) / c->nleaf;
+    if (t1 > t0 && c->wall_khz > 0) {
+      const double us = (double)(t1 - t0) / (c->wall_khz * 1e-3) / c->nleaf;
+      B.us_per_leaf[v] += us;
+      if (B.us_min[v] == 0. || us < B.us_min[v]) B.us_min[v] = us;
+    }
   }
   bool all = true;
   for (int v = 0; v < kStageVariants; v++) all &= B.n[v] >= kTuneTrials;
   if (all && B.chosen < 0) {
     int best = 0;
     for (int v = 1; v < kStageVariants; v++)
-      if (B.us_per_leaf[v] < B.us_per_leaf[best]) best = v;
+      if (B.us_min[v] > 0. && (B.us_min[best] == 0. || B.us_min[v] < B.us_min[best])) best = v;
     B.chosen = best;
   }
 }

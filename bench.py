@@ -40,15 +40,18 @@ ROTOR_STEPS = 4763  # steps of the full rotor run to t = 0.295 without FMA contr
 STAGE_BYTES_PER_ZONE = 176.0
 # HBM bytes per leaf zone one stage launch actually moves, from rocprofv3 FETCH_SIZE
 # (x2.0, copy-calibrated) + WRITE_SIZE (x1.0) on this workload:
-# profiles/stage_traffic_rotor_lv5.md (273.5 read + 270.0 written, tuned 768-lane variant)
-STAGE_TRAFFIC_BYTES_PER_ZONE = 543.5
+# profiles/stage_traffic_rotor_lv5.md (267.6 read + 270.0 written, 768-lane variant
+# specialised to HLLD/MUSCL)
+STAGE_TRAFFIC_BYTES_PER_ZONE = 537.6
 HBM_PEAK_GBS = 8000.0
 # FP64 FLOPs one stage launch executes per leaf zone: rocprofv3 counters
 # (SQ_INSTS_VALU_FLOPS_FP64 + SQ_INSTS_VALU_FLOPS_FP64_TRANS) x 64 lanes per
 # dispatch / (256 x leaves), median over the stage dispatches of this workload
-# (profiles/fp64_flops_rotor_lv5_adjoint_512.md); FP64 division and sqrt count
-# with their full instruction sequences, and the whole 20x20 tile is computed
-STAGE_FP64_FLOPS_PER_ZONE = 3835.0
+# (profiles/fp64_flops_rotor_lv5_adjoint_512.md, 768-lane variant specialised
+# to HLLD/MUSCL); FP64 division and sqrt count with their full instruction
+# sequences, a wave counts every branch any of its lanes takes, and the whole
+# 20x20 tile is computed
+STAGE_FP64_FLOPS_PER_ZONE = 3956.0
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -186,14 +189,20 @@ def ot_bench(steps, warmup):
     if warmup:
         sim.run(warmup)
     torch.cuda.synchronize()
+    # host-side timers only (no device events): the fill-program compiles and
+    # the whole topology rebuilds they are part of
+    sim.profile(("host_compile", "host_topology"))
     t0 = time.perf_counter()
     n, zu = sim.run(steps)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
+    prof = sim.profile_read()
     info = sim.info()
     return {"workload": "Orszag-Tang, periodic, 4 roots of 16x16, 6 levels, WENO5 + HLLD, SSP-RK3, CT, CFL 0.4",
             "warmup": warmup, "steps": n, "zone_updates": zu, "seconds": secs, "value": zu / secs if secs else None,
-            "unit": "zone-updates/s", "t_reached": info["t"], "leaves_at_end": info["nleaf"]}
+            "unit": "zone-updates/s", "t_reached": info["t"], "leaves_at_end": info["nleaf"],
+            "host_compile_ms": prof["host_compile"][0], "compiles": prof["host_compile"][1],
+            "host_topology_ms": prof["host_topology"][0]}
 
 
 def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):

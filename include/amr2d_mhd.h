@@ -167,8 +167,8 @@ int amr_set_graph(amr_ctx *ctx, int on);
  * launches use when every running instance has that pair.  -1 (the default)
  * tunes per specialisation and leaf-count bucket (bucket b holds
  * 2^b <= nleaf < 2^(b+1)): the first stage launches in a bucket try every
- * variant twice, timed by the kernel's device-clock span, and the fastest
- * per leaf block is kept. */
+ * variant three times, timed by the kernel's device-clock span, and the one
+ * with the fastest trial per leaf block is kept. */
 int amr_set_stage_variant(amr_ctx *ctx, int variant);
 /* Tuner state of the specialisation in use: chosen[b] (-1 = not decided) and
  * the mean measured microseconds per leaf block of each variant,
