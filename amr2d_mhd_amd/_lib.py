@@ -16,7 +16,7 @@ SOLVERS = {"RUSA": 0, "HLLE": 1, "HLLC": 2, "HLLD": 3}
 ERRORS = {0: "ok", 1: "non-positive pressure", 2: "NaN velocity", 3: "NaN flux", 4: "CT face mismatch",
           5: "NaN edge state", 6: "HLLC wave-speed failure", 7: "invalid dt", 8: "maxblocks exceeded",
           9: "NaN in refined face field", 10: "bad configuration", 11: "HIP runtime error",
-          12: "slot arena exhausted"}
+          12: "slot arena exhausted", 13: "host-side failure (out of host memory)"}
 
 
 class SimConfig(C.Structure):

@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "../../include/amr2d_mhd.h"
+#include "abi_guard.hpp"
 #include "adjoint_math.hpp"
 
 using namespace amr;
@@ -568,6 +569,7 @@ int reverse(amr_adj* a, const Tape& T, const double* Bx, const double* By, doubl
 extern "C" {
 
 int amr_adj_create(int n, double dx, double gamma, const char* flux, amr_adj** out) {
+  AMR_TRY
   if (!out || n < 4 || dx <= 0) return AMR_ERR_CONFIG;
   int solver = kFluxHLLE;
   if (flux && std::strcmp(flux, "HLLD") == 0) solver = kFluxHLLD;
@@ -591,9 +593,11 @@ int amr_adj_create(int n, double dx, double gamma, const char* flux, amr_adj** o
   }
   *out = a;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_adj_destroy(amr_adj* a) {
+  AMR_TRY
   if (!a) return AMR_OK;
   double* all[] = {a->Fx,  a->Fy,  a->E,    a->Us,   a->Uo,  a->Bxo, a->Byo, a->tape, a->Qb2, a->Usb,
                    a->Fbx, a->Fby, a->Eb,   a->Bsbx, a->Bsby, a->Bbx, a->Bby, a->bb,   a->Sx,  a->Sy};
@@ -601,46 +605,57 @@ int amr_adj_destroy(amr_adj* a) {
     if (p) hipFree(p);
   delete a;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int64_t amr_adj_tape_doubles(const amr_adj* a) { return a ? (int64_t)(kTapePerCell * a->G.n2) : 0; }
 
 int amr_adj_step_tape(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo,
                       double* Bxo, double* Byo, double* tape, void* stream) {
+  AMR_TRY
   if (!a || !U || !Bx || !By || !Uo || !Bxo || !Byo) return AMR_ERR_CONFIG;
   return forward(a, U, Bx, By, dt, Uo, Bxo, Byo, tape_at(tape ? tape : a->tape, a->G.n2), (hipStream_t)stream);
+  AMR_CATCH
 }
 
 int amr_adj_step(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, double* Uo, double* Bxo,
                  double* Byo, void* stream) {
+  AMR_TRY
   return amr_adj_step_tape(a, U, Bx, By, dt, Uo, Bxo, Byo, nullptr, stream);
+  AMR_CATCH
 }
 
 int amr_adj_vjp_tape(amr_adj* a, const double* tape, const double* Bx, const double* By, double dt,
                      const double* Uob, const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb,
                      void* stream) {
+  AMR_TRY
   if (!a || !tape || !Bx || !By || !Uob || !Bxob || !Byob || !Ub || !Bxb || !Byb) return AMR_ERR_CONFIG;
   return reverse(a, tape_at(const_cast<double*>(tape), a->G.n2), Bx, By, dt, Uob, Bxob, Byob, Ub, Bxb, Byb,
                  (hipStream_t)stream);
+  AMR_CATCH
 }
 
 int amr_adj_step_vjp(amr_adj* a, const double* U, const double* Bx, const double* By, double dt, const double* Uob,
                      const double* Bxob, const double* Byob, double* Ub, double* Bxb, double* Byb, void* stream) {
+  AMR_TRY
   if (!a || !U || !Bx || !By || !Uob || !Bxob || !Byob || !Ub || !Bxb || !Byb) return AMR_ERR_CONFIG;
   hipStream_t s = (hipStream_t)stream;
   const Tape T = tape_at(a->tape, a->G.n2);
   // recompute the step into the context's own tape
   if (int e = forward(a, U, Bx, By, dt, a->Uo, a->Bxo, a->Byo, T, s)) return e;
   return reverse(a, T, Bx, By, dt, Uob, Bxob, Byob, Ub, Bxb, Byb, s);
+  AMR_CATCH
 }
 
 // test hook: 1 = fused tiled face reverse on 16x13 tiles (default), 2 = on
 // 32x13 tiles, 0 = per-face kernels and a separate gather (same derivative,
 // different summation order)
 int amr_debug_adj_set_fused(amr_adj* a, int mode) {
+  AMR_TRY
   if (!a || mode < 0 || mode > 2) return AMR_ERR_CONFIG;
   a->fused = mode;
   return AMR_OK;
+  AMR_CATCH
 }
 
 // host entry points of the pointwise reverse pieces (CPU tests)

@@ -19,6 +19,7 @@
 #include <functional>
 #include <thread>
 #include <atomic>
+#include <exception>
 #include <condition_variable>
 #include <memory>
 #include <mutex>
@@ -26,6 +27,7 @@
 #include <vector>
 
 #include "../../include/amr2d_mhd.h"
+#include "abi_guard.hpp"
 #include "amr_types.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
@@ -172,9 +174,18 @@ class RangePool {
     }
     cv_.notify_all();
     work(g);
-    if (spin([this] { return pending_.load(std::memory_order_acquire) == 0; })) return;
-    std::unique_lock<std::mutex> lk(m_);
-    done_.wait(lk, [this] { return pending_.load(std::memory_order_acquire) == 0; });
+    if (!spin([this] { return pending_.load(std::memory_order_acquire) == 0; })) {
+      std::unique_lock<std::mutex> lk(m_);
+      done_.wait(lk, [this] { return pending_.load(std::memory_order_acquire) == 0; });
+    }
+    // an exception of any range (host allocation) surfaces in the caller
+    std::exception_ptr e;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      e = error_;
+      error_ = nullptr;
+    }
+    if (e) std::rethrow_exception(e);
   }
 
  private:
@@ -185,7 +196,12 @@ class RangePool {
       const int R = (int)((v >> 8) & 0xff), r = (int)(v & 0xff);
       if (r >= R) return;
       if (!state_.compare_exchange_weak(v, v + 1, std::memory_order_acq_rel, std::memory_order_acquire)) continue;
-      (*job_)(r);
+      try {
+        (*job_)(r);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(m_);
+        if (!error_) error_ = std::current_exception();
+      }
       if (pending_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
         std::lock_guard<std::mutex> lk(m_);
         done_.notify_all();
@@ -224,6 +240,7 @@ class RangePool {
   std::atomic<uint64_t> gen_{0};
   std::atomic<bool> stop_{false};
   const std::function<void(int)>* job_ = nullptr;
+  std::exception_ptr error_;  // first exception of the current call
 };
 
 // Symbolic fill state of one contiguous range of the schedule.  A compile
@@ -341,21 +358,23 @@ struct amr_ctx {
   int64_t cap_slots = 0, used_slots = 0;
   hipStream_t stream = nullptr;
   // device work lists and buffers (capacities in elements)
-  LeafItem* d_leaves = nullptr;
-  int nleaf = 0, cap_leaves = 0;
+  // topology block, rebuilt at each tree change: the fill-program segment
+  // tables and the leaf work list in one device allocation, filled by one
+  // upload from a pinned staging copy with the same layout
+  char* d_topo = nullptr;
+  char* h_topo = nullptr;
+  size_t cap_topo = 0;
+  LeafItem* d_leaves = nullptr;  // view into d_topo
+  int nleaf = 0;
   std::vector<LeafItem> h_leaves;
   std::vector<int> leaf_inst_begin;  // per active instance, first leaf index
   // host copies kept alive until the stream has consumed them
   std::vector<RegridOp> h_refs, h_coars;
   std::vector<FillScratch> scratch;  // one per compile thread
   // fill program tables: per-instance segment pointers and chunk-count prefix
-  const FillOp** d_inst_ops = nullptr;
+  const FillOp** d_inst_ops = nullptr;  // views into d_topo
   const FillGen** d_inst_gen = nullptr;
   int32_t* d_prefix = nullptr;
-  int cap_tables = 0;
-  PinnedVec<const FillOp*> h_inst_ops;
-  PinnedVec<const FillGen*> h_inst_gen;
-  PinnedVec<int32_t> h_prefix;
   int nchunks = 0;
   InstParams* d_ip = nullptr;
   PinnedVec<InstParams> h_ip;  // pinned: the step graph copies from it
@@ -1727,10 +1746,6 @@ int rebuild_topology(amr_ctx* c) {
     c->stage_spec = hlld_muscl ? kStageHLLDMuscl : kStageGeneric;
     c->tune = c->tune_by_spec[c->stage_spec];
   }
-  if (int e = ensure(c->d_leaves, c->cap_leaves, std::max(1, c->nleaf))) return e;
-  if (c->nleaf)
-    HIPCHK(hipMemcpyAsync(c->d_leaves, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
-                          c->stream));
   // programs of instances whose tree changed: the current one if the tree is
   // the same, a cached one if the tree recurs, else compiled (in parallel)
   {
@@ -1789,32 +1804,46 @@ int rebuild_topology(amr_ctx* c) {
     if (ng) HIPCHK(hipMemcpyAsync(I.d_gen, I.gen.data(), sizeof(FillGen) * ng, hipMemcpyHostToDevice, c->stream));
     I.prog_dirty = false;
   }
-  // segment tables over the running instances
+  // segment tables over the running instances and the leaf work list: one
+  // upload of [program pointers | explicit-operand pointers | chunk prefix |
+  // leaves] from the pinned staging copy
   const int ni = (int)c->inst.size();
-  c->h_inst_ops.clear();
-  c->h_inst_gen.clear();
-  c->h_prefix.clear();
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t off_gen = align(sizeof(FillOp*) * (ni + 1));
+  const size_t off_pre = off_gen + align(sizeof(FillGen*) * (ni + 1));
+  const size_t off_leaf = off_pre + align(sizeof(int32_t) * (ni + 1));
+  const size_t topo_bytes = off_leaf + sizeof(LeafItem) * std::max(1, c->nleaf);
+  if (topo_bytes > c->cap_topo) {
+    const size_t cap = topo_bytes + topo_bytes / 2;
+    hipFree(c->d_topo);
+    if (c->h_topo) hipHostFree(c->h_topo);
+    c->d_topo = nullptr;
+    c->h_topo = nullptr;
+    c->cap_topo = 0;
+    HIPCHK(hipMalloc((void**)&c->d_topo, cap));
+    HIPCHK(hipHostMalloc((void**)&c->h_topo, cap));
+    c->cap_topo = cap;
+  }
+  const FillOp** h_ops = (const FillOp**)c->h_topo;
+  const FillGen** h_gen = (const FillGen**)(c->h_topo + off_gen);
+  int32_t* h_pre = (int32_t*)(c->h_topo + off_pre);
   int total = 0;
-  c->h_prefix.push_back(0);
-  for (auto& I : c->inst) {
-    c->h_inst_ops.push_back(I.d_ops);
-    c->h_inst_gen.push_back(I.d_gen);
+  h_pre[0] = 0;
+  for (int q = 0; q < ni; q++) {
+    const Instance& I = c->inst[q];
+    h_ops[q] = I.d_ops;
+    h_gen[q] = I.d_gen;
     total += I.done ? 0 : I.nops / kFillChunk;
-    c->h_prefix.push_back(total);
+    h_pre[q + 1] = total;
   }
   c->nchunks = total;
-  if (c->cap_tables < ni + 1) {
-    hipFree(c->d_inst_ops);
-    hipFree(c->d_inst_gen);
-    hipFree(c->d_prefix);
-    HIPCHK(hipMalloc((void**)&c->d_inst_ops, sizeof(FillOp*) * (ni + 1)));
-    HIPCHK(hipMalloc((void**)&c->d_inst_gen, sizeof(FillGen*) * (ni + 1)));
-    HIPCHK(hipMalloc((void**)&c->d_prefix, sizeof(int32_t) * (ni + 1)));
-    c->cap_tables = ni + 1;
-  }
-  HIPCHK(hipMemcpyAsync(c->d_inst_ops, c->h_inst_ops.data(), sizeof(FillOp*) * ni, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_inst_gen, c->h_inst_gen.data(), sizeof(FillGen*) * ni, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_prefix, c->h_prefix.data(), sizeof(int32_t) * (ni + 1), hipMemcpyHostToDevice, c->stream));
+  if (c->nleaf) std::memcpy(c->h_topo + off_leaf, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf);
+  HIPCHK(hipMemcpyAsync(c->d_topo, c->h_topo, off_leaf + sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
+                        c->stream));
+  c->d_inst_ops = (const FillOp**)c->d_topo;
+  c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
+  c->d_prefix = (int32_t*)(c->d_topo + off_pre);
+  c->d_leaves = (LeafItem*)(c->d_topo + off_leaf);
   const int old_cap = c->cap_red;
   if (int e = ensure(c->d_red, c->cap_red, 2 * std::max(1, c->nleaf))) return e;
   if (c->cap_red != old_cap || !c->h_red) {
@@ -2220,6 +2249,7 @@ int check_inst(amr_ctx* c, int inst) { return (c && inst >= 0 && inst < (int)c->
 extern "C" {
 
 int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, amr_ctx** out) {
+  AMR_TRY
   if (!out || n_inst <= 0 || !cfgs) return AMR_ERR_CONFIG;
   *out = nullptr;
   amr_ctx* c = new amr_ctx();
@@ -2302,15 +2332,17 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
   }
   *out = c;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_destroy(amr_ctx* c) {
+  AMR_TRY
   if (!c) return AMR_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < kNumFields; f++)
     if (c->F.f[f]) hipFree(c->F.f[f]);
-  hipFree(c->d_leaves);
-  hipFree(c->d_inst_ops); hipFree(c->d_inst_gen); hipFree(c->d_prefix);
+  hipFree(c->d_topo);
+  if (c->h_topo) hipHostFree(c->h_topo);
   for (auto& I : c->inst) {
     hipFree(I.d_ops);
     hipFree(I.d_gen);
@@ -2335,10 +2367,12 @@ int amr_destroy(amr_ctx* c) {
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return AMR_OK;
+  AMR_CATCH
 }
 
 // basegrid.cpp:4-89 + IC2Dtype.cpp + main.cpp:97-110
 int amr_init(amr_ctx* c) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   static const int32_t root_lp[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
@@ -2383,14 +2417,18 @@ int amr_init(amr_ctx* c) {
   c->topo_dirty = true;
   if (int e = sync_errors(c)) return e;
   return running(c) ? AMR_OK : c->call_fail;
+  AMR_CATCH
 }
 
 int amr_step(amr_ctx* c, int* n_active) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   return step_all(c, n_active);
+  AMR_CATCH
 }
 
 int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   int64_t n = 0, zu = 0;
   int active = running(c);
@@ -2410,19 +2448,25 @@ int amr_run(amr_ctx* c, int max_steps, int64_t* steps, int64_t* zone_updates) {
   if (steps) *steps = n;
   if (zone_updates) *zone_updates = zu;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_step_begin(amr_ctx* c) {
+  AMR_TRY
   if (!c || c->in_flight) return AMR_ERR_CONFIG;
   return step_begin(c);
+  AMR_CATCH
 }
 
 int amr_step_end(amr_ctx* c, int* n_active) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   return step_end(c, n_active);
+  AMR_CATCH
 }
 
 int amr_run_group(int n_ctx, amr_ctx** ctxs, int max_steps, int64_t* steps, int64_t* zone_updates) {
+  AMR_TRY
   if (n_ctx <= 0 || !ctxs) return AMR_ERR_CONFIG;
   for (int k = 0; k < n_ctx; k++)
     if (!ctxs[k] || ctxs[k]->in_flight) return AMR_ERR_CONFIG;
@@ -2466,9 +2510,11 @@ int amr_run_group(int n_ctx, amr_ctx** ctxs, int max_steps, int64_t* steps, int6
   if (steps) *steps = n;
   if (zone_updates) *zone_updates = zu;
   return err;
+  AMR_CATCH
 }
 
 int amr_info(amr_ctx* c, int inst, amr_inst_info* o) {
+  AMR_TRY
   if (check_inst(c, inst) || !o) return AMR_ERR_CONFIG;
   const Instance& I = c->inst[inst];
   int nl = 0;
@@ -2476,15 +2522,19 @@ int amr_info(amr_ctx* c, int inst, amr_inst_info* o) {
   *o = amr_inst_info{I.t, I.dt_dom, I.dt_main, I.gamma, I.tEnd, I.lastActive, nl, I.count, I.done ? 1 : 0,
                      I.status, I.maxblocks};
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_last_error(amr_ctx* c, amr_diag* d) {
+  AMR_TRY
   if (!c || !d) return AMR_ERR_CONFIG;
   *d = c->diag;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_download(amr_ctx* c, int inst, int field, int nslots, double* host) {
+  AMR_TRY
   if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
@@ -2495,9 +2545,11 @@ int amr_download(amr_ctx* c, int inst, int field, int nslots, double* host) {
     HIPCHK(hipMemcpy(host + per * nb, c->F.f[field] + per * gs, per * sizeof(double), hipMemcpyDeviceToHost));
   }
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) {
+  AMR_TRY
   if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
@@ -2508,9 +2560,11 @@ int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) 
     HIPCHK(hipMemcpy(c->F.f[field] + per * gs, host + per * nb, per * sizeof(double), hipMemcpyHostToDevice));
   }
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_get_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* ic, int32_t* lf) {
+  AMR_TRY
   if (check_inst(c, inst)) return AMR_ERR_CONFIG;
   const Instance& I = c->inst[inst];
   if (n > (int)I.lp.size()) return AMR_ERR_CONFIG;
@@ -2521,9 +2575,11 @@ int amr_get_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* ic, int32_t*
     lf[nb] = I.leafs[nb];
   }
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* ic, const int32_t* lf) {
+  AMR_TRY
   if (check_inst(c, inst) || n > c->inst[inst].maxblocks) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   grow_tree(I, n);
@@ -2538,9 +2594,11 @@ int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* 
   I.prog_dirty = true;
   c->topo_dirty = true;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int32_t count) {
+  AMR_TRY
   if (check_inst(c, inst)) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   I.t = t;
@@ -2548,18 +2606,22 @@ int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int3
   I.dt_main = dt_next;
   I.count = count;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_fill(amr_ctx* c, int field, int kind) {
+  AMR_TRY
   if (!c || field < 0 || field >= kNumFields || kind < 0 || kind > 2) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   run_fill(c, make_set(c, {{field, kind}}));
   return finish_call(c);
+  AMR_CATCH
 }
 
 int amr_stage(amr_ctx* c, int step) {
+  AMR_TRY
   if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -2567,9 +2629,11 @@ int amr_stage(amr_ctx* c, int step) {
   if (int e = upload_params(c)) return e;
   run_stage(c, step, false);
   return finish_call(c);
+  AMR_CATCH
 }
 
 int amr_ct(amr_ctx* c, int step) {
+  AMR_TRY
   if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -2577,9 +2641,11 @@ int amr_ct(amr_ctx* c, int step) {
   if (int e = upload_params(c)) return e;
   run_ct(c, step);
   return finish_call(c);
+  AMR_CATCH
 }
 
 int amr_cfl(amr_ctx* c, int inst, double* lambda) {
+  AMR_TRY
   if (check_inst(c, inst) || !lambda) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -2591,9 +2657,11 @@ int amr_cfl(amr_ctx* c, int inst, double* lambda) {
   for (int l = c->leaf_inst_begin[inst]; l < c->leaf_inst_begin[inst + 1]; l++) lam = std_min(lam, c->h_lam[l]);
   *lambda = lam;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_regrid(amr_ctx* c) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -2602,9 +2670,11 @@ int amr_regrid(amr_ctx* c) {
   if (int e = cfl_and_indicators(c, nullptr)) return e;
   if (int e = regrid_all(c)) return e;
   return finish_call(c);
+  AMR_CATCH
 }
 
 int amr_schedule(amr_ctx* c, int inst, int cap, int32_t* ib3, int* n) {
+  AMR_TRY
   if (check_inst(c, inst) || !n) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   build_schedule(I);
@@ -2612,33 +2682,41 @@ int amr_schedule(amr_ctx* c, int inst, int cap, int32_t* ib3, int* n) {
   for (int e = 0; e < *n && e < cap; e++)
     for (int q = 0; q < 3; q++) ib3[3 * e + q] = I.sched[e][q];
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_riemann_batch(int solver, int dir, double gamma, int n, const double* wL, const double* wR, double* flux,
                       int32_t* status, void* stream) {
+  AMR_TRY
   if (solver < 0 || solver >= kFluxCount || (dir != 0 && dir != 1) || n < 0) return AMR_ERR_CONFIG;
   launch_riemann_batch(solver, dir, gamma, n, wL, wR, flux, status, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_recon_batch(int NT, int nghosts, const char* limiter, double dx, int nt, const double* W, double* out,
                     void* stream) {
+  AMR_TRY
   const int lim = limiter ? lim_code(limiter) : -1;
   if (lim < 0 || NT < 2 * nghosts + 4) return AMR_ERR_CONFIG;
   launch_recon_batch(NT, nghosts, lim, dx, nt, W, out, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_set_stage_variant(amr_ctx* c, int variant) {
+  AMR_TRY
   if (!c || variant < -1 || variant >= kStageVariants) return AMR_ERR_CONFIG;
   c->stage_variant = (variant < 0 && c->wall_khz <= 0) ? 0 : variant;
   c->graph_dirty = true;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_leaf) {
+  AMR_TRY
   if (!c || nbuckets < 0) return AMR_ERR_CONFIG;
   for (int b = 0; b < nbuckets && b < 24; b++) {
     if (chosen) chosen[b] = c->tune[b].chosen;
@@ -2647,16 +2725,20 @@ int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_l
           c->tune[b].n[v] ? c->tune[b].us_per_leaf[v] / c->tune[b].n[v] : 0.0;
   }
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_set_graph(amr_ctx* c, int on) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   c->graph_on = on != 0;
   c->graph_dirty = true;
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_set_fill_cache(amr_ctx* c, int entries) {
+  AMR_TRY
   if (!c || entries < 0 || entries > 64) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->prog_cache_max = entries;
@@ -2668,24 +2750,29 @@ int amr_set_fill_cache(amr_ctx* c, int entries) {
     I.prog_cache.clear();
   }
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_profile_enable(amr_ctx* c, int mask) {
+  AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
   flush_profile(c);
   c->prof_mask = (unsigned)mask;
   for (auto& p : c->prof) p = Profile{};
   return AMR_OK;
+  AMR_CATCH
 }
 
 int amr_profile_read(amr_ctx* c, int family, double* ms, int64_t* launches) {
+  AMR_TRY
   if (!c || family < 0 || family >= PF_COUNT) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
   flush_profile(c);
   if (ms) *ms = c->prof[family].ms;
   if (launches) *launches = c->prof[family].launches;
   return AMR_OK;
+  AMR_CATCH
 }
 
 }  // extern "C"
@@ -2733,6 +2820,7 @@ extern "C" {
 // restriction through the explicit-operand path.
 int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
                         double* Q, int nv, int kind, int* nnodes, int force_general) {
+  AMR_TRY
   amr_ctx c;
   c.NT = NT;
   c.ng = ng;
@@ -2788,6 +2876,7 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
   int live = 0;
   for (size_t e = 0; e < m; e++) live += ops[e].op != kFillNop;
   return live;
+  AMR_CATCH
 }
 
 // Compiled fill program of the given tree: up to `cap` ops copied to `ops`
@@ -2795,6 +2884,7 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
 // the op count (negative on error).
 int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs, int cap,
                       void* ops, int* ngen) {
+  AMR_TRY
   Instance I;
   I.lastActive = n;
   grow_tree(I, n);
@@ -2811,6 +2901,7 @@ int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* i
   if (ops) std::memcpy(ops, I.ops.data(), sizeof(FillOp) * std::min(m, cap));
   if (ngen) *ngen = (int)I.gen.size();
   return m;
+  AMR_CATCH
 }
 
 // Fill program of the given tree compiled with its schedule cut into
@@ -2823,6 +2914,7 @@ int amr_debug_compile(int NT, int ng, int n, const int32_t* lp, const int32_t* i
 int amr_debug_compile_ranges(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
                              int ranges, int nthreads, int reuse, int cap, void* ops, int gcap, void* gen,
                              int* ngen) {
+  AMR_TRY
   static FillScratch kept;
   Instance I;
   I.lastActive = n;
@@ -2843,6 +2935,7 @@ int amr_debug_compile_ranges(int NT, int ng, int n, const int32_t* lp, const int
   if (gen) std::memcpy(gen, I.gen.data(), sizeof(FillGen) * std::min<size_t>(I.gen.size(), (size_t)gcap));
   if (ngen) *ngen = (int)I.gen.size();
   return m;
+  AMR_CATCH
 }
 
 // Host cost of the schedule + fill-program compile for the given tree:
@@ -2850,6 +2943,7 @@ int amr_debug_compile_ranges(int NT, int ng, int n, const int32_t* lp, const int
 // reuse one scratch (as the engine does).
 double amr_debug_compile_ns(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
                             int reps, int nthreads) {
+  AMR_TRY
   Instance I;
   I.lastActive = n;
   grow_tree(I, n);
@@ -2868,19 +2962,23 @@ double amr_debug_compile_ns(int NT, int ng, int n, const int32_t* lp, const int3
     compile_instance(I, NT, ng, S, nthreads);
   }
   return std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / std::max(1, reps);
+  AMR_CATCH_NEG
 }
 
 // coarsen source maps (see coarsen_maps): mU[NT*NT], mB[2*NT*NT]
 int amr_debug_coarsen_maps(int NT, int ng, int32_t* mU, int32_t* mB) {
+  AMR_TRY
   std::vector<int32_t> u, b;
   coarsen_maps(NT, ng, u, b);
   std::memcpy(mU, u.data(), u.size() * sizeof(int32_t));
   std::memcpy(mB, b.data(), b.size() * sizeof(int32_t));
   return 0;
+  AMR_CATCH
 }
 
 // the locate_bounds schedule of a tree (no device work)
 int amr_debug_schedule(int n, const int32_t* lp, const int32_t* leafs, int cap, int32_t* ib3) {
+  AMR_TRY
   Instance I;
   I.lastActive = n;
   grow_tree(I, n);
@@ -2892,6 +2990,7 @@ int amr_debug_schedule(int n, const int32_t* lp, const int32_t* leafs, int cap, 
   for (int e = 0; e < (int)I.sched.size() && e < cap; e++)
     for (int q = 0; q < 3; q++) ib3[3 * e + q] = I.sched[e][q];
   return (int)I.sched.size();
+  AMR_CATCH
 }
 
 }  // extern "C"

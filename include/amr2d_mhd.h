@@ -30,7 +30,9 @@ enum {
   AMR_ERR_REFINE_NAN = 9, /* refineBi_CT.cpp:47 */
   AMR_ERR_CONFIG = 10,
   AMR_ERR_HIP = 11,
-  AMR_ERR_CAPACITY = 12   /* slot arena of the context exhausted */
+  AMR_ERR_CAPACITY = 12,  /* slot arena of the context exhausted */
+  AMR_ERR_HOST = 13       /* host-side failure (out of host memory): no exception
+                             crosses this interface; destroy the context */
 };
 
 /* slot fields (meshblock arrays, defs.hpp:69-83); layout [slot][var][j][i] */
