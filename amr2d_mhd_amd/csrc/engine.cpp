@@ -414,7 +414,11 @@ struct amr_ctx {
   // four pointers are views set at each topology rebuild
   double *d_red = nullptr, *h_red = nullptr;
   double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
-  int cap_red = 0;
+  // d_err/h_err records first, then the reduction arrays, in one device and
+  // one pinned host block sized for every slot: one download per step brings
+  // the errors and the per-leaf values
+  char *d_rec = nullptr, *h_rec = nullptr;
+  size_t rec_err_bytes = 0;
   DevError* d_err = nullptr;  // one record per instance
   DevError* h_err = nullptr;
   int call_fail = 0;          // code of the first instance that failed in the current call
@@ -1844,13 +1848,6 @@ int rebuild_topology(amr_ctx* c) {
   c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
   c->d_prefix = (int32_t*)(c->d_topo + off_pre);
   c->d_leaves = (LeafItem*)(c->d_topo + off_leaf);
-  const int old_cap = c->cap_red;
-  if (int e = ensure(c->d_red, c->cap_red, 2 * std::max(1, c->nleaf))) return e;
-  if (c->cap_red != old_cap || !c->h_red) {
-    // the pinned host copy follows the device capacity
-    if (c->h_red) hipHostFree(c->h_red);
-    HIPCHK(hipHostMalloc((void**)&c->h_red, sizeof(double) * c->cap_red));
-  }
   c->d_lam = c->d_red;
   c->d_grad = c->d_red + c->nleaf;
   c->h_lam = c->h_red;
@@ -2029,9 +2026,9 @@ int cfl_and_indicators(amr_ctx* c, const double* dW) {
     launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
                         c->stream);
   }
-  if (c->nleaf)
-    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * 2 * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-  return sync_errors(c);
+  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
+                        hipMemcpyDeviceToHost, c->stream));
+  return wait_errors(c);
 }
 
 // admesh for every running instance from the downloaded indicators
@@ -2072,9 +2069,20 @@ int regrid_all(amr_ctx* c) {
 int enqueue_step(amr_ctx* c) {
   const bool trial = tuning(c);
   if (trial) HIPCHK(hipMemsetAsync(c->d_span, 0, 4 * sizeof(unsigned long long), c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->inst.size(), hipMemcpyHostToDevice,
-                        c->stream));
-  run_fill(c, make_set(c, {{kU, kFillCell}, {kBi, kFillFace}}));
+  {
+    // the step's parameters (dt) ride on the first fill launch
+    FillSet s0 = make_set(c, {{kU, kFillCell}, {kBi, kFillFace}});
+    static_assert(sizeof(InstParams) % sizeof(double) == 0, "parameters copied as 8-byte words");
+    if (c->nchunks > 0) {
+      s0.copy_words = (int32_t)(sizeof(InstParams) * c->inst.size() / sizeof(double));
+      s0.copy_src = (const double*)c->h_ip.data();
+      s0.copy_dst = (double*)c->d_ip;
+    } else {
+      HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->inst.size(), hipMemcpyHostToDevice,
+                            c->stream));
+    }
+    run_fill(c, s0);
+  }
   run_stage(c, 1);
   run_fill(c, make_set(c, {{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}}));
   run_ct(c, 1);
@@ -2094,9 +2102,9 @@ int enqueue_step(amr_ctx* c) {
     launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
                         c->stream);
   }
-  if (c->nleaf)
-    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * 2 * c->nleaf, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
+  // error records and per-leaf values in one download
+  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
+                        hipMemcpyDeviceToHost, c->stream));
   if (trial)
     HIPCHK(hipMemcpyAsync(c->h_span, c->d_span, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   return 0;
@@ -2304,17 +2312,23 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
       return AMR_ERR_HIP;
     }
   }
+  c->rec_err_bytes = (sizeof(DevError) * n_inst + 255) & ~size_t(255);
+  const size_t rec_bytes = c->rec_err_bytes + sizeof(double) * 2 * (size_t)std::max<int64_t>(1, c->cap_slots);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
-      hipMalloc((void**)&c->d_err, sizeof(DevError) * n_inst) != hipSuccess ||
-      hipMemset(c->d_err, 0, sizeof(DevError) * n_inst) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_err, sizeof(DevError) * n_inst) != hipSuccess ||
+      hipMalloc((void**)&c->d_rec, rec_bytes) != hipSuccess ||
+      hipMemset(c->d_rec, 0, rec_bytes) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_rec, rec_bytes) != hipSuccess ||
       hipMalloc((void**)&c->d_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     amr_destroy(c);
     return AMR_ERR_HIP;
   }
+  c->d_err = (DevError*)c->d_rec;
+  c->h_err = (DevError*)c->h_rec;
+  c->d_red = (double*)(c->d_rec + c->rec_err_bytes);
+  c->h_red = (double*)(c->h_rec + c->rec_err_bytes);
   {
     int dev = 0, khz = 0;
     if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
@@ -2351,10 +2365,9 @@ int amr_destroy(amr_ctx* c) {
       hipFree(P.d_gen);
     }
   }
-  hipFree(c->d_ip); hipFree(c->d_red); hipFree(c->d_err); hipFree(c->d_rops);
+  hipFree(c->d_ip); hipFree(c->d_rec); hipFree(c->d_rops);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
-  if (c->h_red) hipHostFree(c->h_red);
-  if (c->h_err) hipHostFree(c->h_err);
+  if (c->h_rec) hipHostFree(c->h_rec);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (auto& e : c->ev_pool) {

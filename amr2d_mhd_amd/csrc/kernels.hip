@@ -372,6 +372,8 @@ __device__ inline void fill_field(const FillCtx& f, double* base, const FillOp& 
 }
 
 __global__ __launch_bounds__(kThreads) void fill_kernel(FillProgram P, FillSet set, int NT) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < set.copy_words; i += gridDim.x * kThreads)
+    set.copy_dst[i] = set.copy_src[i];
   const int w = blockIdx.x;
   const int q = find_segment(P, w);
   const FillOp op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + threadIdx.x];

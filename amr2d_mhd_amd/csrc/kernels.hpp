@@ -16,6 +16,12 @@ struct FillSet {
   int32_t nv[4];
   int32_t kind[4];
   int32_t n;
+  // optional: copy_words 8-byte words copy_src -> copy_dst done by the same
+  // launch (the step's parameter upload from pinned host memory, so that it
+  // costs no copy dispatch of its own); no fill op reads copy_dst
+  int32_t copy_words;
+  const double* copy_src;
+  double* copy_dst;
 };
 
 // refine (4 sons from a parent) or coarsen (parent from 4 sons) operation
