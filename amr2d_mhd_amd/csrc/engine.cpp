@@ -401,7 +401,7 @@ struct amr_ctx {
     int n[kStageVariants] = {0};
     int chosen = -1;
   };
-  TuneBucket tune_by_spec[2][24];  // per stage-kernel specialisation
+  TuneBucket tune_by_spec[kStageSpecs][24];  // per stage-kernel specialisation
   int stage_spec = kStageGeneric;   // of the running instances (set at each topology rebuild)
   TuneBucket* tune = tune_by_spec[0];
   unsigned long long* d_span = nullptr;  // [stage 1 start, end, stage 2 start, end]
@@ -1742,12 +1742,15 @@ int rebuild_topology(amr_ctx* c) {
   c->leaf_inst_begin[c->inst.size()] = (int)c->h_leaves.size();
   c->nleaf = (int)c->h_leaves.size();
   {
-    // the specialised stage kernel serves launches whose instances all use
-    // HLLD with a MUSCL-family limiter
-    bool hlld_muscl = true;
-    for (const Instance& I : c->inst)
-      if (!I.done && !(I.solver == kFluxHLLD && I.lim < kLimWENO3)) hlld_muscl = false;
-    c->stage_spec = hlld_muscl ? kStageHLLDMuscl : kStageGeneric;
+    // a specialised stage kernel serves launches whose instances all use
+    // HLLD with a MUSCL-family limiter, or all HLLD with WENO5 (24x24 tiles)
+    bool hlld_muscl = true, hlld_weno5 = c->NT == 24 && c->ng == 4;
+    for (const Instance& I : c->inst) {
+      if (I.done) continue;
+      if (!(I.solver == kFluxHLLD && I.lim < kLimWENO3)) hlld_muscl = false;
+      if (!(I.solver == kFluxHLLD && I.lim == kLimWENO5)) hlld_weno5 = false;
+    }
+    c->stage_spec = hlld_muscl ? kStageHLLDMuscl : hlld_weno5 ? kStageHLLDWeno5 : kStageGeneric;
     c->tune = c->tune_by_spec[c->stage_spec];
   }
   // programs of instances whose tree changed: the current one if the tree is

@@ -60,9 +60,10 @@ __device__ inline void stamp_end(unsigned long long* span) { atomicMax(span + 1,
 // Variants (same arithmetic, same results): TPB lanes per leaf, and the face
 // fluxes kept in LDS for the RK update (FLDS) or read back from the global
 // ff/gg the kernel has just written (smaller LDS footprint, more resident
-// workgroups per CU).  SPEC = kStageHLLDMuscl fixes the solver to HLLD and the
-// reconstruction to the MUSCL family at compile time (used when every leaf of
-// the launch has them): the other solvers' and WENO's registers drop out.
+// workgroups per CU).  SPEC = kStageHLLDMuscl (kStageHLLDWeno5) fixes the
+// solver to HLLD and the reconstruction to the MUSCL family (WENO5) at compile
+// time, used when every leaf of the launch has them: the other paths'
+// registers drop out.
 template <int NT, int NG, int TPB, bool FLDS, int WPE, int SPEC = kStageGeneric>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
     FieldPtrs F, const LeafItem* __restrict__ leaves, const InstParams* __restrict__ ip, int step, DevError* err,
@@ -78,8 +79,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const LeafItem L = leaves[blockIdx.x];
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
-  constexpr bool muscl_only = SPEC == kStageHLLDMuscl;
-  const int lim = P.limiter, solver = SPEC == kStageHLLDMuscl ? (int)kFluxHLLD : P.solver;
+  constexpr bool muscl_only = SPEC == kStageHLLDMuscl, weno5_only = SPEC == kStageHLLDWeno5;
+  const int lim = weno5_only ? (int)kLimWENO5 : P.limiter;
+  const int solver = SPEC != kStageGeneric ? (int)kFluxHLLD : P.solver;
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
   const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
@@ -133,10 +135,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const double* Wk = sW + k * N2;
-      if (!muscl_only && lim == kLimWENO5) {
+      if (weno5_only || (!muscl_only && lim == kLimWENO5)) {
         wl[k] = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
         wr[k] = weno5_edge(Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], Wk[c + 3 * s], false);
-      } else if (!muscl_only && lim == kLimWENO3) {
+      } else if (!muscl_only && !weno5_only && lim == kLimWENO3) {
         wl[k] = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
         wr[k] = weno3_edge(Wk[c], Wk[c + s], Wk[c + 2 * s], false);
       } else {
@@ -669,13 +671,25 @@ void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf,
   stage_kernel<T, G, TPB, FLDS, WPE, SPEC><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span);
 }
 
+// WENO5 needs nghosts >= 3: the 24x24 tile only
+#define AMR_DISPATCH_TILE_WENO5(NT, NG, CALL)                         \
+  do {                                                                \
+    if (NT == 24 && NG == 4) { constexpr int T = 24, G = 4; CALL; }   \
+  } while (0)
+
 template <int SPEC>
 void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                        const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
                        hipEvent_t ev_stop, unsigned long long* span) {
-#define AMR_STAGE(TPB, FLDS, WPE)                                                                     \
-  AMR_DISPATCH_TILE(NT, ng, (launch_stage_variant<T, G, TPB, FLDS, WPE, SPEC>(F, leaves, nleaf, ip, step, err, \
-                                                                              s, ev_start, ev_stop, span)))
+#define AMR_STAGE_CALL (launch_stage_variant<T, G, TPB, FLDS, WPE, SPEC>(F, leaves, nleaf, ip, step, err, s, \
+                                                                         ev_start, ev_stop, span))
+#define AMR_STAGE(TPB_, FLDS_, WPE_)                                           \
+  do {                                                                         \
+    constexpr int TPB = TPB_, WPE = WPE_;                                      \
+    constexpr bool FLDS = FLDS_;                                               \
+    if constexpr (SPEC == kStageHLLDWeno5) AMR_DISPATCH_TILE_WENO5(NT, ng, AMR_STAGE_CALL); \
+    else AMR_DISPATCH_TILE(NT, ng, AMR_STAGE_CALL);                            \
+  } while (0)
   switch (variant) {
     case 1: AMR_STAGE(512, true, 1); break;
     case 2: AMR_STAGE(768, true, 1); break;
@@ -685,6 +699,7 @@ void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const Le
     default: AMR_STAGE(256, true, 1); break;
   }
 #undef AMR_STAGE
+#undef AMR_STAGE_CALL
 }
 
 void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
@@ -693,6 +708,8 @@ void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, con
   if (nleaf <= 0) return;
   if (spec == kStageHLLDMuscl)
     launch_stage_spec<kStageHLLDMuscl>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
+  else if (spec == kStageHLLDWeno5 && NT == 24 && ng == 4)
+    launch_stage_spec<kStageHLLDWeno5>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
   else
     launch_stage_spec<kStageGeneric>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
 }

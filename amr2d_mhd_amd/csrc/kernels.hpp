@@ -45,8 +45,9 @@ bool tile_supported(int NT, int ng);
 // launch's device wall-clock span (see stamp_start/stamp_end).
 constexpr int kStageVariants = 6;
 // stage-kernel specialisations: generic (solver and limiter read per leaf), or
-// HLLD with a MUSCL-family limiter fixed at compile time
-enum { kStageGeneric = 0, kStageHLLDMuscl = 1 };
+// HLLD with a MUSCL-family limiter, or HLLD with WENO5 (24x24 tiles, the only
+// ones with the ghost depth WENO5 needs), fixed at compile time
+enum { kStageGeneric = 0, kStageHLLDMuscl = 1, kStageHLLDWeno5 = 2, kStageSpecs = 3 };
 void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
                   hipEvent_t ev_stop = nullptr, unsigned long long* span = nullptr);

@@ -165,8 +165,9 @@ int amr_set_graph(amr_ctx *ctx, int on);
  * 2: 768 lanes, LDS; 3: 256 lanes, fluxes read back from ff/gg, 4 waves per
  * SIMD; 4: 512 lanes, read back, 4 waves per SIMD; 5: 512 lanes, LDS, 4 waves
  * per SIMD.  Each variant exists generic (solver and limiter read per
- * instance) and specialised to HLLD with a MUSCL-family limiter, which
- * launches use when every running instance has that pair.  -1 (the default)
+ * instance), specialised to HLLD with a MUSCL-family limiter, and, for
+ * nghosts 4, specialised to HLLD with WENO5; launches use a specialisation
+ * when every running instance has its pair.  -1 (the default)
  * tunes per specialisation and leaf-count bucket (bucket b holds
  * 2^b <= nleaf < 2^(b+1)): the first stage launches in a bucket try every
  * variant three times, timed by the kernel's device-clock span, and the one
