@@ -106,6 +106,7 @@ _SIGS = {
     "amr_debug_coarsen_maps": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_debug_compile_ns": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_int]),
+    "amr_debug_pool_throw": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "amr_debug_compile_ranges": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                            C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_debug_compile": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,

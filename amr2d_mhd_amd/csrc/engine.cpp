@@ -2954,6 +2954,21 @@ int amr_debug_compile_ranges(int NT, int ng, int n, const int32_t* lp, const int
   AMR_CATCH
 }
 
+// Exception barrier check: runs `ranges` ranges of the compile's thread pool
+// on up to `nthreads` threads, range `throw_at` raising std::bad_alloc; the
+// entry point must return AMR_ERR_HOST (and 0 when throw_at is out of range).
+int amr_debug_pool_throw(int ranges, int nthreads, int throw_at) {
+  AMR_TRY
+  FillScratch S;
+  std::atomic<int> ran{0};
+  for_ranges(S, std::max(1, std::min(ranges, 255)), nthreads, [&](int r) {
+    ran.fetch_add(1);
+    if (r == throw_at) throw std::bad_alloc();
+  });
+  return ran.load() == std::max(1, std::min(ranges, 255)) ? AMR_OK : AMR_ERR_CONFIG;
+  AMR_CATCH
+}
+
 // Host cost of the schedule + fill-program compile for the given tree:
 // mean nanoseconds over `reps` compiles on up to `nthreads` threads that
 // reuse one scratch (as the engine does).

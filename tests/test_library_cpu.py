@@ -115,6 +115,19 @@ def test_split_compile_equals_single_range(tmp_path, ng, lv, seed, pr, pc, round
     assert len(ref_ops) % 256 == 0
 
 
+@pytest.mark.parametrize("ranges,threads", [(1, 1), (5, 1), (16, 4), (16, 16)])
+def test_no_exception_crosses_the_abi(ranges, threads):
+    """A host allocation failure inside a range of the compile's thread pool
+    (helper thread or caller) comes back as AMR_ERR_HOST, the pool stays
+    usable, and every range runs when nothing throws."""
+    from amr2d_mhd_amd._lib import lib
+
+    L = lib()
+    for throw_at in range(ranges):
+        assert L.amr_debug_pool_throw(ranges, threads, throw_at) == 13
+    assert L.amr_debug_pool_throw(ranges, threads, -1) == 0
+
+
 def test_schedule_matches_oracle():
     """locate_bounds order of the product equals the oracle's on evolving trees."""
     from amr2d_mhd_amd._lib import lib
