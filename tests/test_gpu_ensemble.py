@@ -193,3 +193,42 @@ def test_full_ensemble_runs_and_samples_match():
             code, k = dom.step(), k + 1
         assert (code, k) == (batch.info(q)["status"], at), f"instance {q}: oracle {code} at {k}, gpu " \
                                                              f"{batch.info(q)['status']} at {at}"
+
+
+ROTOR_LV5 = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5, lenx=1.0)
+
+
+def test_slot_arena_exhaustion_raises():
+    """A context whose slot arena is too small for the tree its instance grows
+    stops that instance with AMR_ERR_CAPACITY (12) and, having no instance
+    left running, raises it."""
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(ROTOR_LV5, slot_capacity=24)
+    with pytest.raises(A.AmrError) as ei:
+        sim.init()
+        sim.run(200)
+    assert ei.value.code == 12
+    assert sim.info()["status"] == 12
+    sim.close()
+
+
+def test_slot_arena_exhaustion_is_isolated():
+    """In a batch sharing one arena, the instance that runs out of slots stops
+    with AMR_ERR_CAPACITY; an instance that needs no new slot (one level:
+    never refines) keeps running and equals its alone run bit for bit."""
+    import amr2d_mhd_amd as A
+
+    quiet = dict(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=1)
+    steps = 20  # the one-level Brio-Wu run is still going after 20 steps
+    batch = A.Simulation([ROTOR_LV5, quiet], slot_capacity=28)
+    batch.init()
+    for _ in range(steps):
+        assert batch.step() >= 1
+    assert batch.info(0)["status"] == 12
+    assert batch.info(1)["status"] == 0
+    alone = _alone(quiet, steps)
+    d = compare_states(gpu_state(batch, 1), gpu_state(alone))
+    assert not d, "; ".join(d[:4])
+    batch.close()
+    alone.close()
