@@ -265,3 +265,42 @@ def test_rk3_mirror_boundaries_bitwise():
     assert dom.init() == 0
     sim.init()
     lockstep(sim, dom, 40, full_every=10, label="BWx rk3")
+
+
+def test_rotor_lv5_first_1200_steps_equal_reference_log():
+    """The bench workload against the compiled reference's own log
+    (tests/golden/run_rotor_MC_HLLD_lv5_1200.npy, serial build): the rotor, 5
+    levels, MC + HLLD, CFL 0.5, first 1200 steps (to t = 0.073).  Every step's
+    applied dt, time, slot count and leaf count equal the reference's; the
+    FNV-1a hashes of every slot field (U Us Bi Bis ff gg EMF) and of the
+    quadtree -- the oracle's hash over the state downloaded from the GPU --
+    equal the reference's at the start, every 200 steps and at the end."""
+    import os
+
+    import amr2d_mhd_amd as A
+
+    log = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "run_rotor_MC_HLLD_lv5_1200.npy"))
+    cfg = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
+    sim = A.Simulation(cfg)
+    sim.init()
+    dom = O.Domain(**cfg)  # used for its hash only
+    assert dom.init() == 0
+
+    def hashes():
+        st = sim.state()
+        dom.set_tree(st["lp"], st["icoord"], st["leafs"])
+        for name, f in FIELD_MAP.items():
+            dom.set_field(f, st[name])
+        return dom.hash_fields(), dom.hash_tree()
+
+    assert hashes() == (int(log[0]["hf_post"]), int(log[0]["ht_post"]))
+    n = len(log) - 1
+    assert n == 1200
+    for s in range(n):
+        applied = sim.info()["dt"]
+        sim.step()
+        r, inf = log[s + 1], sim.info()
+        assert (applied, inf["t"], inf["last_active"], inf["nleaf"]) == (r["dt"], r["t"], r["la_post"], r["nleaf_post"]), s
+        if (s + 1) % 200 == 0:
+            assert hashes() == (int(r["hf_post"]), int(r["ht_post"])), f"state hash differs after step {s}"
+    sim.close()
