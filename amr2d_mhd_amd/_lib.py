@@ -16,7 +16,8 @@ SOLVERS = {"RUSA": 0, "HLLE": 1, "HLLC": 2, "HLLD": 3}
 ERRORS = {0: "ok", 1: "non-positive pressure", 2: "NaN velocity", 3: "NaN flux", 4: "CT face mismatch",
           5: "NaN edge state", 6: "HLLC wave-speed failure", 7: "invalid dt", 8: "maxblocks exceeded",
           9: "NaN in refined face field", 10: "bad configuration", 11: "HIP runtime error",
-          12: "slot arena exhausted", 13: "host-side failure (out of host memory)"}
+          12: "slot arena exhausted", 13: "host-side failure (out of host memory)",
+          14: "checkpoint file missing, unwritable, truncated or corrupt"}
 
 
 class SimConfig(C.Structure):
@@ -62,6 +63,9 @@ _SIGS = {
     "amr_get_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_set_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_set_time": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "amr_validate_tree": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "amr_checkpoint_save": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+    "amr_checkpoint_load": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
     "amr_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "amr_stage": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_ct": (C.c_int, [C.c_void_p, C.c_int]),

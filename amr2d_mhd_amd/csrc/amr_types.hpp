@@ -102,9 +102,26 @@ struct FillGen {
   int64_t ref[4];
 };
 
-// device error record (first error wins)
+// Device error record, one per instance.  Each of the reference's throw sites
+// a kernel meets records a 64-bit key that orders it as the reference's
+// sequential execution would meet it; the earliest one is kept (atomicMax of
+// the complement; 0 = no error), so the record does not depend on which lane
+// gets there first.  Key fields, most significant first:
+//   launch ordinal (4 bits: stage step 1-3, CFL 4, refine 5, coarsen 6),
+//   sequence (16: block id, or regrid-op index in the reference's order),
+//   phase (4), i (8), j (8), check (4), reported block id (16), code (4).
 struct DevError {
-  int32_t code, inst, nb, i, j, pad[3];
+  unsigned long long nkey;
+  int32_t pad[6];
 };
+enum { kOrdCfl = 4, kOrdRefine = 5, kOrdCoarsen = 6 };
+
+inline __host__ __device__ unsigned long long error_key(int ord, int seq, int phase, int i, int j, int chk, int nb,
+                                                        int code) {
+  return ((unsigned long long)(ord & 0xf) << 60) | ((unsigned long long)(seq & 0xffff) << 44) |
+         ((unsigned long long)(phase & 0xf) << 40) | ((unsigned long long)(i & 0xff) << 32) |
+         ((unsigned long long)(j & 0xff) << 24) | ((unsigned long long)(chk & 0xf) << 20) |
+         ((unsigned long long)(nb & 0xffff) << 4) | (unsigned long long)(code & 0xf);
+}
 
 }  // namespace amr

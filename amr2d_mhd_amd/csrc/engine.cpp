@@ -38,6 +38,12 @@ namespace {
 
 constexpr int kMaxLev = 16;
 constexpr int kNbRoots = 4;  // defs.hpp:29
+
+// block capacity of a simulation with `maxlevs` levels (defs.hpp:30)
+int maxblocks_of(int maxlevs) {
+  const double p = std::pow(kNbRoots, maxlevs - 1);
+  return kNbRoots > p ? kNbRoots : (int)(std::pow(4, maxlevs) * 2);
+}
 constexpr double kPiRef = 3.14159265359;  // defs.hpp:23
 
 int lim_code(const char* s) {
@@ -53,9 +59,10 @@ int flux_code(const char* s) {
     if (!std::strcmp(s, names[q])) return q;
   return -1;
 }
-enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_COUNT };
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_ROTOR_BS, IC_COUNT };
 int ic_code(const char* s) {
-  static const char* names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p", "OT"};
+  static const char* names[] = {"SST",   "SST1",    "SSTy", "BWx",     "BWx1", "BWy", "BWy1",
+                                "rotor", "rotor_p", "OT",   "rotor_bs"};
   for (int q = 0; q < IC_COUNT; q++)
     if (!std::strcmp(s, names[q])) return q;
   return -1;
@@ -588,19 +595,24 @@ void initial_tile(const Instance& I, int nb, int NT, int ng, std::vector<double>
         w[4] = 5. / (12. * pi);
         w[5] = -B0 * sin(2. * pi * y);
         w[6] = B0 * sin(4. * pi * x);
-      } else if (I.ic == IC_ROTOR) {
+      } else if (I.ic == IC_ROTOR || I.ic == IC_ROTOR_BS) {
+        // IC_ROTOR_BS: the Balsara-Spicer rotor of the rotor workload's text
+        // (p = 1, v0 = 2, Bx = 5/sqrt(4 pi)) in the reference rotor's form;
+        // scaling by v0 = 1 or 2 is exact
+        const bool bs = I.ic == IC_ROTOR_BS;
+        const double v0 = bs ? 2. : 1.;
         const double rad = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
         const double fr = (23. - 200. * rad) / 3.;
-        w[4] = 0.5;
-        w[5] = 2.5 / (sqrt(4 * kPiRef));
+        w[4] = bs ? 1.0 : 0.5;
+        w[5] = bs ? 5.0 / (sqrt(4 * kPiRef)) : 2.5 / (sqrt(4 * kPiRef));
         if (rad <= 0.1) {
           w[0] = 10.;
-          w[1] = -(y - 0.5) / 0.1;
-          w[2] = (x - 0.5) / 0.1;
+          w[1] = -v0 * (y - 0.5) / 0.1;
+          w[2] = v0 * (x - 0.5) / 0.1;
         } else if (rad > 0.1 && rad < 0.115) {
           w[0] = (1 + 9. * fr);
-          w[1] = (-(y - 0.5) * fr / 0.1);
-          w[2] = (x - 0.5) * fr / 0.1;
+          w[1] = (-v0 * (y - 0.5) * fr / 0.1);
+          w[2] = v0 * (x - 0.5) * fr / 0.1;
         } else {
           w[0] = 1.;
         }
@@ -1399,9 +1411,15 @@ void link_sons(Instance& I, int nb, const int s[4]) {
   I.icoarse[nb] = 0;
 }
 
-// updatelpdown.cpp:5-54
-void unlink_sons(Instance& I, int dad, const int s[4]) {
+// updatelpdown.cpp:5-54.  A refined neighbour always has a son group
+// (validate_tree); a neighbour without one means a corrupt tree, reported as
+// AMR_ERR_CONFIG before anything is changed.
+int unlink_sons(Instance& I, int dad, const int s[4]) {
   auto& lp = I.lp;
+  for (int side = 4; side < 8; side++) {
+    const int n = lp[dad][side];
+    if (n != -1 && !I.leafs[n] && lp[n][2] < kNbRoots) return AMR_ERR_CONFIG;
+  }
   I.leafs[dad] = 1;
   for (int q = 0; q < 4; q++) I.leafs[s[q]] = 0;
   // side, first son id offset to reach the adjacent column/row, pointer to fix, step to the second son
@@ -1413,6 +1431,58 @@ void unlink_sons(Instance& I, int dad, const int s[4]) {
   if (n != -1 && !I.leafs[n]) { n = lp[lp[n][2]][7]; lp[n][7] = dad; lp[lp[n][5]][7] = dad; }
   n = lp[dad][7];
   if (n != -1 && !I.leafs[n]) { n = lp[n][2]; lp[n][6] = dad; lp[lp[n][5]][6] = dad; }
+  return 0;
+}
+
+// Structural check of a quadtree handed in through the C ABI (amr_set_tree,
+// checkpoint load), before anything is changed.  The trees the reference
+// builds (basegrid.cpp, updatelpup.cpp, updatelpdown.cpp) satisfy:
+//  * blocks 0..3 are the roots: level 0, father 0, sibling id 0;
+//  * every other block is son lp[nb][3] of the son group of its father,
+//    one level below it (son groups are 4 consecutive ids, allocated once per
+//    father and reused, admesh.cpp:33-51), including stale blocks;
+//  * inside a son group the sibling pointers are those updatelpup.cpp sets;
+//  * neighbour pointers are -1 or a block id, and a refined neighbour of a
+//    leaf has a son group;
+//  * leaf flags are 0/1 and icoord lies inside the finest-level index range.
+// Every index the schedule, the fill compiler and the regrid follow is then
+// in range, and every level indexes the per-level cell sizes.
+int validate_tree(int n, int maxblocks, int maxlevs, int side, const int32_t* lp, const int32_t* ic,
+                  const int32_t* lf) {
+  if (!lp || !ic || !lf || n < kNbRoots || n > maxblocks) return AMR_ERR_CONFIG;
+  auto L = [&](int nb, int s) { return lp[8 * nb + s]; };
+  const int64_t coord_max = (int64_t)side << maxlevs;
+  for (int nb = 0; nb < n; nb++) {
+    if (lf[nb] != 0 && lf[nb] != 1) return AMR_ERR_CONFIG;
+    if (ic[2 * nb] < 0 || ic[2 * nb] >= coord_max || ic[2 * nb + 1] < 0 || ic[2 * nb + 1] >= coord_max)
+      return AMR_ERR_CONFIG;
+    for (int s = 4; s < 8; s++)
+      if (L(nb, s) < -1 || L(nb, s) >= n) return AMR_ERR_CONFIG;
+    const int s0 = L(nb, 2);
+    if (s0 != 0 && (s0 < kNbRoots || s0 > n - 4)) return AMR_ERR_CONFIG;
+    if (nb < kNbRoots) {
+      if (L(nb, 0) != 0 || L(nb, 1) != 0 || L(nb, 3) != 0) return AMR_ERR_CONFIG;
+      continue;
+    }
+    const int f = L(nb, 1), q = L(nb, 3);
+    if (f < 0 || f >= n || q < 0 || q > 3) return AMR_ERR_CONFIG;
+    if (L(f, 2) + q != nb || L(nb, 0) != L(f, 0) + 1 || L(nb, 0) >= maxlevs) return AMR_ERR_CONFIG;
+  }
+  for (int nb = 0; nb < n; nb++) {
+    const int s0 = L(nb, 2);
+    if (s0 != 0) {
+      // the group's internal pointers (updatelpup.cpp:9-16)
+      if (L(s0, 5) != s0 + 1 || L(s0, 7) != s0 + 2 || L(s0 + 1, 4) != s0 || L(s0 + 1, 7) != s0 + 3 ||
+          L(s0 + 2, 5) != s0 + 3 || L(s0 + 2, 6) != s0 || L(s0 + 3, 4) != s0 + 2 || L(s0 + 3, 6) != s0 + 1)
+        return AMR_ERR_CONFIG;
+    }
+    if (!lf[nb]) continue;
+    for (int s = 4; s < 8; s++) {
+      const int m = L(nb, s);
+      if (m != -1 && !lf[m] && L(m, 2) == 0) return AMR_ERR_CONFIG;
+    }
+  }
+  return 0;
 }
 
 void grow_tree(Instance& I, int n) {
@@ -1431,6 +1501,8 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
                   std::vector<RegridOp>& coars) {
   Instance& I = c->inst[ii];
   auto& lp = I.lp;
+  // this instance's ops in the reference's order (error-record ordering)
+  const size_t first_ref = refs.size(), first_coars = coars.size();
   for (int nb = 0; nb < I.lastActive; nb++) { I.iref[nb] = 0; I.icoarse[nb] = 0; }
   for (int nb = 0; nb < I.lastActive; nb++) {
     if (!(I.leafs[nb] && lp[nb][0] >= 0)) continue;
@@ -1470,6 +1542,8 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
       }
       op.inst = ii;
       op.nb = nb;
+      op.son_nb = s[0];
+      op.seq = (int32_t)(refs.size() - first_ref);
       refs.push_back(op);
       // refine.cpp:61-84
       const int x0 = I.icoord[nb][0] * 2, y0 = I.icoord[nb][1] * 2;
@@ -1498,8 +1572,10 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
       for (int q = 0; q < 4; q++) op.son_gs[q] = slot_of(c, I, s[q]);
       op.inst = ii;
       op.nb = dad;
+      op.son_nb = s[0];
+      op.seq = (int32_t)(refs.size() - first_ref + coars.size() - first_coars);
+      if (int e = unlink_sons(I, dad, s)) { c->fail_nb = dad; return e; }
       coars.push_back(op);
-      unlink_sons(I, dad, s);
     }
   return 0;
 }
@@ -1976,9 +2052,12 @@ int wait_errors(amr_ctx* c) {
   bool any = false;
   for (int q = 0; q < ni; q++) {
     const DevError& e = c->h_err[q];
-    if (!e.code) continue;
+    if (!e.nkey) continue;
     any = true;
-    fail_instance(c, q, e.code, e.nb, e.i, e.j);
+    // error_key fields: code, block id, j, i
+    const unsigned long long key = ~e.nkey;
+    fail_instance(c, q, (int)(key & 0xf), (int)((key >> 4) & 0xffff), (int)((key >> 32) & 0xff),
+                  (int)((key >> 24) & 0xff));
   }
   if (any) HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(DevError) * ni, c->stream));
   return 0;
@@ -2284,9 +2363,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
       delete c;
       return AMR_ERR_CONFIG;
     }
-    // defs.hpp:30
-    const double p = std::pow(kNbRoots, g.maxlevs - 1);
-    I.maxblocks = kNbRoots > p ? kNbRoots : (int)(std::pow(4, g.maxlevs) * 2);
+    I.maxblocks = maxblocks_of(g.maxlevs);
     total_mb += I.maxblocks;
     // meshblock.cpp:90-96; a periodic domain needs its exact length, so it
     // uses lenx / (sqrt(nbroots) * nx) instead
@@ -2294,7 +2371,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     I.dx[0] = g.lenx / (std::sqrt((double)kNbRoots) * (c->NT - 2 * c->ng - (I.periodic ? 0 : 1)));
     for (int l = 1; l < kMaxLev; l++) I.dx[l] = I.dx[l - 1] / 2.;
     // IC2Dtype.cpp:18-80 (gamma, tEnd per problem)
-    if (I.ic == IC_ROTOR) { I.gamma = 1.4; I.tEnd = 0.295; }
+    if (I.ic == IC_ROTOR || I.ic == IC_ROTOR_BS) { I.gamma = 1.4; I.tEnd = 0.295; }
     else if (I.ic == IC_ROTOR_P) { I.gamma = g.ic_params[6]; I.tEnd = g.ic_params[7]; }
     else if (I.ic == IC_OT) { I.gamma = 5. / 3.; I.tEnd = 0.5; }
     else { I.gamma = 2.0; I.tEnd = 0.2; }
@@ -2553,12 +2630,16 @@ int amr_download(amr_ctx* c, int inst, int field, int nslots, double* host) {
   AMR_TRY
   if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
+  if (nslots < 0 || nslots > I.maxblocks) return AMR_ERR_CONFIG;
   const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
   HIPCHK(hipStreamSynchronize(c->stream));
+  // blocks the instance has never used hold no data: zeros, no slot is taken
   for (int nb = 0; nb < nslots; nb++) {
-    const int gs = slot_of(c, I, nb);
-    if (gs < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
-    HIPCHK(hipMemcpy(host + per * nb, c->F.f[field] + per * gs, per * sizeof(double), hipMemcpyDeviceToHost));
+    if (nb >= (int)I.gs.size()) {
+      std::memset(host + per * nb, 0, per * sizeof(double));
+      continue;
+    }
+    HIPCHK(hipMemcpy(host + per * nb, c->F.f[field] + per * I.gs[nb], per * sizeof(double), hipMemcpyDeviceToHost));
   }
   return AMR_OK;
   AMR_CATCH
@@ -2568,13 +2649,12 @@ int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) 
   AMR_TRY
   if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
+  // only blocks the instance uses (amr_init, amr_set_tree, regrids) have slots
+  if (nslots < 0 || nslots > (int)I.gs.size()) return AMR_ERR_CONFIG;
   const size_t per = (size_t)kFieldVars[field] * c->NT * c->NT;
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int nb = 0; nb < nslots; nb++) {
-    const int gs = slot_of(c, I, nb);
-    if (gs < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
-    HIPCHK(hipMemcpy(c->F.f[field] + per * gs, host + per * nb, per * sizeof(double), hipMemcpyHostToDevice));
-  }
+  for (int nb = 0; nb < nslots; nb++)
+    HIPCHK(hipMemcpy(c->F.f[field] + per * I.gs[nb], host + per * nb, per * sizeof(double), hipMemcpyHostToDevice));
   return AMR_OK;
   AMR_CATCH
 }
@@ -2596,20 +2676,30 @@ int amr_get_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* ic, int32_t*
 
 int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* ic, const int32_t* lf) {
   AMR_TRY
-  if (check_inst(c, inst) || n > c->inst[inst].maxblocks) return AMR_ERR_CONFIG;
+  if (check_inst(c, inst) || c->in_flight) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
+  if (int e = validate_tree(n, I.maxblocks, I.maxlevs, c->nxi, lp, ic, lf)) return e;
+  // every block gets its slot first, so a full arena leaves the instance unchanged
+  for (int nb = 0; nb < n; nb++)
+    if (slot_of(c, I, nb) < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
   grow_tree(I, n);
   for (int nb = 0; nb < n; nb++) {
     for (int s = 0; s < 8; s++) I.lp[nb][s] = lp[8 * nb + s];
     I.icoord[nb] = {ic[2 * nb], ic[2 * nb + 1]};
     I.leafs[nb] = (uint8_t)lf[nb];
-    if (slot_of(c, I, nb) < 0) return set_error(c, AMR_ERR_CAPACITY, inst, nb, 0, 0);
   }
   I.lastActive = n;
   I.done = I.status != 0;
   I.prog_dirty = true;
   c->topo_dirty = true;
   return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_validate_tree(int nx, int maxlevs, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs) {
+  AMR_TRY
+  if (nx <= 0 || maxlevs < 1 || maxlevs > 8) return AMR_ERR_CONFIG;
+  return validate_tree(n, maxblocks_of(maxlevs), maxlevs, nx, lp, icoord, leafs);
   AMR_CATCH
 }
 
@@ -2621,6 +2711,185 @@ int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int3
   I.dt_dom = dt;
   I.dt_main = dt_next;
   I.count = count;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------- checkpoint
+// Binary checkpoint of one instance (no counterpart in the reference, whose
+// only output, save4py in savedata.cpp:7-46, keeps 6 significant digits of
+// the primitives of the leaves and cannot restart a run).  It holds what the
+// next steps read: the configuration it belongs to, t and the two-deep dt lag
+// (main.cpp:120-122, 231-234: the dt applied in the current step and the one
+// computed for the next), the step count, the status, the whole quadtree of
+// every block ever allocated (stale blocks included: their slots are reused,
+// admesh.cpp:33-51) and all seven slot fields of those blocks (stale per-slot
+// ff/gg/EMF values feed later ghost fills).  Little-endian, FNV-1a-64 of all
+// preceding bytes at the end:
+//   CkptHeader | lp int32[n][8] | icoord int32[n][2] | leafs int32[n] |
+//   U, Us, Bi, Bis, ff, gg, EMF float64[n][nv][NT][NT] | uint64 checksum
+namespace {
+
+struct CkptHeader {
+  char magic[8];
+  int32_t version, header_bytes;
+  int32_t nx, nghosts, maxlevs, NT, rk, periodic, lim, solver, ic, last_active, count, status;
+  int32_t nfields, pad;
+  double t, dt, dt_next, cfl, lenx, gamma, t_end;
+  double ic_params[8];
+};
+constexpr char kCkptMagic[8] = {'A', 'M', 'R', '2', 'D', 'C', 'K', 'P'};
+constexpr int32_t kCkptVersion = 1;
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t k = 0; k < n; k++) {
+    h ^= b[k];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+constexpr uint64_t kFnvBasis = 14695981039346656037ull;
+
+CkptHeader ckpt_header(const amr_ctx* c, const Instance& I) {
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = kCkptVersion;
+  h.header_bytes = (int32_t)sizeof(CkptHeader);
+  h.nx = c->nxi;
+  h.nghosts = c->ng;
+  h.maxlevs = I.maxlevs;
+  h.NT = c->NT;
+  h.rk = c->rk;
+  h.periodic = I.periodic ? 1 : 0;
+  h.lim = I.lim;
+  h.solver = I.solver;
+  h.ic = I.ic;
+  h.last_active = I.lastActive;
+  h.count = I.count;
+  h.status = I.status;
+  h.nfields = kNumFields;
+  h.t = I.t;
+  h.dt = I.dt_dom;
+  h.dt_next = I.dt_main;
+  h.cfl = I.cfl;
+  h.lenx = I.cfg.lenx;
+  h.gamma = I.gamma;
+  h.t_end = I.tEnd;
+  for (int k = 0; k < 8; k++) h.ic_params[k] = I.ic == IC_ROTOR_P ? I.cfg.ic_params[k] : 0.;
+  return h;
+}
+
+// the configuration fields a checkpoint must agree on with the instance
+bool ckpt_matches(const CkptHeader& a, const CkptHeader& b) {
+  return a.nx == b.nx && a.nghosts == b.nghosts && a.maxlevs == b.maxlevs && a.NT == b.NT && a.rk == b.rk &&
+         a.periodic == b.periodic && a.lim == b.lim && a.solver == b.solver && a.ic == b.ic &&
+         a.nfields == b.nfields && a.cfl == b.cfl && a.lenx == b.lenx && a.gamma == b.gamma &&
+         a.t_end == b.t_end && std::memcmp(a.ic_params, b.ic_params, sizeof(a.ic_params)) == 0;
+}
+
+struct FileCloser {
+  void operator()(std::FILE* f) const {
+    if (f) std::fclose(f);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int amr_checkpoint_save(amr_ctx* c, int inst, const char* path) {
+  AMR_TRY
+  if (check_inst(c, inst) || !path || c->in_flight) return AMR_ERR_CONFIG;
+  const Instance& I = c->inst[inst];
+  const int n = I.lastActive;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const std::string tmp = std::string(path) + ".tmp";
+  std::unique_ptr<std::FILE, FileCloser> fp(std::fopen(tmp.c_str(), "wb"));
+  if (!fp) return AMR_ERR_IO;
+  uint64_t h = kFnvBasis;
+  bool ok = true;
+  auto put = [&](const void* p, size_t bytes) {
+    h = fnv1a(h, p, bytes);
+    ok = ok && std::fwrite(p, 1, bytes, fp.get()) == bytes;
+  };
+  const CkptHeader hd = ckpt_header(c, I);
+  put(&hd, sizeof hd);
+  std::vector<int32_t> lp(8 * (size_t)n), ic(2 * (size_t)n), lf(n);
+  for (int nb = 0; nb < n; nb++) {
+    for (int s = 0; s < 8; s++) lp[8 * nb + s] = I.lp[nb][s];
+    ic[2 * nb] = I.icoord[nb][0];
+    ic[2 * nb + 1] = I.icoord[nb][1];
+    lf[nb] = I.leafs[nb];
+  }
+  put(lp.data(), lp.size() * 4);
+  put(ic.data(), ic.size() * 4);
+  put(lf.data(), lf.size() * 4);
+  std::vector<double> tile;
+  for (int f = 0; f < kNumFields && ok; f++) {
+    const size_t per = (size_t)kFieldVars[f] * c->NT * c->NT;
+    tile.resize(per);
+    for (int nb = 0; nb < n && ok; nb++) {
+      HIPCHK(hipMemcpy(tile.data(), c->F.f[f] + per * I.gs[nb], per * sizeof(double), hipMemcpyDeviceToHost));
+      put(tile.data(), per * sizeof(double));
+    }
+  }
+  ok = ok && std::fwrite(&h, 1, sizeof h, fp.get()) == sizeof h;
+  ok = std::fclose(fp.release()) == 0 && ok;
+  // the file appears under its name only when complete
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return AMR_ERR_IO;
+  }
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_checkpoint_load(amr_ctx* c, int inst, const char* path) {
+  AMR_TRY
+  if (check_inst(c, inst) || !path || c->in_flight) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  std::unique_ptr<std::FILE, FileCloser> fp(std::fopen(path, "rb"));
+  if (!fp) return AMR_ERR_IO;
+  uint64_t h = kFnvBasis;
+  auto get = [&](void* p, size_t bytes) {
+    if (std::fread(p, 1, bytes, fp.get()) != bytes) return false;
+    h = fnv1a(h, p, bytes);
+    return true;
+  };
+  CkptHeader hd{};
+  if (!get(&hd, sizeof hd) || std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != kCkptVersion ||
+      hd.header_bytes != (int32_t)sizeof(CkptHeader))
+    return AMR_ERR_IO;
+  // a checkpoint restarts only the run it was written from
+  if (!ckpt_matches(hd, ckpt_header(c, I))) return AMR_ERR_CONFIG;
+  const int n = hd.last_active;
+  if (n < kNbRoots || n > I.maxblocks) return AMR_ERR_IO;
+  std::vector<int32_t> lp(8 * (size_t)n), ic(2 * (size_t)n), lf(n);
+  if (!get(lp.data(), lp.size() * 4) || !get(ic.data(), ic.size() * 4) || !get(lf.data(), lf.size() * 4))
+    return AMR_ERR_IO;
+  std::vector<std::vector<double>> fields(kNumFields);
+  for (int f = 0; f < kNumFields; f++) {
+    fields[f].resize((size_t)n * kFieldVars[f] * c->NT * c->NT);
+    if (!get(fields[f].data(), fields[f].size() * sizeof(double))) return AMR_ERR_IO;
+  }
+  uint64_t stored = 0;
+  if (std::fread(&stored, 1, sizeof stored, fp.get()) != sizeof stored || stored != h) return AMR_ERR_IO;
+  if (validate_tree(n, I.maxblocks, I.maxlevs, c->nxi, lp.data(), ic.data(), lf.data())) return AMR_ERR_IO;
+  // the file is whole and consistent: apply it
+  if (int e = amr_set_tree(c, inst, n, lp.data(), ic.data(), lf.data())) return e;
+  for (int f = 0; f < kNumFields; f++)
+    if (int e = amr_upload(c, inst, f, n, fields[f].data())) return e;
+  I.t = hd.t;
+  I.dt_dom = hd.dt;
+  I.dt_main = hd.dt_next;
+  I.count = hd.count;
+  I.status = hd.status;
+  I.err = amr_diag{hd.status, inst, -1, 0, 0, 0};
+  I.done = I.status != 0 || !(I.t <= I.tEnd);
+  c->topo_dirty = true;
   return AMR_OK;
   AMR_CATCH
 }

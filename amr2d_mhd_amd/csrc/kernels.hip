@@ -17,16 +17,11 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// per-instance error records: the first error of an instance wins, so a
-// failing instance never hides or corrupts another instance's record
-__device__ inline void report(DevError* err, int code, int inst, int nb, int i, int j) {
-  DevError* e = err + inst;
-  if (atomicCAS(&e->code, 0, code) == 0) {
-    e->inst = inst;
-    e->nb = nb;
-    e->i = i;
-    e->j = j;
-  }
+// per-instance error records: of the errors an instance's kernels meet, the
+// one the reference's sequential execution meets first is kept (see
+// DevError), and a failing instance never touches another instance's record
+__device__ inline void report(DevError* err, int inst, unsigned long long key) {
+  atomicMax(&err[inst].nkey, ~key);
 }
 
 // conserved -> primitive of one cell (meshblock.cpp:119-179); returns the
@@ -102,8 +97,11 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
 #pragma unroll
     for (int k = 0; k < 8; k++) sW[k * N2 + c] = w[k];
     const int i = c % NT, j = c / NT;
-    if (w[4] <= 0 && i > LO && i < HI + 1 && j > LO && j < HI + 1) report(err, kErrPressure, L.inst, L.nb, i, j);
-    if (step == 2 && (isnan(w[1]) || isnan(w[2]) || isnan(w[3]))) report(err, kErrNanVel, L.inst, L.nb, i, j);
+    // meshblock.cpp:135-142 / 163-176, cell by cell: pressure, then velocity NaN
+    if (w[4] <= 0 && i > LO && i < HI + 1 && j > LO && j < HI + 1)
+      report(err, L.inst, error_key(step, L.nb, 0, i, j, 0, L.nb, kErrPressure));
+    if (step == 2 && (isnan(w[1]) || isnan(w[2]) || isnan(w[3])))
+      report(err, L.inst, error_key(step, L.nb, 0, i, j, 1, L.nb, kErrNanVel));
     if (lim < kLimWENO3) {
       bool bad = false;
 #pragma unroll
@@ -121,7 +119,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
   }
   __syncthreads();
   if (sNan) {
-    if (tid == 0) report(err, kErrNanEdge, L.inst, L.nb, 0, 0);
+    if (tid == 0) report(err, L.inst, error_key(step, L.nb, 1, 0, 0, 0, L.nb, kErrNanEdge));
     return;
   }
 
@@ -160,11 +158,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
       st = normal_flux(solver, wl, wr, g, fl);
       swap_xy(fl);
     }
-    if (st) report(err, kErrHLLC, L.inst, L.nb, i, j);
+    // MUSCL2D.cpp:62-115: per (i, j) the x face (solver, NaN), then the y face
+    if (st) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 3 : 0, L.nb, kErrHLLC));
     bool nan = false;
 #pragma unroll
     for (int k = 0; k < 8; k++) nan |= isnan(fl[k]);
-    if (nan) report(err, kErrNanFlux, L.inst, L.nb, i, j);
+    if (nan) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 4 : 1, L.nb, kErrNanFlux));
     if (!ydir) {
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -419,7 +418,7 @@ __global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, con
       for (int k = 0; k < 8; k++) sW[k * N2 + c] = w[k];
       const int i = c % NT, j = c / NT;
       if (w[4] <= 0 && i > LO - 1 && i < HI + 1 && j > LO - 1 && j < HI + 1)
-        report(err, kErrPressure, L.inst, L.nb, i, j);
+        report(err, L.inst, error_key(kOrdCfl, L.nb, 0, i, j, 0, L.nb, kErrPressure));
     }
   }
   __syncthreads();
@@ -528,13 +527,16 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(FieldPtrs F, const Reg
   }
   double w[8];
   cons_to_prim(u, ip[op.inst].gamma, w);
+  // refine.cpp:69-83: U2W of son 1..4 in turn; then refineBi_CT.cpp:59-71
+  // over (i, j, k) with the four sons innermost
+  const int seq = op.seq;
   if (w[4] <= 0 && i > NG - 1 && i < NT - NG && j > NG - 1 && j < NT - NG)
-    report(err, kErrPressure, op.inst, op.son_gs[q], i, j);
+    report(err, op.inst, error_key(kOrdRefine, seq, q, i, j, 0, op.son_nb + q, kErrPressure));
   const double* Bp = F.f[kBi] + (size_t)op.parent_gs * 2 * N2;
   double* Bs = F.f[kBi] + (size_t)op.son_gs[q] * 2 * N2;
   for (int k = 0; k < 2; k++) {
     const double v = prolong_face(Bp, NT, N2, pi, pj, k, sx, sy);
-    if (isnan(v)) report(err, kErrRefineNan, op.inst, op.nb, i, j);
+    if (isnan(v)) report(err, op.inst, error_key(kOrdRefine, seq, 4, i, j, 4 * k + q, op.nb, kErrRefineNan));
     Bs[k * N2 + c] = v;
   }
 }
@@ -572,7 +574,7 @@ __global__ __launch_bounds__(kThreads) void coarsen_kernel(FieldPtrs F, const Re
     double w[8];
     cons_to_prim(u, ip[op.inst].gamma, w);
     if (w[4] <= 0 && i > NG - 1 && i < NT - NG && j > NG - 1 && j < NT - NG)
-      report(err, kErrPressure, op.inst, op.nb, i, j);
+      report(err, op.inst, error_key(kOrdCoarsen, op.seq, 0, i, j, 0, op.nb, kErrPressure));
   }
   double* Bd = F.f[kBi] + (size_t)op.parent_gs * 2 * N2;
   for (int k = 0; k < 2; k++) {

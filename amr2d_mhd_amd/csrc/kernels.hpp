@@ -29,8 +29,10 @@ struct RegridOp {
   int32_t parent_gs;
   int32_t son_gs[4];
   int32_t inst;
-  int32_t nb;
-  int32_t pad;
+  int32_t nb;      // parent (refine) or dad (coarsen) block id
+  int32_t son_nb;  // first son's block id; the four sons are son_nb .. son_nb + 3
+  int32_t seq;     // position among the instance's ops of this regrid, in the
+                   // reference's order (all refinements, then all coarsenings)
 };
 
 bool tile_supported(int NT, int ng);

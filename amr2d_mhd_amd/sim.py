@@ -109,18 +109,32 @@ class Simulation:
         _check(self.h, lib().amr_set_time(self.h, inst, t, dt, dt_next, count))
 
     def load_state(self, st, inst=0):
-        """Load a full state (tree + all slot fields) given as a dict of arrays."""
+        """Load a full state (tree, all slot fields, t and the dt lag) given as
+        a dict of arrays, as returned by state(); dt_next and count default to
+        0 for states that carry only t and dt (the reference's own dumps)."""
         self.set_tree(st["lp"], st["icoord"], st["leafs"], inst)
         for f in FIELDS:
             self.upload(f, st[f], inst)
-        self.set_time(st["t"], st["dt"], 0.0, 0, inst)
+        self.set_time(st["t"], st["dt"], st.get("dt_next", 0.0), st.get("count", 0), inst)
 
     def state(self, inst=0):
+        """Tree, every slot field of every allocated block, t, the dt applied
+        by the current step, the dt computed for the next one and the step
+        count: everything the following steps read."""
         lp, ic, lf = self.get_tree(inst)
-        st = dict(lp=lp, icoord=ic, leafs=lf)
+        inf = self.info(inst)
+        st = dict(lp=lp, icoord=ic, leafs=lf, t=inf["t"], dt=inf["dt"], dt_next=inf["dt_next"], count=inf["count"])
         for f in FIELDS:
             st[f] = self.download(f, inst)
         return st
+
+    def save_checkpoint(self, path, inst=0):
+        """Write a binary checkpoint of one instance (amr_checkpoint_save)."""
+        _check(self.h, lib().amr_checkpoint_save(self.h, inst, str(path).encode()))
+
+    def load_checkpoint(self, path, inst=0):
+        """Restart one instance from a checkpoint of the same configuration."""
+        _check(self.h, lib().amr_checkpoint_load(self.h, inst, str(path).encode()))
 
     # ---------------------------------------------------------------- phases
     def fill(self, field, kind):

@@ -31,8 +31,9 @@ enum {
   AMR_ERR_CONFIG = 10,
   AMR_ERR_HIP = 11,
   AMR_ERR_CAPACITY = 12,  /* slot arena of the context exhausted */
-  AMR_ERR_HOST = 13       /* host-side failure (out of host memory): no exception
+  AMR_ERR_HOST = 13,      /* host-side failure (out of host memory): no exception
                              crosses this interface; destroy the context */
+  AMR_ERR_IO = 14         /* checkpoint file missing, unwritable, truncated or corrupt */
 };
 
 /* slot fields (meshblock arrays, defs.hpp:69-83); layout [slot][var][j][i] */
@@ -116,10 +117,37 @@ int amr_last_error(amr_ctx *ctx, amr_diag *diag);
  * [nb][var][j][i] float64. */
 int amr_download(amr_ctx *ctx, int inst, int field, int nslots, double *host);
 int amr_upload(amr_ctx *ctx, int inst, int field, int nslots, const double *host);
-/* Quadtree of one instance: lp[nb][8], icoord[nb][2], leafs[nb] (defs.hpp:54-58) */
+/* Quadtree of one instance: lp[nb][8], icoord[nb][2], leafs[nb] (defs.hpp:54-58).
+ * amr_set_tree rejects (AMR_ERR_CONFIG, instance unchanged) a tree that is not
+ * one the reference's bookkeeping can produce (basegrid.cpp:4-89,
+ * updatelpup.cpp:4-118, updatelpdown.cpp:5-54): 4 <= nslots <= maxblocks,
+ * roots 0..3 at level 0, every other block son lp[nb][3] of its father's group
+ * of 4 consecutive ids one level down, consistent sibling pointers inside each
+ * group, neighbours in [-1, nslots), refined neighbours of leaves with a son
+ * group, leaf flags 0/1, icoord in [0, nx << maxlevs).  amr_validate_tree
+ * applies the same check without a context (host only).  amr_download
+ * returns zeros for blocks the instance has never used (nslots <= maxblocks);
+ * amr_upload accepts only blocks it uses. */
 int amr_get_tree(amr_ctx *ctx, int inst, int nslots, int32_t *lp, int32_t *icoord, int32_t *leafs);
 int amr_set_tree(amr_ctx *ctx, int inst, int nslots, const int32_t *lp, const int32_t *icoord, const int32_t *leafs);
+int amr_validate_tree(int nx, int maxlevs, int nslots, const int32_t *lp, const int32_t *icoord,
+                      const int32_t *leafs);
+/* t, the dt applied by the current step (meshblock::dt), the dt computed for
+ * the next one and the step count: the two-deep dt lag of main.cpp:120-122,
+ * 231-234 */
 int amr_set_time(amr_ctx *ctx, int inst, double t, double dt, double dt_next, int32_t count);
+
+/* Binary checkpoint / restart of one instance (no counterpart in the
+ * reference: its save4py, savedata.cpp:7-46, writes 6-digit ASCII primitives
+ * of the leaves and cannot restart a run).  The file holds the configuration,
+ * t, dt, dt_next, the step count and status, the quadtree of every allocated
+ * block and all seven slot fields of those blocks, with a checksum; a run
+ * loaded into a context of the same configuration continues bit for bit as
+ * the run that wrote it.  Load fails with AMR_ERR_CONFIG for another
+ * configuration and AMR_ERR_IO for a missing, truncated or corrupt file,
+ * leaving the instance unchanged. */
+int amr_checkpoint_save(amr_ctx *ctx, int inst, const char *path);
+int amr_checkpoint_load(amr_ctx *ctx, int inst, const char *path);
 
 /* Individual phases of one step, for phase-level parity checks.
  *  amr_fill:  boundary.cpp:7 (kind 0), boundaryfc.cpp:8 (1), boundaryec.cpp:7 (2)

@@ -10,13 +10,17 @@
 #
 # Variants: ng<G>_lv<L>      serial, -O3 -ffp-contract=off (the oracle build)
 #           ng<G>_lv<L>_omp  -fopenmp -DOPENMP, generic x86-64 (CPU baseline)
+#           ng<G>_lv<L>_bs   serial; its "rotor" is the Balsara-Spicer rotor of
+#                            the rotor workload's text (p = 1, v0 = 2,
+#                            Bx = 5/sqrt(4 pi)): a temporary copy of
+#                            IC2Dtype.cpp with only those constants changed
 # Output: oracle/_ref/<variant>/refrun  (harness + reference objects)
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${REF:-/root/reference}"
 OUT="$HERE/_ref"
 JOBS="${JOBS:-8}"
-VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp}"
+VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp ng2_lv5_bs}"
 
 if [ ! -f "$REF/defs.hpp" ]; then
   echo "build_ref: reference not present at $REF; skipping" >&2
@@ -49,12 +53,27 @@ build_variant() {
   ng="$(echo "$v" | sed -E 's/^ng([0-9]+)_.*/\1/')"
   lv="$(echo "$v" | sed -E 's/^ng[0-9]+_lv([0-9]+).*/\1/')"
   omp=0; [[ "$v" == *_omp ]] && omp=1
+  bs=0; [[ "$v" == *_bs ]] && bs=1
   local inc="$TMP/$v" obj="$OUT/$v/obj" bin="$OUT/$v/refrun"
   mkdir -p "$inc" "$obj"
   sed -e "s/^const int nghosts=2;/const int nghosts=${ng};/" \
       -e "s/^const int maxlevs=5;/const int maxlevs=${lv};/" "$REF/defs.hpp" > "$inc/defs.hpp"
   grep -q "^const int nghosts=${ng};" "$inc/defs.hpp" || { echo "patch nghosts failed" >&2; return 1; }
   grep -q "^const int maxlevs=${lv};" "$inc/defs.hpp" || { echo "patch maxlevs failed" >&2; return 1; }
+  if [ "$bs" = 1 ]; then
+    # rotor constants of IC2Dtype.cpp:79-80, 100-101, 104-105 (v0 = 2: both
+    # velocity components doubled, an exact scaling)
+    sed -e 's|pinit\[0\]=0\.5;|pinit[0]=1.0;|' \
+        -e 's|Bxinit\[0\]=2\.5/(sqrt(4\*pi));|Bxinit[0]=5.0/(sqrt(4*pi));|' \
+        -e 's|\]=-(y-0\.5)/0\.1;|]=-2.*(y-0.5)/0.1;|' \
+        -e 's|\]=(x-0\.5)/0\.1;|]=2.*(x-0.5)/0.1;|' \
+        -e 's|\]=(-(y-0\.5)\*fr/0\.1);|]=(-2.*(y-0.5)*fr/0.1);|' \
+        -e 's|\]=(x-0\.5)\*fr/0\.1;|]=2.*(x-0.5)*fr/0.1;|' "$REF/IC2Dtype.cpp" > "$inc/IC2Dtype.cpp"
+    for pat in 'pinit\[0\]=1\.0;' 'Bxinit\[0\]=5\.0/' '=-2\.\*(y-0\.5)/0\.1;' '=2\.\*(x-0\.5)/0\.1;' \
+               '=(-2\.\*(y-0\.5)\*fr/0\.1);' '=2\.\*(x-0\.5)\*fr/0\.1;'; do
+      [ "$(grep -c "$pat" "$inc/IC2Dtype.cpp")" = 1 ] || { echo "patch rotor_bs failed: $pat" >&2; return 1; }
+    done
+  fi
   local flags="-O3 -std=c++17 -ffp-contract=off -DDOUBLE_PREC -w"
   if [ "$omp" = 1 ]; then flags="-O3 -std=c++17 -funroll-loops -ffp-contract=off -DDOUBLE_PREC -DOPENMP -fopenmp -w"; fi
   # skip when up to date
@@ -65,6 +84,7 @@ build_variant() {
   local objs=()
   for src in "$REF"/*.cpp; do
     local b; b="$(basename "$src" .cpp)"
+    [ -f "$inc/$b.cpp" ] && src="$inc/$b.cpp"
     # shellcheck disable=SC2046
     g++ $flags $(rename_flags "$b.cpp") -I"$inc" -I- -I"$REF" -c "$src" -o "$obj/$b.o" 2>"$obj/$b.err" || { cat "$obj/$b.err" >&2; return 1; }
     objs+=("$obj/$b.o")

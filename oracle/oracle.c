@@ -31,7 +31,7 @@ static inline double stdmax(double a, double b) { return (a < b) ? b : a; }
 
 enum { L_1ST, L_NOSL, L_MINMOD, L_VANLEER, L_SUPERBEE, L_MC, L_KOREN, L_CADA, L_WENO3, L_WENO5, L_BAD };
 enum { F_RUSA, F_HLLE, F_HLLC, F_HLLD, F_BAD };
-enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_BAD };
+enum { IC_SST, IC_SST1, IC_SSTY, IC_BWX, IC_BWX1, IC_BWY, IC_BWY1, IC_ROTOR, IC_ROTOR_P, IC_OT, IC_ROTOR_BS, IC_BAD };
 
 static int lim_id(const char *s) {
   if (!strcmp(s, "1st")) return L_1ST;
@@ -54,8 +54,9 @@ static int flux_id(const char *s) {
   return F_BAD;
 }
 static int ic_id(const char *s) {
-  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p", "OT"};
-  for (int q = 0; q < 10; q++)
+  static const char *names[] = {"SST", "SST1", "SSTy", "BWx", "BWx1", "BWy", "BWy1", "rotor", "rotor_p", "OT",
+                                "rotor_bs"};
+  for (int q = 0; q < IC_BAD; q++)
     if (!strcmp(s, names[q])) return q;
   return IC_BAD;
 }
@@ -641,6 +642,9 @@ static void initial_condition(ora_dom *d, int nb) {
     case IC_BWY1: SET4(p4, 1.0, 1.0, 0.1, 0.1); SET4(r4, 1.0, 1.0, 0.125, 0.125);
       SET4(by4, 0.75, 0.75, 0.75, 0.75); SET4(bx4, 1.0, 1.0, -1.0, -1.0); break;
     case IC_ROTOR: gamma = 1.4; tEnd = 0.295; p4[0] = 0.5; bx4[0] = 2.5 / (sqrt(4 * PI_REF)); break;
+    /* the Balsara-Spicer rotor (p = 1, v0 = 2, Bx = 5/sqrt(4 pi)) in the form of
+       IC2Dtype.cpp:75-111 with only those constants changed (build-defined) */
+    case IC_ROTOR_BS: gamma = 1.4; tEnd = 0.295; p4[0] = 1.0; bx4[0] = 5.0 / (sqrt(4 * PI_REF)); break;
     case IC_ROTOR_P: gamma = d->icp[6]; tEnd = d->icp[7]; break;
     case IC_OT: gamma = 5. / 3.; tEnd = 0.5; break;
   }
@@ -663,7 +667,10 @@ static void initial_condition(ora_dom *d, int nb) {
         AT(W, NT, 4, i, j) = 5. / (12. * pi);
         AT(W, NT, 5, i, j) = -B0 * sin(2. * pi * y);
         AT(W, NT, 6, i, j) = B0 * sin(4. * pi * x);
-      } else if (d->ic == IC_ROTOR) {
+      } else if (d->ic == IC_ROTOR || d->ic == IC_ROTOR_BS) {
+        /* spin speed v0 at r = 0.1: 1 (reference), 2 (Balsara-Spicer); scaling
+           by 1 or 2 is exact, so the reference rotor's values are unchanged */
+        const double v0 = d->ic == IC_ROTOR_BS ? 2. : 1.;
         double radius = sqrt(((x - 0.5) * (x - 0.5)) + ((y - 0.5) * (y - 0.5)));
         double fr = (23. - 200. * radius) / 3.;
         for (int k = 0; k < 8; k++) AT(W, NT, k, i, j) = 0.;
@@ -671,12 +678,12 @@ static void initial_condition(ora_dom *d, int nb) {
         AT(W, NT, 5, i, j) = bx4[0];
         if (radius <= 0.1) {
           AT(W, NT, 0, i, j) = 10.;
-          AT(W, NT, 1, i, j) = -(y - 0.5) / 0.1;
-          AT(W, NT, 2, i, j) = (x - 0.5) / 0.1;
+          AT(W, NT, 1, i, j) = -v0 * (y - 0.5) / 0.1;
+          AT(W, NT, 2, i, j) = v0 * (x - 0.5) / 0.1;
         } else if (radius > 0.1 && radius < 0.115) {
           AT(W, NT, 0, i, j) = (1 + 9. * fr);
-          AT(W, NT, 1, i, j) = (-(y - 0.5) * fr / 0.1);
-          AT(W, NT, 2, i, j) = (x - 0.5) * fr / 0.1;
+          AT(W, NT, 1, i, j) = (-v0 * (y - 0.5) * fr / 0.1);
+          AT(W, NT, 2, i, j) = v0 * (x - 0.5) * fr / 0.1;
         } else {
           AT(W, NT, 0, i, j) = 1.;
         }

@@ -304,3 +304,30 @@ def test_rotor_lv5_first_2400_steps_equal_reference_log():
         if (s + 1) % 200 == 0:
             assert hashes() == (int(r["hf_post"]), int(r["ht_post"])), f"state hash differs after step {s}"
     sim.close()
+
+
+def test_rotor_bs_aborts_like_reference():
+    """The rotor workload's stated initial condition (rotor_bs) at 5 levels:
+    the GPU state equals the oracle's at the start and after step 0, and the
+    next step fails with the reference's own abort (p <= 0 in the refined son
+    39 at cell (2, 2), the oracle being pinned to the patched-IC reference)."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd import AmrError
+
+    cfg = dict(ic="rotor_bs", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
+    sim = A.Simulation(cfg)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "initial state differs: " + "; ".join(d[:4])
+    assert dom.step() == 0
+    sim.step()
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "state after step 0 differs: " + "; ".join(d[:4])
+    assert dom.step() == 1 and dom.last_error() == [1, 39, 2, 2]
+    with pytest.raises(AmrError) as e:
+        sim.step()
+    assert e.value.code == 1
+    assert (e.value.diag.nb, e.value.diag.i, e.value.diag.j) == (39, 2, 2)
+    assert sim.info()["status"] == 1

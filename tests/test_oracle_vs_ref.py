@@ -337,3 +337,36 @@ def test_cfl05_brio_wu_fails_like_reference():
     dom.init()
     assert [dom.step() for _ in range(3)] == [0, 0, 1]
     assert dom.last_error() == [1, 53, 17, 2]
+
+
+@pytest.mark.skipif(not os.access(os.path.join(refrun.REF_DIR, "ng2_lv5_bs", "refrun"), os.X_OK),
+                    reason="patched-IC reference (oracle/_ref/ng2_lv5_bs) not built")
+@pytest.mark.parametrize("cfl", [0.5, 0.3])
+def test_rotor_bs_aborts_like_reference(tmp_path, cfl):
+    """The rotor workload's stated initial condition (Balsara-Spicer: p = 1,
+    v0 = 2, Bx = 5/sqrt(4 pi)) against the reference built with only those
+    constants of IC2Dtype.cpp changed: identical state hashes through step 0,
+    then the reference itself aborts in its second step -- U2W of a freshly
+    refined son (refine.cpp:69-83) finds p <= 0 at block 39, i = j = 2 (the
+    domain corner) -- and the oracle stops with the same code at the same
+    block and cell, at any CFL."""
+    import re
+    import subprocess
+
+    exe = os.path.join(refrun.REF_DIR, "ng2_lv5_bs", "refrun")
+    p = subprocess.run([exe, "run", "--dir", str(tmp_path), "--verbose", "--log", str(tmp_path / "log.bin"),
+                        "--input", refrun.input_line(16, 1.0, cfl, "rotor", "MC", "HLLD")],
+                       capture_output=True, text=True, timeout=600)
+    assert p.returncode == 10, p.stderr
+    m = re.search(r"Error at U2W as zero/negative pressure! P=\S+, at i=(\d+), j=(\d+) & nb=(\d+)", p.stdout)
+    assert m, p.stdout[-2000:]
+    ref_i, ref_j, ref_nb = (int(g) for g in m.groups())
+    log = refrun.read_log(str(tmp_path))
+    assert len(log) == 2  # the initial state and step 0
+    dom = O.Domain(ic="rotor_bs", limiter="MC", flux="HLLD", cfl=cfl, nx=16, nghosts=2, maxlevs=5)
+    assert dom.init() == 0
+    assert (dom.hash_fields(), dom.hash_tree()) == (int(log[0]["hf_post"]), int(log[0]["ht_post"]))
+    assert dom.step() == 0
+    assert (dom.hash_fields(), dom.hash_tree()) == (int(log[1]["hf_post"]), int(log[1]["ht_post"]))
+    assert dom.step() == 1
+    assert dom.last_error() == [1, ref_nb, ref_i, ref_j] == [1, 39, 2, 2]
