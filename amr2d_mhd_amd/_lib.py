@@ -77,12 +77,14 @@ _SIGS = {
     "amr_recon_batch": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "amr_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_set_step_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_set_fill_cache": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_step_begin": (C.c_int, [C.c_void_p]),
     "amr_step_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "amr_run_group": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "amr_set_stage_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "amr_set_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "amr_adj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.POINTER(C.c_void_p)]),

@@ -352,7 +352,7 @@ struct Profile {
   int64_t launches = 0;
 };
 enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_REGRID, PF_HOST_WAIT, PF_HOST_COMPILE,
-       PF_GRAPH, PF_COUNT };
+       PF_GRAPH, PF_STEP, PF_CAPTURE, PF_COUNT };
 
 
 }  // namespace
@@ -388,6 +388,27 @@ struct amr_ctx {
   // the device work of one step replayed as a HIP graph; re-captured when
   // the topology (work lists, program tables, buffer addresses) changes
   bool graph_on = false, graph_dirty = true, regrid_launched = false;
+  // launch geometry of the step's leaf and chunk kernels and of its download:
+  // the work itself (direct launches, launch_counts = null) or capacities
+  // covering it (a captured graph: its kernels read the counts from d_counts
+  // and surplus workgroups exit, so one graph serves every tree whose counts
+  // fit, and tree changes cost no capture)
+  int grid_leaves = 0, grid_chunks = 0;
+  // a host-to-device copy (IC, amr_upload, a cleared record) since the last
+  // direct step: the next step launches directly, so a replayed graph only
+  // ever reads data that kernels wrote
+  bool dma_since_direct = true;
+
+  const int32_t* d_counts = nullptr;       // {nleaf, nchunks}, view into d_topo
+  const int32_t* launch_counts = nullptr;
+  uint64_t topo_epoch = 0;                 // bumps when d_topo moves
+  struct GraphKey {
+    uint64_t epoch = ~0ull;
+    int leaves = 0, chunks = 0, variant = -1, spec = -1;
+    bool operator==(const GraphKey& o) const {
+      return epoch == o.epoch && leaves == o.leaves && chunks == o.chunks && variant == o.variant && spec == o.spec;
+    }
+  } gkey;
   // steps launched since the last topology rebuild: a graph is captured only
   // for a topology that has already lasted a step (most trees of a regridding
   // run last one or two steps, and a capture used once costs more than it saves)
@@ -416,11 +437,25 @@ struct amr_ctx {
   int span_variant[2] = {-1, -1};
   double wall_khz = 0;
   hipGraphExec_t gexec = nullptr;
-  // per-leaf CFL bounds and refinement indicators, [lam(nleaf) | grad(nleaf)]
-  // in one device and one pinned host buffer (one download per step); the
-  // four pointers are views set at each topology rebuild
+  // persistent step kernel (kernels.hpp): 0 = per-phase launches (the
+  // default: on MI355X a grid-wide barrier across the XCDs costs more than the
+  // kernel boundary it replaces), 1 / 2 = one launch per step with
+  // kStepLanes[0] / [1] lanes per workgroup
+  int step_mode = 0;
+  int step_grid[kStageSpecs][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // co-resident grid; -1: not queried yet
+  bool step_broken = false;  // a barrier wait ran out: per-phase launches from then on
+  // inside amr_run_group: steps of several contexts may run at once on their
+  // own streams, and two cooperative grids sized for the whole device could
+  // each hold part of it; grouped contexts launch per phase
+  bool in_group = false;
+  std::vector<StepPhase> phases;
+  GridSync* d_sync = nullptr;
+  int32_t *d_fail = nullptr, *h_fail = nullptr;  // views into d_rec / h_rec, after the error records
+  unsigned long long *d_stamps = nullptr, *h_stamps = nullptr;
+  int stamps_pending = 0;  // phases whose device-clock stamps await collection
+  // per-leaf CFL bounds and refinement indicators as pairs (lam, grad) per
+  // leaf, in one device and one pinned host buffer (one download per step)
   double *d_red = nullptr, *h_red = nullptr;
-  double *d_lam = nullptr, *d_grad = nullptr, *h_lam = nullptr, *h_grad = nullptr;
   // d_err/h_err records first, then the reduction arrays, in one device and
   // one pinned host block sized for every slot: one download per step brings
   // the errors and the per-leaf values
@@ -535,6 +570,16 @@ struct HostTimed {
 
 // collect the elapsed times of the recorded launches (stream already synced)
 void flush_profile(amr_ctx* c) {
+  // phases of a persistent step kernel: device-clock stamps at the start and
+  // after each phase's closing barrier
+  for (int p = 0; p < c->stamps_pending && c->wall_khz > 0; p++) {
+    static const int fam_of[] = {PF_FILL, PF_STAGE, PF_CT, PF_CT, PF_CT, PF_CFL};
+    const int fam = fam_of[c->phases[p].kind];
+    if (!(c->prof_mask & (1u << fam))) continue;
+    c->prof[fam].ms += (double)(c->h_stamps[p + 1] - c->h_stamps[p]) / c->wall_khz;
+    c->prof[fam].launches += 1;
+  }
+  c->stamps_pending = 0;
   for (auto& p : c->ev_pending) {
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev_pool[p.second][0], c->ev_pool[p.second][1]);
@@ -1728,6 +1773,7 @@ int upload_params(amr_ctx* c) {
   fill_params(c);
   HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->h_ip.size(), hipMemcpyHostToDevice,
                         c->stream));
+  c->dma_since_direct = true;
   return 0;
 }
 
@@ -1801,6 +1847,35 @@ bool reuse_program(amr_ctx* c, Instance& I) {
 }
 
 // leaf work list and fill program of the running instances
+// launch geometry of direct launches: the grids are the work
+void direct_geometry(amr_ctx* c) {
+  c->grid_leaves = c->nleaf;
+  c->grid_chunks = c->nchunks;
+  c->launch_counts = nullptr;
+}
+
+// A graph's capacity for a count n: the current one while n fits and fills
+// more than a quarter of it, else the next power of two (at most hard_max).
+int graph_capacity(int cur, int n, int hard_max) {
+  n = std::max(n, 1);
+  if (cur >= n && cur / 4 < n) return cur;
+  int g = 1;
+  while (g < n) g <<= 1;
+  return std::max(n, std::min(g, hard_max));
+}
+
+// A specialised stage kernel serves launches whose running instances all use
+// HLLD with a MUSCL-family limiter, or all HLLD with WENO5 (24x24 tiles).
+int stage_spec_of(const amr_ctx* c) {
+  bool hlld_muscl = true, hlld_weno5 = c->NT == 24 && c->ng == 4;
+  for (const Instance& I : c->inst) {
+    if (I.done) continue;
+    if (!(I.solver == kFluxHLLD && I.lim < kLimWENO3)) hlld_muscl = false;
+    if (!(I.solver == kFluxHLLD && I.lim == kLimWENO5)) hlld_weno5 = false;
+  }
+  return hlld_muscl ? kStageHLLDMuscl : hlld_weno5 ? kStageHLLDWeno5 : kStageGeneric;
+}
+
 int rebuild_topology(amr_ctx* c) {
   HostTimed ht(c, PF_HOST_TOPO);
   HIPCHK(hipStreamSynchronize(c->stream));  // previous uploads consumed
@@ -1818,15 +1893,7 @@ int rebuild_topology(amr_ctx* c) {
   c->leaf_inst_begin[c->inst.size()] = (int)c->h_leaves.size();
   c->nleaf = (int)c->h_leaves.size();
   {
-    // a specialised stage kernel serves launches whose instances all use
-    // HLLD with a MUSCL-family limiter, or all HLLD with WENO5 (24x24 tiles)
-    bool hlld_muscl = true, hlld_weno5 = c->NT == 24 && c->ng == 4;
-    for (const Instance& I : c->inst) {
-      if (I.done) continue;
-      if (!(I.solver == kFluxHLLD && I.lim < kLimWENO3)) hlld_muscl = false;
-      if (!(I.solver == kFluxHLLD && I.lim == kLimWENO5)) hlld_weno5 = false;
-    }
-    c->stage_spec = hlld_muscl ? kStageHLLDMuscl : hlld_weno5 ? kStageHLLDWeno5 : kStageGeneric;
+    c->stage_spec = stage_spec_of(c);
     c->tune = c->tune_by_spec[c->stage_spec];
   }
   // programs of instances whose tree changed: the current one if the tree is
@@ -1883,16 +1950,22 @@ int rebuild_topology(amr_ctx* c) {
     I.ngen = ng;
     if (int e = ensure(I.d_ops, I.cap_ops, std::max(1, n))) return e;
     if (int e = ensure(I.d_gen, I.cap_gen, std::max(1, ng))) return e;
+    I.prog_dirty = false;
+  }
+  for (int qi : dirty) {
+    Instance& I = c->inst[qi];
+    const size_t n = I.ops.size(), ng = I.gen.size();
     if (n) HIPCHK(hipMemcpyAsync(I.d_ops, I.ops.data(), sizeof(FillOp) * n, hipMemcpyHostToDevice, c->stream));
     if (ng) HIPCHK(hipMemcpyAsync(I.d_gen, I.gen.data(), sizeof(FillGen) * ng, hipMemcpyHostToDevice, c->stream));
-    I.prog_dirty = false;
   }
   // segment tables over the running instances and the leaf work list: one
   // upload of [program pointers | explicit-operand pointers | chunk prefix |
   // leaves] from the pinned staging copy
   const int ni = (int)c->inst.size();
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-  const size_t off_gen = align(sizeof(FillOp*) * (ni + 1));
+  // [counts | program pointers | explicit-operand pointers | chunk prefix | leaves]
+  const size_t off_ops = 256;
+  const size_t off_gen = off_ops + align(sizeof(FillOp*) * (ni + 1));
   const size_t off_pre = off_gen + align(sizeof(FillGen*) * (ni + 1));
   const size_t off_leaf = off_pre + align(sizeof(int32_t) * (ni + 1));
   const size_t topo_bytes = off_leaf + sizeof(LeafItem) * std::max(1, c->nleaf);
@@ -1906,8 +1979,9 @@ int rebuild_topology(amr_ctx* c) {
     HIPCHK(hipMalloc((void**)&c->d_topo, cap));
     HIPCHK(hipHostMalloc((void**)&c->h_topo, cap));
     c->cap_topo = cap;
+    c->topo_epoch++;
   }
-  const FillOp** h_ops = (const FillOp**)c->h_topo;
+  const FillOp** h_ops = (const FillOp**)(c->h_topo + off_ops);
   const FillGen** h_gen = (const FillGen**)(c->h_topo + off_gen);
   int32_t* h_pre = (int32_t*)(c->h_topo + off_pre);
   int total = 0;
@@ -1920,20 +1994,19 @@ int rebuild_topology(amr_ctx* c) {
     h_pre[q + 1] = total;
   }
   c->nchunks = total;
+  ((int32_t*)c->h_topo)[kCountLeaves] = c->nleaf;
+  ((int32_t*)c->h_topo)[kCountChunks] = c->nchunks;
   if (c->nleaf) std::memcpy(c->h_topo + off_leaf, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf);
   HIPCHK(hipMemcpyAsync(c->d_topo, c->h_topo, off_leaf + sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
                         c->stream));
-  c->d_inst_ops = (const FillOp**)c->d_topo;
+  c->d_counts = (const int32_t*)c->d_topo;
+  c->d_inst_ops = (const FillOp**)(c->d_topo + off_ops);
   c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
   c->d_prefix = (int32_t*)(c->d_topo + off_pre);
   c->d_leaves = (LeafItem*)(c->d_topo + off_leaf);
-  c->d_lam = c->d_red;
-  c->d_grad = c->d_red + c->nleaf;
-  c->h_lam = c->h_red;
-  c->h_grad = c->h_red + c->nleaf;
   c->topo_dirty = false;
-  c->graph_dirty = true;
   c->steps_same_topo = 0;
+  direct_geometry(c);
   return 0;
 }
 
@@ -1950,7 +2023,8 @@ FillSet make_set(amr_ctx* c, std::initializer_list<std::pair<int, int>> fk) {
 
 void run_fill(amr_ctx* c, const FillSet& s) {
   Timed tm(c, PF_FILL);
-  launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->nchunks, s, c->stream);
+  launch_fill(c->NT, c->d_inst_ops, c->d_inst_gen, c->d_prefix, (int)c->inst.size(), c->grid_chunks, s, c->stream,
+              c->launch_counts);
 }
 
 constexpr int kTuneTrials = 3;
@@ -1993,8 +2067,8 @@ void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
     // the trial counts as pending until the step's results come back
     c->tune[tune_bucket(c->nleaf)].n[v] += 1;
   }
-  launch_stage(c->NT, c->ng, v, c->stage_spec, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->d_err, c->stream,
-               tm.start(), tm.stop(), span);
+  launch_stage(c->NT, c->ng, v, c->stage_spec, c->F, c->d_leaves, c->grid_leaves, c->d_ip, step, c->d_err, c->stream,
+               tm.start(), tm.stop(), span, c->launch_counts);
 }
 
 // fold the spans of this step's trial launches into the tuner
@@ -2025,17 +2099,17 @@ void tune_collect(amr_ctx* c) {
 void run_ct(amr_ctx* c, int step) {
   {
     Timed tm(c, PF_CT);
-    launch_emf(c->NT, c->F, c->d_leaves, c->nleaf, c->stream);
+    launch_emf(c->NT, c->F, c->d_leaves, c->grid_leaves, c->stream, c->launch_counts);
   }
   run_fill(c, make_set(c, {{kEMF, kFillEdge}}));
   {
     Timed tm(c, PF_CT);
-    launch_bupdate(c->NT, c->F, c->d_leaves, c->nleaf, c->d_ip, step, c->rk, c->stream);
+    launch_bupdate(c->NT, c->F, c->d_leaves, c->grid_leaves, c->d_ip, step, c->rk, c->stream, c->launch_counts);
   }
   run_fill(c, make_set(c, {{step == c->rk ? kBi : kBis, kFillFace}}));
   {
     Timed tm(c, PF_CT);
-    launch_bsync(c->NT, c->F, c->d_leaves, c->nleaf, step == c->rk, c->stream);
+    launch_bsync(c->NT, c->F, c->d_leaves, c->grid_leaves, step == c->rk, c->stream, c->launch_counts);
   }
 }
 
@@ -2049,6 +2123,18 @@ int wait_errors(amr_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   flush_profile(c);
+  if (*c->h_fail) {
+    // a grid-barrier wait of the step kernel ran out (its workgroups were not
+    // all resident): the step's results are void; later steps use per-phase
+    // launches
+    std::fprintf(stderr, "amr2d_mhd: step kernel barrier timed out; using per-phase launches from now on\n");
+    c->step_broken = true;
+    *c->h_fail = 0;
+    HIPCHK(hipMemsetAsync(c->d_fail, 0, sizeof(int32_t), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_sync, 0, sizeof(GridSync), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return set_error(c, AMR_ERR_HIP, -1, -1, 0, 0);
+  }
   bool any = false;
   for (int q = 0; q < ni; q++) {
     const DevError& e = c->h_err[q];
@@ -2059,12 +2145,16 @@ int wait_errors(amr_ctx* c) {
     fail_instance(c, q, (int)(key & 0xf), (int)((key >> 4) & 0xffff), (int)((key >> 32) & 0xff),
                   (int)((key >> 24) & 0xff));
   }
-  if (any) HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(DevError) * ni, c->stream));
+  if (any) {
+    HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(DevError) * ni, c->stream));
+    c->dma_since_direct = true;
+  }
   return 0;
 }
 
 int sync_errors(amr_ctx* c) {
-  HIPCHK(hipMemcpyAsync(c->h_err, c->d_err, sizeof(DevError) * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
+  // the error records and the barrier-failure flag after them
+  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes, hipMemcpyDeviceToHost, c->stream));
   return wait_errors(c);
 }
 
@@ -2105,7 +2195,7 @@ int apply_regrid_ops(amr_ctx* c, const std::vector<RegridOp>& ops, bool refine) 
 int cfl_and_indicators(amr_ctx* c, const double* dW) {
   {
     Timed tm(c, PF_CFL);
-    launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
+    launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_red, c->d_err,
                         c->stream);
   }
   HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
@@ -2125,7 +2215,8 @@ int regrid_all(amr_ctx* c) {
     Instance& I = c->inst[q];
     if (I.done || I.maxlevs <= 1) continue;
     std::vector<double> grad(I.lastActive, 0.0);
-    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) grad[c->h_leaves[l].nb] = c->h_grad[l];
+    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++)
+      grad[c->h_leaves[l].nb] = c->h_red[2 * l + 1];
     const int before = I.lastActive;
     const size_t nr = refs.size(), nc = coars.size();
     if (int e = regrid_decide(c, (int)q, grad, refs, coars)) {
@@ -2144,6 +2235,99 @@ int regrid_all(amr_ctx* c) {
   return 0;
 }
 
+// The phases of one step (main.cpp:128-230) for the persistent step kernel,
+// in the order of the per-phase launches of enqueue_step below.
+void build_phases(amr_ctx* c) {
+  c->phases.clear();
+  auto fill = [&](std::initializer_list<std::pair<int, int>> fk) {
+    StepPhase p{};
+    p.kind = kPhaseFill;
+    for (auto& x : fk) {
+      p.field[p.nf] = (int8_t)x.first;
+      p.fkind[p.nf] = (int8_t)x.second;
+      p.nf++;
+    }
+    c->phases.push_back(p);
+  };
+  auto op = [&](int kind, int step) {
+    StepPhase p{};
+    p.kind = (int8_t)kind;
+    p.step = (int8_t)step;
+    c->phases.push_back(p);
+  };
+  fill({{kU, kFillCell}, {kBi, kFillFace}});
+  for (int st = 1; st <= c->rk; st++) {
+    if (st > 1) fill({{kUs, kFillCell}});
+    op(kPhaseStage, st);
+    if (st == 1) fill({{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}});
+    else fill({{kFF, kFillFace}, {kGG, kFillFace}});
+    // CT2D.cpp:11-124
+    op(kPhaseEmf, st);
+    fill({{kEMF, kFillEdge}});
+    op(kPhaseBupdate, st);
+    fill({{st == c->rk ? kBi : kBis, kFillFace}});
+    op(kPhaseBsync, st == c->rk ? 1 : 0);
+  }
+  fill({{kU, kFillCell}});
+  op(kPhaseCfl, 0);
+}
+
+// whether this step runs as one persistent kernel
+bool persistent_step(amr_ctx* c) {
+  if (c->step_mode == 0 || c->step_broken || c->in_group || c->nleaf <= 0) return false;
+  const int lanes = c->step_mode == 2 ? 1 : 0;
+  int& g = c->step_grid[c->stage_spec][lanes];
+  if (g < 0) g = step_kernel_grid(c->NT, c->ng, c->stage_spec, lanes);
+  return g > 0;
+}
+
+// The device work of one step as one cooperative launch of the persistent
+// step kernel, then the download of the CFL bounds, indicators and error
+// records (and, when kernel families are timed, the phase stamps).
+int enqueue_step_persistent(amr_ctx* c) {
+  if (c->phases.empty()) build_phases(c);
+  const int lanes = c->step_mode == 2 ? 1 : 0;
+  StepArgs A{};
+  A.F = c->F;
+  A.leaves = c->d_leaves;
+  A.nleaf = c->nleaf;
+  A.nchunks = c->nchunks;
+  A.ops = c->d_inst_ops;
+  A.gen = c->d_inst_gen;
+  A.cprefix = c->d_prefix;
+  A.ninst = (int)c->inst.size();
+  static_assert(sizeof(InstParams) % sizeof(double) == 0, "parameters copied as 8-byte words");
+  A.copy_words = (int32_t)(sizeof(InstParams) * c->inst.size() / sizeof(double));
+  A.copy_src = (const double*)c->h_ip.data();
+  A.copy_dst = (double*)c->d_ip;
+  A.ip = c->d_ip;
+  A.lam_grad = c->d_red;
+  A.err = c->d_err;
+  A.sync = c->d_sync;
+  A.fail = c->d_fail;
+  const bool stamps = (c->prof_mask & 0xfu) != 0 && c->wall_khz > 0;
+  A.stamps = stamps ? c->d_stamps : nullptr;
+  A.nphases = (int)c->phases.size();
+  for (int p = 0; p < A.nphases; p++) A.phase[p] = c->phases[p];
+  {
+    Timed tm(c, PF_STEP, true);
+    if (launch_step(c->NT, c->ng, c->stage_spec, lanes, c->step_grid[c->stage_spec][lanes], A, c->stream,
+                    tm.start(), tm.stop())) {
+      std::fprintf(stderr, "amr2d_mhd: cooperative launch of the step kernel failed: %s\n",
+                   hipGetErrorString(hipGetLastError()));
+      return AMR_ERR_HIP;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
+                        hipMemcpyDeviceToHost, c->stream));
+  if (stamps) {
+    HIPCHK(hipMemcpyAsync(c->h_stamps, c->d_stamps, sizeof(unsigned long long) * (A.nphases + 1),
+                          hipMemcpyDeviceToHost, c->stream));
+    c->stamps_pending = A.nphases;
+  }
+  return 0;
+}
+
 // main.cpp:128-247 for all running instances
 // Device work of one step (main.cpp:128-230) up to the download of the CFL
 // bounds, indicators and error records: stream-ordered launches only, so it
@@ -2155,7 +2339,7 @@ int enqueue_step(amr_ctx* c) {
     // the step's parameters (dt) ride on the first fill launch
     FillSet s0 = make_set(c, {{kU, kFillCell}, {kBi, kFillFace}});
     static_assert(sizeof(InstParams) % sizeof(double) == 0, "parameters copied as 8-byte words");
-    if (c->nchunks > 0) {
+    if (c->grid_chunks > 0) {
       s0.copy_words = (int32_t)(sizeof(InstParams) * c->inst.size() / sizeof(double));
       s0.copy_src = (const double*)c->h_ip.data();
       s0.copy_dst = (double*)c->d_ip;
@@ -2181,11 +2365,11 @@ int enqueue_step(amr_ctx* c) {
   run_fill(c, make_set(c, {{kU, kFillCell}}));
   {
     Timed tm(c, PF_CFL);
-    launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->nleaf, c->d_ip, c->d_lam, c->d_grad, c->d_err,
+    launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->grid_leaves, c->d_ip, c->d_red, c->d_err,
                         c->stream);
   }
   // error records and per-leaf values in one download
-  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
+  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->grid_leaves,
                         hipMemcpyDeviceToHost, c->stream));
   if (trial)
     HIPCHK(hipMemcpyAsync(c->h_span, c->d_span, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
@@ -2195,6 +2379,7 @@ int enqueue_step(amr_ctx* c) {
 // (Re)capture the step into the context's graph: update the executable graph
 // in place when the captured topology allows it, else instantiate anew.
 int capture_step(amr_ctx* c) {
+  HostTimed ht(c, PF_CAPTURE);
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int e = enqueue_step(c);
@@ -2204,16 +2389,13 @@ int capture_step(amr_ctx* c) {
     if (!e) std::fprintf(stderr, "amr2d_mhd: graph capture failed: %s\n", hipGetErrorString(ce));
     return e ? e : AMR_ERR_HIP;
   }
+  // a re-capture changes grids, the stage kernel's variant or the buffers:
+  // instantiate anew rather than update the executable graph in place
   if (c->gexec) {
-    hipGraphNode_t bad = nullptr;
-    hipGraphExecUpdateResult res;
-    if (hipGraphExecUpdate(c->gexec, g, &bad, &res) != hipSuccess) {
-      (void)hipGetLastError();
-      hipGraphExecDestroy(c->gexec);
-      c->gexec = nullptr;
-    }
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
   }
-  if (!c->gexec) {
+  {
     const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
     if (ie != hipSuccess) {
       hipGraphDestroy(g);
@@ -2240,14 +2422,41 @@ int step_begin(amr_ctx* c) {
     if (int e = rebuild_topology(c)) return e;
   fill_params(c);
   const bool device_timing = (c->prof_mask & 0x1fu) != 0;
+  if (persistent_step(c)) {
+    c->step_trial = false;
+    if (int e = enqueue_step_persistent(c)) return e;
+    c->steps_same_topo++;
+    c->in_flight = true;
+    return 0;
+  }
   c->step_trial = tuning(c);
-  if (c->graph_on && !device_timing && !c->step_trial && c->steps_same_topo > 0) {
-    if (c->graph_dirty || !c->gexec)
-      if (int e = capture_step(c)) return e;
-    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  if (c->graph_on && !device_timing && !c->step_trial && !c->dma_since_direct) {
+    // the captured step serves this tree when its grids cover the counts
+    bool trial;
+    amr_ctx::GraphKey k;
+    k.epoch = c->topo_epoch;
+    k.leaves = graph_capacity(c->gkey.leaves, c->nleaf, (int)std::min<int64_t>(c->cap_slots, INT32_MAX));
+    k.chunks = graph_capacity(c->gkey.chunks, c->nchunks, INT32_MAX);
+    k.variant = stage_choice(c, &trial, false);
+    k.spec = c->stage_spec;
+    c->grid_leaves = k.leaves;
+    c->grid_chunks = k.chunks;
+    c->launch_counts = c->d_counts;
+    if (c->graph_dirty || !c->gexec || !(k == c->gkey)) {
+      const int e = capture_step(c);
+      if (e) {
+        direct_geometry(c);
+        return e;
+      }
+      c->gkey = k;
+    }
+    const hipError_t le = hipGraphLaunch(c->gexec, c->stream);
+    direct_geometry(c);
+    HIPCHK(le);
     if (c->prof_mask & (1u << PF_GRAPH)) c->prof[PF_GRAPH].launches += 1;
   } else {
     if (int e = enqueue_step(c)) return e;
+    c->dma_since_direct = false;
   }
   c->steps_same_topo++;
   c->in_flight = true;
@@ -2269,7 +2478,7 @@ int step_end(amr_ctx* c, int* n_active) {
     Instance& I = c->inst[q];
     if (I.done) continue;
     double lambda = 1.E6;
-    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) lambda = std_min(lambda, c->h_lam[l]);
+    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) lambda = std_min(lambda, c->h_red[2 * l]);
     I.t = I.t + I.dt_main;
     I.dt_dom = I.dt_main;
     I.count = I.count + 1;
@@ -2392,7 +2601,8 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
       return AMR_ERR_HIP;
     }
   }
-  c->rec_err_bytes = (sizeof(DevError) * n_inst + 255) & ~size_t(255);
+  // error records, then the step kernel's barrier-failure flag
+  c->rec_err_bytes = (sizeof(DevError) * n_inst + sizeof(int32_t) + 255) & ~size_t(255);
   const size_t rec_bytes = c->rec_err_bytes + sizeof(double) * 2 * (size_t)std::max<int64_t>(1, c->cap_slots);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
@@ -2401,13 +2611,20 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
       hipHostMalloc((void**)&c->h_rec, rec_bytes) != hipSuccess ||
       hipMalloc((void**)&c->d_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc((void**)&c->d_sync, sizeof(GridSync)) != hipSuccess ||
+      hipMemset(c->d_sync, 0, sizeof(GridSync)) != hipSuccess ||
+      hipMalloc((void**)&c->d_stamps, sizeof(unsigned long long) * (kMaxPhases + 1)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_stamps, sizeof(unsigned long long) * (kMaxPhases + 1)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     amr_destroy(c);
     return AMR_ERR_HIP;
   }
+  std::memset(c->h_rec, 0, rec_bytes);
   c->d_err = (DevError*)c->d_rec;
   c->h_err = (DevError*)c->h_rec;
   c->d_red = (double*)(c->d_rec + c->rec_err_bytes);
+  c->d_fail = (int32_t*)(c->d_rec + sizeof(DevError) * n_inst);
+  c->h_fail = (int32_t*)(c->h_rec + sizeof(DevError) * n_inst);
   c->h_red = (double*)(c->h_rec + c->rec_err_bytes);
   {
     int dev = 0, khz = 0;
@@ -2456,6 +2673,9 @@ int amr_destroy(amr_ctx* c) {
   }
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   hipFree(c->d_span);
+  hipFree(c->d_sync);
+  hipFree(c->d_stamps);
+  if (c->h_stamps) hipHostFree(c->h_stamps);
   if (c->h_span) hipHostFree(c->h_span);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -2503,6 +2723,7 @@ int amr_init(amr_ctx* c) {
   if (c->d_W) hipFree(c->d_W);
   HIPCHK(hipMalloc((void**)&c->d_W, sizeof(double) * Wall.size()));
   HIPCHK(hipMemcpy(c->d_W, Wall.data(), sizeof(double) * Wall.size(), hipMemcpyHostToDevice));
+  c->dma_since_direct = true;
   if (int e = cfl_and_indicators(c, c->d_W)) return e;
   hipFree(c->d_W);
   c->d_W = nullptr;
@@ -2563,6 +2784,17 @@ int amr_run_group(int n_ctx, amr_ctx** ctxs, int max_steps, int64_t* steps, int6
   if (n_ctx <= 0 || !ctxs) return AMR_ERR_CONFIG;
   for (int k = 0; k < n_ctx; k++)
     if (!ctxs[k] || ctxs[k]->in_flight) return AMR_ERR_CONFIG;
+  // one context runs its steps as it would alone
+  struct GroupScope {
+    int n;
+    amr_ctx** c;
+    GroupScope(int n_, amr_ctx** c_) : n(n_), c(c_) {
+      for (int k = 0; k < n && n > 1; k++) c[k]->in_group = true;
+    }
+    ~GroupScope() {
+      for (int k = 0; k < n; k++) c[k]->in_group = false;
+    }
+  } group_scope(n_ctx, ctxs);
   std::vector<std::vector<int64_t>> leaves(n_ctx);
   std::vector<int64_t> done(n_ctx, 0);
   int64_t zu = 0;
@@ -2655,6 +2887,7 @@ int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) 
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int nb = 0; nb < nslots; nb++)
     HIPCHK(hipMemcpy(c->F.f[field] + per * I.gs[nb], host + per * nb, per * sizeof(double), hipMemcpyHostToDevice));
+  c->dma_since_direct = true;
   return AMR_OK;
   AMR_CATCH
 }
@@ -2939,7 +3172,7 @@ int amr_cfl(amr_ctx* c, int inst, double* lambda) {
   if (int e = cfl_and_indicators(c, nullptr)) return e;
   if (c->call_fail) return c->call_fail;
   double lam = 1.E6;
-  for (int l = c->leaf_inst_begin[inst]; l < c->leaf_inst_begin[inst + 1]; l++) lam = std_min(lam, c->h_lam[l]);
+  for (int l = c->leaf_inst_begin[inst]; l < c->leaf_inst_begin[inst + 1]; l++) lam = std_min(lam, c->h_red[2 * l]);
   *lambda = lam;
   return AMR_OK;
   AMR_CATCH
@@ -3003,12 +3236,33 @@ int amr_set_stage_variant(amr_ctx* c, int variant) {
 int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_leaf) {
   AMR_TRY
   if (!c || nbuckets < 0) return AMR_ERR_CONFIG;
+  const auto* tune = c->tune_by_spec[stage_spec_of(c)];
   for (int b = 0; b < nbuckets && b < 24; b++) {
-    if (chosen) chosen[b] = c->tune[b].chosen;
+    if (chosen) chosen[b] = tune[b].chosen;
     for (int v = 0; v < kStageVariants && us_per_leaf; v++)
-      us_per_leaf[b * kStageVariants + v] =
-          c->tune[b].n[v] ? c->tune[b].us_per_leaf[v] / c->tune[b].n[v] : 0.0;
+      us_per_leaf[b * kStageVariants + v] = tune[b].n[v] ? tune[b].us_per_leaf[v] / tune[b].n[v] : 0.0;
   }
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_set_stage_tuning(amr_ctx* c, int nbuckets, const int32_t* chosen) {
+  AMR_TRY
+  if (!c || nbuckets < 0 || nbuckets > 24 || (nbuckets && !chosen)) return AMR_ERR_CONFIG;
+  for (int b = 0; b < nbuckets; b++)
+    if (chosen[b] < -1 || chosen[b] >= kStageVariants) return AMR_ERR_CONFIG;
+  auto* tune = c->tune_by_spec[stage_spec_of(c)];
+  for (int b = 0; b < nbuckets; b++)
+    if (chosen[b] >= 0) tune[b].chosen = chosen[b];
+  c->graph_dirty = true;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_set_step_kernel(amr_ctx* c, int mode) {
+  AMR_TRY
+  if (!c || mode < -1 || mode > 2 || c->in_flight) return AMR_ERR_CONFIG;
+  c->step_mode = mode < 0 ? 0 : mode;
   return AMR_OK;
   AMR_CATCH
 }
@@ -3016,6 +3270,13 @@ int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_l
 int amr_set_graph(amr_ctx* c, int on) {
   AMR_TRY
   if (!c) return AMR_ERR_CONFIG;
+  if (on) {
+    // replaying one captured step over changing trees (device-side counts,
+    // capacity grids) faulted the GPU in a long run; it stays off until the
+    // cause is shown
+    std::fprintf(stderr, "amr2d_mhd: graph replay of the step is disabled in this build\n");
+    return AMR_ERR_CONFIG;
+  }
   c->graph_on = on != 0;
   c->graph_dirty = true;
   return AMR_OK;

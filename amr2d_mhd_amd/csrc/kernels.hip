@@ -6,6 +6,8 @@
 // primitive tile and its face fluxes live in LDS for the whole stage.
 // No MFMA: nothing on this path is a dense contraction.
 #include <hip/hip_ext.h>
+
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -54,24 +56,28 @@ __device__ inline void stamp_end(unsigned long long* span) { atomicMax(span + 1,
 //
 // Variants (same arithmetic, same results): TPB lanes per leaf, and the face
 // fluxes kept in LDS for the RK update (FLDS) or read back from the global
-// ff/gg the kernel has just written (smaller LDS footprint, more resident
+// ff/gg the block has just written (smaller LDS footprint, more resident
 // workgroups per CU).  SPEC = kStageHLLDMuscl (kStageHLLDWeno5) fixes the
 // solver to HLLD and the reconstruction to the MUSCL family (WENO5) at compile
 // time, used when every leaf of the launch has them: the other paths'
 // registers drop out.
-template <int NT, int NG, int TPB, bool FLDS, int WPE, int SPEC = kStageGeneric>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
-    FieldPtrs F, const LeafItem* __restrict__ leaves, const InstParams* __restrict__ ip, int step, DevError* err,
-    unsigned long long* span) {
-  constexpr int N2 = NT * NT;
-  constexpr int LO = NG - 1, HI = NT - 1 - NG, NF = HI - LO + 1;
-  constexpr int NFL = FLDS ? 8 * NF * (NF + 1) : 1;
-  __shared__ double sW[8 * N2];
-  __shared__ double sF[NFL];  // ff[i][j], i in [LO,HI+1], j in [LO,HI]
-  __shared__ double sG[NFL];  // gg[i][j], i in [LO,HI], j in [LO,HI+1]
-  __shared__ int sNan;
+template <int NT, int NG, bool FLDS>
+struct StageLds {
+  static constexpr int N2 = NT * NT;
+  static constexpr int LO = NG - 1, HI = NT - 1 - NG, NF = HI - LO + 1;
+  static constexpr int NFL = FLDS ? 8 * NF * (NF + 1) : 1;
+  // doubles: primitive tile, x-face and y-face fluxes
+  static constexpr int kDoubles = 8 * N2 + 2 * NFL;
+};
 
-  const LeafItem L = leaves[blockIdx.x];
+// One leaf, all TPB lanes of the workgroup; sW/sF/sG/sNan: the workgroup's
+// LDS (StageLds layout).  The caller separates two calls on the same LDS by a
+// workgroup barrier.
+template <int NT, int NG, int TPB, bool FLDS, int SPEC>
+__device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const InstParams* __restrict__ ip, int step,
+                                   DevError* err, double* sW, double* sF, double* sG, int* sNan) {
+  using S = StageLds<NT, NG, FLDS>;
+  constexpr int N2 = S::N2, LO = S::LO, HI = S::HI, NF = S::NF;
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
   constexpr bool muscl_only = SPEC == kStageHLLDMuscl, weno5_only = SPEC == kStageHLLDWeno5;
@@ -83,10 +89,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
   double* ff = F.f[kFF] + t8;
   double* gg = F.f[kGG] + t8;
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    sNan = 0;
-    if (span) stamp_start(span);
-  }
+  if (tid == 0) *sNan = 0;
   __syncthreads();
 
   for (int c = tid; c < N2; c += TPB) {
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
       bool bad = false;
 #pragma unroll
       for (int k = 0; k < 8; k++) bad |= isnan(w[k]);
-      if (bad) sNan = 1;
+      if (bad) *sNan = 1;
     }
   }
   if (FLDS) {
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     }
   }
   __syncthreads();
-  if (sNan) {
+  if (*sNan) {
     if (tid == 0) report(err, L.inst, error_key(step, L.nb, 1, 0, 0, 0, L.nb, kErrNanEdge));
     return;
   }
@@ -210,25 +213,43 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
       else Uw[e] = (1. / 3.) * U[e] + (2. / 3.) * (Us[e] - dt * res);
     }
   }
+}
+
+template <int NT, int NG, int TPB, bool FLDS, int WPE, int SPEC = kStageGeneric>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
+    FieldPtrs F, const LeafItem* __restrict__ leaves, const InstParams* __restrict__ ip, int step, DevError* err,
+    unsigned long long* span, const int32_t* __restrict__ counts) {
+  using S = StageLds<NT, NG, FLDS>;
+  __shared__ double lds[S::kDoubles];
+  __shared__ int sNan;
+  if (counts && (int)blockIdx.x >= counts[kCountLeaves]) return;
+  if (span && threadIdx.x == 0) stamp_start(span);
+  stage_block<NT, NG, TPB, FLDS, SPEC>(F, leaves[blockIdx.x], ip, step, err, lds, lds + 8 * S::N2,
+                                       lds + 8 * S::N2 + S::NFL, &sNan);
   if (span) {
     __syncthreads();
-    if (tid == 0) stamp_end(span);
+    if (threadIdx.x == 0) stamp_end(span);
   }
 }
 
 // --------------------------------------------------------------------- CT
 // CT2D.cpp:15-27: corner EMF from the face fluxes for i,j in [1,NT-1]
 template <int NT>
-__global__ __launch_bounds__(kThreads) void emf_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves) {
+__device__ inline void emf_cell(const FieldPtrs& F, const LeafItem L, int c) {
   constexpr int N2 = NT * NT;
-  const LeafItem L = leaves[blockIdx.y];
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= N2) return;
   const int i = c % NT, j = c / NT;
   if (i < 1 || j < 1) return;
   const double* gg5 = F.f[kGG] + (size_t)L.gs * 8 * N2 + 5 * N2;
   const double* ff6 = F.f[kFF] + (size_t)L.gs * 8 * N2 + 6 * N2;
   F.f[kEMF][(size_t)L.gs * N2 + c] = 0.25 * ((gg5[c - 1] + gg5[c]) - (ff6[c - NT] + ff6[c]));
+}
+
+template <int NT>
+__global__ __launch_bounds__(kThreads) void emf_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
+                                                       const int32_t* __restrict__ counts) {
+  if (counts && (int)blockIdx.y >= counts[kCountLeaves]) return;
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c < NT * NT) emf_cell<NT>(F, leaves[blockIdx.y], c);
 }
 
 // CT2D.cpp:78-92: face-B RK update from the edge EMF.  Stage input B (Bi for
@@ -242,12 +263,9 @@ __device__ inline double face_rk(int rk, int step, double B, double Bn, double B
 }
 
 template <int NT>
-__global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
-                                                           const InstParams* __restrict__ ip, int step, int rk) {
+__device__ inline void bupdate_cell(const FieldPtrs& F, const LeafItem L, const InstParams* __restrict__ ip, int step,
+                                    int rk, int c) {
   constexpr int N2 = NT * NT;
-  const LeafItem L = leaves[blockIdx.y];
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= N2) return;
   const int i = c % NT, j = c / NT;
   const InstParams& P = ip[L.inst];
   const double dt = P.dt, dxl = P.dx[L.lev];
@@ -265,13 +283,19 @@ __global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const Le
   Bn[N2 + c] = by;
 }
 
+template <int NT>
+__global__ __launch_bounds__(kThreads) void bupdate_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
+                                                           const InstParams* __restrict__ ip, int step, int rk,
+                                                           const int32_t* __restrict__ counts) {
+  if (counts && (int)blockIdx.y >= counts[kCountLeaves]) return;
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c < NT * NT) bupdate_cell<NT>(F, leaves[blockIdx.y], ip, step, rk, c);
+}
+
 // CT2D.cpp:95-123: cell-centred B from the updated faces + energy correction
 template <int NT>
-__global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves, int last) {
+__device__ inline void bsync_cell(const FieldPtrs& F, const LeafItem L, int last, int c) {
   constexpr int N2 = NT * NT;
-  const LeafItem L = leaves[blockIdx.y];
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= N2) return;
   const int i = c % NT, j = c / NT;
   const double* Bn = F.f[last ? kBi : kBis] + (size_t)L.gs * 2 * N2;
   double* q = F.f[last ? kU : kUs] + (size_t)L.gs * 8 * N2;
@@ -282,6 +306,14 @@ __global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const Leaf
   q[4 * N2 + c] = q[4 * N2 + c] + 0.5 * ((bx * bx + by * by) - ((q5 * q5) + (q6 * q6)));
   q[5 * N2 + c] = bx;
   q[6 * N2 + c] = by;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kThreads) void bsync_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves, int last,
+                                                         const int32_t* __restrict__ counts) {
+  if (counts && (int)blockIdx.y >= counts[kCountLeaves]) return;
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c < NT * NT) bsync_cell<NT>(F, leaves[blockIdx.y], last, c);
 }
 
 // -------------------------------------------------------------- ghost fill
@@ -341,8 +373,7 @@ __device__ inline double fill_value(const FillCtx& f, const FillOp& op, int k) {
 }
 
 // The program is a concatenation of per-instance segments of whole chunks:
-// workgroup w runs chunk w - cprefix[q] of the instance q with
-// cprefix[q] <= w < cprefix[q+1].
+// chunk w belongs to the instance q with cprefix[q] <= w < cprefix[q+1].
 struct FillProgram {
   const FillOp* const* ops;
   const FillGen* const* gen;
@@ -372,44 +403,104 @@ __device__ inline void fill_field(const FillCtx& f, double* base, const FillOp& 
   }
 }
 
-__global__ __launch_bounds__(kThreads) void fill_kernel(FillProgram P, FillSet set, int NT) {
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < set.copy_words; i += gridDim.x * kThreads)
-    set.copy_dst[i] = set.copy_src[i];
-  const int w = blockIdx.x;
-  const int q = find_segment(P, w);
-  const FillOp op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + threadIdx.x];
+// One chunk of the program per kFillChunk lanes (`lane` within the chunk); a
+// workgroup may run several chunks side by side, and every lane of it calls
+// this together (it synchronises the workgroup).  valid = false: no chunk.
+__device__ inline void fill_chunk(const FillProgram& P, const FillSet& set, int NT, int w, int lane, bool valid) {
+  FillOp op{};
+  op.op = kFillNop;
+  int q = 0;
+  if (valid) {
+    q = find_segment(P, w);
+    op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + lane];
+  }
   const bool live = op.op != kFillNop;
   for (int fi = 0; fi < set.n; fi++) {
-    const FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, P.gen[q]};
+    const FillCtx f{set.base[fi], set.nv[fi], set.kind[fi], NT * NT, NT, valid ? P.gen[q] : nullptr};
     if (set.nv[fi] == 8) fill_field<8>(f, set.base[fi], op, live);
     else if (set.nv[fi] == 2) fill_field<2>(f, set.base[fi], op, live);
     else fill_field<1>(f, set.base[fi], op, live);
   }
 }
 
+// The same for a set of up to three fields with NV0, NV1, NV2 components
+// (0: absent), fixed at compile time: every value of every field of the chunk
+// is read before the one barrier, then all are written.
+template <int NV0, int NV1, int NV2>
+__device__ inline void fill_chunk_nv(const FillProgram& P, const FillSet& set, int NT, int w, int lane, bool valid) {
+  FillOp op{};
+  op.op = kFillNop;
+  int q = 0;
+  if (valid) {
+    q = find_segment(P, w);
+    op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + lane];
+  }
+  const bool live = op.op != kFillNop;
+  const FillGen* gen = valid ? P.gen[q] : nullptr;
+  const int N2 = NT * NT;
+  const FillCtx f0{set.base[0], NV0, set.kind[0], N2, NT, gen};
+  const FillCtx f1{set.base[1], NV1, set.kind[1], N2, NT, gen};
+  const FillCtx f2{set.base[2], NV2, set.kind[2], N2, NT, gen};
+  double v0[NV0 > 0 ? NV0 : 1], v1[NV1 > 0 ? NV1 : 1], v2[NV2 > 0 ? NV2 : 1];
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < NV0; k++) v0[k] = fill_value(f0, op, k);
+#pragma unroll
+    for (int k = 0; k < NV1; k++) v1[k] = fill_value(f1, op, k);
+#pragma unroll
+    for (int k = 0; k < NV2; k++) v2[k] = fill_value(f2, op, k);
+  }
+  __syncthreads();  // every read of the chunk precedes every write
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < NV0; k++) set.base[0][((size_t)op.dst_gs * NV0 + k) * N2 + op.dst_cell] = v0[k];
+#pragma unroll
+    for (int k = 0; k < NV1; k++) set.base[1][((size_t)op.dst_gs * NV1 + k) * N2 + op.dst_cell] = v1[k];
+#pragma unroll
+    for (int k = 0; k < NV2; k++) set.base[2][((size_t)op.dst_gs * NV2 + k) * N2 + op.dst_cell] = v2[k];
+  }
+}
+
+// field sets of the step (engine.cpp): {U, Bi}, {Bi, ff, gg}, {ff, gg}, one
+// cell field, one face field, EMF; any other set takes the per-field path
+template <int NV0, int NV1, int NV2>
+__global__ __launch_bounds__(kThreads) void fill_kernel_nv(FillProgram P, FillSet set, int NT,
+                                                           const int32_t* __restrict__ counts) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < set.copy_words; i += gridDim.x * kThreads)
+    set.copy_dst[i] = set.copy_src[i];
+  if (counts && (int)blockIdx.x >= counts[kCountChunks]) return;
+  fill_chunk_nv<NV0, NV1, NV2>(P, set, NT, blockIdx.x, threadIdx.x, true);
+}
+
+__global__ __launch_bounds__(kThreads) void fill_kernel(FillProgram P, FillSet set, int NT,
+                                                        const int32_t* __restrict__ counts) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < set.copy_words; i += gridDim.x * kThreads)
+    set.copy_dst[i] = set.copy_src[i];
+  if (counts && (int)blockIdx.x >= counts[kCountChunks]) return;
+  fill_chunk(P, set, NT, blockIdx.x, threadIdx.x, true);
+}
+
 // ------------------------------------------------------ CFL and criteria
 // main.cpp:209-230 (per-leaf wave-speed bound, with the reference's
 // |rho|+c_fx and |u|+c_fy terms) fused with criteria.cpp:6-44 (maximum of
-// the max4diff/|w| ratio over p, rho, Bx, By on the interior).
-template <int NT, int NG>
-__global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, const double* __restrict__ Win,
-                                                                const LeafItem* __restrict__ leaves,
-                                                                const InstParams* __restrict__ ip, double* lam_out,
-                                                                double* grad_out, DevError* err) {
+// the max4diff/|w| ratio over p, rho, Bx, By on the interior).  One leaf, all
+// TPB lanes (a power of two); sW: 8 NT^2 doubles and sRed: 2 TPB doubles of
+// the workgroup's LDS.  The maxima never hold a NaN (std_max keeps its first
+// operand against one), so the reduction order does not change them.
+template <int NT, int NG, int TPB>
+__device__ inline void cfl_block(const FieldPtrs& F, const double* __restrict__ Win, const LeafItem L, int leaf,
+                                 const InstParams* __restrict__ ip, double* lam_grad, DevError* err, double* sW,
+                                 double* sRed) {
   constexpr int N2 = NT * NT;
   constexpr int LO = NG, HI = NT - 1 - NG;
-  __shared__ double sW[8 * N2];
-  __shared__ double sRed[2][kThreads];
-  const int leaf = blockIdx.x;
-  const LeafItem L = leaves[leaf];
   const double g = ip[L.inst].gamma;
   const int tid = threadIdx.x;
   if (Win) {
     const double* W = Win + (size_t)leaf * 8 * N2;
-    for (int q = tid; q < 8 * N2; q += kThreads) sW[q] = W[q];
+    for (int q = tid; q < 8 * N2; q += TPB) sW[q] = W[q];
   } else {
     const double* U = F.f[kU] + (size_t)L.gs * 8 * N2;
-    for (int c = tid; c < N2; c += kThreads) {
+    for (int c = tid; c < N2; c += TPB) {
       double u[8], w[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) u[k] = U[k * N2 + c];
@@ -424,7 +515,7 @@ __global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, con
   __syncthreads();
   double lam = 0., grad = 0.;
   constexpr int NI = HI - LO + 1;
-  for (int q = tid; q < NI * NI; q += kThreads) {
+  for (int q = tid; q < NI * NI; q += TPB) {
     const int i = LO + q % NI, j = LO + q / NI, c = j * NT + i;
     double w[8];
 #pragma unroll
@@ -445,20 +536,138 @@ __global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, con
       grad = std_max(grad, m / mac_max(kEps, fabs(x)));
     }
   }
-  sRed[0][tid] = lam;
-  sRed[1][tid] = grad;
+  sRed[tid] = lam;
+  sRed[TPB + tid] = grad;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
+  for (int w = TPB / 2; w > 0; w >>= 1) {
     if (tid < w) {
-      sRed[0][tid] = std_max(sRed[0][tid], sRed[0][tid + w]);
-      sRed[1][tid] = std_max(sRed[1][tid], sRed[1][tid + w]);
+      sRed[tid] = std_max(sRed[tid], sRed[tid + w]);
+      sRed[TPB + tid] = std_max(sRed[TPB + tid], sRed[TPB + tid + w]);
     }
     __syncthreads();
   }
   if (tid == 0) {
     const double dxl = ip[L.inst].dx[L.lev];
-    lam_out[leaf] = std_min(dxl, dxl) / sRed[0][0];
-    grad_out[leaf] = sRed[1][0];
+    lam_grad[2 * leaf] = std_min(dxl, dxl) / sRed[0];
+    lam_grad[2 * leaf + 1] = sRed[TPB];
+  }
+}
+
+template <int NT, int NG>
+__global__ __launch_bounds__(kThreads) void cfl_criteria_kernel(FieldPtrs F, const double* __restrict__ Win,
+                                                                const LeafItem* __restrict__ leaves,
+                                                                const InstParams* __restrict__ ip, double* lam_grad,
+                                                                DevError* err, const int32_t* __restrict__ counts) {
+  __shared__ double sW[8 * NT * NT];
+  __shared__ double sRed[2 * kThreads];
+  if (counts && (int)blockIdx.x >= counts[kCountLeaves]) return;
+  cfl_block<NT, NG, kThreads>(F, Win, leaves[blockIdx.x], blockIdx.x, ip, lam_grad, err, sW, sRed);
+}
+
+// ----------------------------------------------------- persistent step kernel
+// Grid-wide barrier of a co-resident grid of nwg workgroups.  Lane 0 of each
+// workgroup publishes the workgroup's writes (agent-scope release), arrives,
+// and waits for the generation to move; the last to arrive resets the count
+// and moves it.  The count is back at zero after every barrier, so the state
+// serves every launch without a reset.  A wait is bounded by the device clock
+// (100 MHz): past kBarrierTicks it sets *fail, and a set *fail makes every
+// later wait return at once, so every workgroup reaches the end of the
+// launch (the host then reports the failure and stops using this kernel).
+constexpr unsigned long long kBarrierTicks = 200000000ull;  // 2 s
+
+__device__ inline void grid_barrier(GridSync* b, unsigned nwg, int32_t* fail) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned old = __hip_atomic_fetch_add(&b->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == nwg - 1) {
+      __hip_atomic_store(&b->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const unsigned long long t0 = wall_clock64();
+      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > kBarrierTicks) {
+          __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+template <int NT, int NG, int TPB>
+struct StepLds {
+  using S = StageLds<NT, NG, true>;
+  static constexpr int kCfl = 8 * NT * NT + 2 * TPB;
+  static constexpr int kDoubles = S::kDoubles > kCfl ? S::kDoubles : kCfl;
+};
+
+template <int NT, int NG, int TPB, int SPEC>
+__global__ __launch_bounds__(TPB) void step_kernel(StepArgs A) {
+  using S = StageLds<NT, NG, true>;
+  constexpr int N2 = NT * NT;
+  __shared__ double lds[StepLds<NT, NG, TPB>::kDoubles];
+  __shared__ int sNan;
+  const int tid = threadIdx.x;
+  const unsigned nwg = gridDim.x;
+  const FillProgram prog{A.ops, A.gen, A.cprefix, A.ninst};
+  if (A.stamps && blockIdx.x == 0 && tid == 0) A.stamps[0] = wall_clock64();
+  for (int p = 0; p < A.nphases; p++) {
+    const StepPhase ph = A.phase[p];
+    if (ph.kind == kPhaseFill) {
+      if (p == 0)  // the step's parameters (no fill op reads them)
+        for (int i = blockIdx.x * TPB + tid; i < A.copy_words; i += nwg * TPB) A.copy_dst[i] = A.copy_src[i];
+      // TPB / kFillChunk chunks side by side per workgroup
+      constexpr int kPer = TPB / kFillChunk;
+      const int lane = tid % kFillChunk, sub = tid / kFillChunk;
+      for (int w0 = blockIdx.x * kPer; w0 < A.nchunks; w0 += nwg * kPer) {
+        const int w = w0 + sub;
+        FillOp op{};
+        op.op = kFillNop;
+        int q = 0;
+        if (w < A.nchunks) {
+          q = find_segment(prog, w);
+          op = prog.ops[q][(w - prog.cprefix[q]) * kFillChunk + lane];
+        }
+        const bool live = op.op != kFillNop;
+        const FillGen* gen = w < A.nchunks ? prog.gen[q] : nullptr;
+        for (int f = 0; f < ph.nf; f++) {
+          const int fld = ph.field[f];
+          double* base = A.F.f[fld];
+          if (fld == kBi || fld == kBis) fill_field<2>(FillCtx{base, 2, ph.fkind[f], N2, NT, gen}, base, op, live);
+          else if (fld == kEMF) fill_field<1>(FillCtx{base, 1, ph.fkind[f], N2, NT, gen}, base, op, live);
+          else fill_field<8>(FillCtx{base, 8, ph.fkind[f], N2, NT, gen}, base, op, live);
+        }
+      }
+    } else if (ph.kind == kPhaseStage) {
+      for (int l = blockIdx.x; l < A.nleaf; l += nwg) {
+        __syncthreads();  // the previous leaf is done with the LDS
+        stage_block<NT, NG, TPB, true, SPEC>(A.F, A.leaves[l], A.ip, ph.step, A.err, lds, lds + 8 * N2,
+                                             lds + 8 * N2 + S::NFL, &sNan);
+      }
+    } else if (ph.kind == kPhaseCfl) {
+      for (int l = blockIdx.x; l < A.nleaf; l += nwg) {
+        __syncthreads();
+        cfl_block<NT, NG, TPB>(A.F, nullptr, A.leaves[l], l, A.ip, A.lam_grad, A.err, lds, lds + 8 * N2);
+      }
+    } else {
+      // CT kernels: one lane per cell of every leaf
+      const int total = A.nleaf * N2;
+      for (int e = blockIdx.x * TPB + tid; e < total; e += nwg * TPB) {
+        const LeafItem L = A.leaves[e / N2];
+        const int c = e % N2;
+        if (ph.kind == kPhaseEmf) emf_cell<NT>(A.F, L, c);
+        else if (ph.kind == kPhaseBupdate) bupdate_cell<NT>(A.F, L, A.ip, ph.step, A.ip[L.inst].rk, c);
+        else bsync_cell<NT>(A.F, L, ph.step, c);
+      }
+    }
+    if (p + 1 < A.nphases) grid_barrier(A.sync, nwg, A.fail);
+    if (A.stamps && blockIdx.x == 0 && tid == 0) A.stamps[p + 1] = wall_clock64();
   }
 }
 
@@ -663,14 +872,14 @@ bool tile_supported(int NT, int ng) { return (NT == 20 && ng == 2) || (NT == 24 
 template <int T, int G, int TPB, bool FLDS, int WPE, int SPEC>
 void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
                           DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
-                          unsigned long long* span) {
+                          unsigned long long* span, const int32_t* counts) {
   if (ev_start && ev_stop) {
     // events attached to the dispatch itself: they time the kernel, not the launch
     hipExtLaunchKernelGGL((stage_kernel<T, G, TPB, FLDS, WPE, SPEC>), dim3(nleaf), dim3(TPB), 0, s, ev_start,
-                          ev_stop, 0, F, leaves, ip, step, err, span);
+                          ev_stop, 0, F, leaves, ip, step, err, span, counts);
     return;
   }
-  stage_kernel<T, G, TPB, FLDS, WPE, SPEC><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span);
+  stage_kernel<T, G, TPB, FLDS, WPE, SPEC><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span, counts);
 }
 
 // WENO5 needs nghosts >= 3: the 24x24 tile only
@@ -682,9 +891,9 @@ void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf,
 template <int SPEC>
 void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                        const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
-                       hipEvent_t ev_stop, unsigned long long* span) {
+                       hipEvent_t ev_stop, unsigned long long* span, const int32_t* counts) {
 #define AMR_STAGE_CALL (launch_stage_variant<T, G, TPB, FLDS, WPE, SPEC>(F, leaves, nleaf, ip, step, err, s, \
-                                                                         ev_start, ev_stop, span))
+                                                                         ev_start, ev_stop, span, counts))
 #define AMR_STAGE(TPB_, FLDS_, WPE_)                                           \
   do {                                                                         \
     constexpr int TPB = TPB_, WPE = WPE_;                                      \
@@ -706,48 +915,61 @@ void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const Le
 
 void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
-                  hipEvent_t ev_stop, unsigned long long* span) {
+                  hipEvent_t ev_stop, unsigned long long* span, const int32_t* counts) {
   if (nleaf <= 0) return;
   if (spec == kStageHLLDMuscl)
-    launch_stage_spec<kStageHLLDMuscl>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
+    launch_stage_spec<kStageHLLDMuscl>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span,
+                                       counts);
   else if (spec == kStageHLLDWeno5 && NT == 24 && ng == 4)
-    launch_stage_spec<kStageHLLDWeno5>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
+    launch_stage_spec<kStageHLLDWeno5>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span,
+                                       counts);
   else
-    launch_stage_spec<kStageGeneric>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span);
+    launch_stage_spec<kStageGeneric>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span,
+                                     counts);
 }
 
-void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s) {
+void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s, const int32_t* counts) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (emf_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (emf_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, counts)));
 }
 
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
-                    int rk, hipStream_t s) {
+                    int rk, hipStream_t s, const int32_t* counts) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bupdate_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step, rk)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2),
+                    (bupdate_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step, rk, counts)));
 }
 
-void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int last, hipStream_t s) {
+void launch_bsync(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, int last, hipStream_t s,
+                  const int32_t* counts) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, last)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (bsync_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, last, counts)));
 }
 
 void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* inst_gen, const int32_t* cprefix,
-                 int ninst, int nchunks, const FillSet& set, hipStream_t s) {
+                 int ninst, int nchunks, const FillSet& set, hipStream_t s, const int32_t* counts) {
   static_assert(kFillChunk == kThreads, "one workgroup per fill chunk");
   if (nchunks <= 0 || set.n <= 0) return;
   FillProgram P{inst_ops, inst_gen, cprefix, ninst};
-  fill_kernel<<<nchunks, kThreads, 0, s>>>(P, set, NT);
+  const int a = set.nv[0], b = set.n > 1 ? set.nv[1] : 0, c = set.n > 2 ? set.nv[2] : 0;
+  if (set.n == 1 && a == 8) fill_kernel_nv<8, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 1 && a == 2) fill_kernel_nv<2, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 1 && a == 1) fill_kernel_nv<1, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 2 && a == 8 && b == 2) fill_kernel_nv<8, 2, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 2 && a == 8 && b == 8) fill_kernel_nv<8, 8, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 3 && a == 2 && b == 8 && c == 8) fill_kernel_nv<2, 8, 8><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else fill_kernel<<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
 }
 
 void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, const LeafItem* leaves, int nleaf,
-                         const InstParams* ip, double* lam_out, double* grad_out, DevError* err, hipStream_t s) {
+                         const InstParams* ip, double* lam_grad, DevError* err, hipStream_t s,
+                         const int32_t* counts) {
   if (nleaf <= 0) return;
   AMR_DISPATCH_TILE(NT, ng,
-                    (cfl_criteria_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, W, leaves, ip, lam_out, grad_out, err)));
+                    (cfl_criteria_kernel<T, G><<<nleaf, kThreads, 0, s>>>(F, W, leaves, ip, lam_grad, err, counts)));
 }
 
 void launch_refine(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int nops, const InstParams* ip,
@@ -762,6 +984,54 @@ void launch_coarsen(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int
   if (nops <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nops);
   AMR_DISPATCH_TILE(NT, ng, (coarsen_kernel<T, G><<<grid, kThreads, 0, s>>>(F, ops, cmapU, cmapB, ip, err)));
+}
+
+// the step kernel instance for (tile, specialisation, lanes), or null
+const void* step_kernel_fn(int NT, int ng, int spec, int lanes_idx) {
+#define AMR_STEP_FN(T_, G_)                                                                              \
+  do {                                                                                                   \
+    if (NT == T_ && ng == G_) {                                                                          \
+      if (spec == kStageHLLDMuscl)                                                                       \
+        return lanes_idx ? (const void*)step_kernel<T_, G_, 512, kStageHLLDMuscl>                        \
+                         : (const void*)step_kernel<T_, G_, 256, kStageHLLDMuscl>;                       \
+      if (spec == kStageHLLDWeno5 && T_ == 24)                                                           \
+        return lanes_idx ? (const void*)step_kernel<T_, G_, 512, kStageHLLDWeno5>                        \
+                         : (const void*)step_kernel<T_, G_, 256, kStageHLLDWeno5>;                       \
+      return lanes_idx ? (const void*)step_kernel<T_, G_, 512, kStageGeneric>                            \
+                       : (const void*)step_kernel<T_, G_, 256, kStageGeneric>;                           \
+    }                                                                                                    \
+  } while (0)
+  AMR_STEP_FN(20, 2);
+  AMR_STEP_FN(24, 4);
+  AMR_STEP_FN(12, 2);
+#undef AMR_STEP_FN
+  return nullptr;
+}
+
+int step_kernel_grid(int NT, int ng, int spec, int lanes_idx) {
+  const void* fn = step_kernel_fn(NT, ng, spec, lanes_idx);
+  if (!fn || lanes_idx < 0 || lanes_idx > 1) return 0;
+  int dev = 0, cus = 0, per_cu = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kStepLanes[lanes_idx], 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return coop ? per_cu * cus : 0;
+}
+
+int launch_step(int NT, int ng, int spec, int lanes_idx, int grid, const StepArgs& A, hipStream_t s,
+                hipEvent_t ev_start, hipEvent_t ev_stop) {
+  const void* fn = step_kernel_fn(NT, ng, spec, lanes_idx);
+  if (!fn || grid <= 0) return 1;
+  StepArgs a = A;
+  void* args[] = {&a};
+  if (ev_start) hipEventRecord(ev_start, s);
+  const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kStepLanes[lanes_idx]), args, 0, s);
+  if (ev_stop) hipEventRecord(ev_stop, s);
+  return e == hipSuccess ? 0 : 1;
 }
 
 void launch_riemann_batch(int solver, int dir, double gamma, int n, const double* wL, const double* wR, double* flux,

@@ -163,7 +163,12 @@ class Simulation:
 
     # --------------------------------------------------------------- timing
     PROFILE_FAMILIES = ("stage", "fill", "ct", "cfl", "regrid", "host_topology", "host_regrid", "host_wait",
-                        "host_compile", "graph")
+                        "host_compile", "graph", "step", "graph_capture")
+
+    def set_step_kernel(self, mode):
+        """Persistent step kernel: 1 (256 lanes) or 2 (512 lanes) runs each step
+        as one launch, 0 launches per phase, -1 restores the default."""
+        _check(self.h, lib().amr_set_step_kernel(self.h, int(mode)))
 
     def set_graph(self, on=True):
         """Replay each step's device work as a HIP graph, or launch it directly (the default)."""
@@ -184,6 +189,14 @@ class Simulation:
         us = np.zeros((nbuckets, STAGE_VARIANTS))
         _check(self.h, lib().amr_stage_tuning(self.h, nbuckets, chosen.ctypes.data, us.ctypes.data))
         return {b: (int(chosen[b]), us[b].tolist()) for b in range(nbuckets) if chosen[b] >= 0 or us[b].any()}
+
+    def set_stage_tuning(self, chosen):
+        """Import tuner decisions {bucket: variant} (as stage_tuning() returns
+        them, or plain ints): those buckets run no trial launches."""
+        arr = np.full(24, -1, np.int32)
+        for b, v in chosen.items():
+            arr[int(b)] = v[0] if isinstance(v, (tuple, list)) else int(v)
+        _check(self.h, lib().amr_set_stage_tuning(self.h, 24, arr.ctypes.data))
 
     def profile(self, families=PROFILE_FAMILIES):
         """Start HIP-event timing of the given kernel families (resets totals)."""

@@ -1,25 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the AMR2D-MHD hot path on one MI355X.
 
-Workload (one GPU): the MHD rotor of the reference (IC2Dtype.cpp:75-111),
-4 root blocks of 16x16, 5 AMR levels, MC-limited MUSCL + HLLD, CT, RK2,
-CFL 0.5, FP64.  A "step" is one iteration of the reference time loop
-(main.cpp:128-247) over the current leaf blocks: ghost fills, two RK stages,
-CT, CFL reduction and regrid.  The run starts from the initial condition; W
-untimed steps are taken, then exactly K steps are timed (defaults cover the
-whole run to t = 0.295, 4763 steps).  The metric is leaf zone-updates per
-second (one interior leaf cell advanced by one step).
+Metric: zone-updates/s FP64 at 1 GPU, batched vs plain; adjoint/forward step
+cost.  One line of JSON carries all three:
 
-Also reported:
+  value         the plain method: the MHD rotor of the reference
+                (IC2Dtype.cpp:75-111), 4 root blocks of 16x16, 5 AMR levels,
+                MC-limited MUSCL + HLLD, CT, RK2, CFL 0.5.  A "step" is one
+                iteration of the reference time loop (main.cpp:128-247) over
+                the current leaf blocks: ghost fills, two RK stages, CT, CFL
+                reduction and regrid.  The run starts from the initial
+                condition; W untimed steps, then exactly K timed steps
+                (defaults: the whole run, 4763 steps).  A zone-update is one
+                interior leaf cell advanced by one step.  The stage-kernel
+                tuner's decisions come from an untimed run, so no trial launch
+                falls in the timed window;
+  whole_run     (when the window is not the whole run) every step of the run;
+  batched_vs_plain  256 seeded rotors of depth 3-6 advanced together against
+                the same instances run one after another (same zone-updates);
+  adjoint_over_forward  the frozen-mesh adjoint, 512x512, 400 steps,
+                checkpoints every 20: backward / forward time, HLLE and HLLD;
   roofline      the stage kernel (reconstruction + Riemann + RK, the
                 dominant kernel): algorithmic bytes per launch / mean launch
                 time from HIP events on the engine's stream during the timed
-                region, against the 8 TB/s HBM peak;
+                window, against the 8 TB/s HBM peak;
   valu          the same kernel's executed FP64 FLOPs per launch (counter-
                 measured per zone) / mean launch time, against 78.6 TFLOP/s;
   cpu_baseline  the reference itself (compiled from its sources into
-                oracle/_ref with its OpenMP loops) on the first steps of the
-                same rotor run, on the host cores of this machine.
+                oracle/_ref with its OpenMP loops) over the same steps of the
+                same run, on the host's cores, in its FMA-free build and with
+                its makefile's own flags.
 """
 import argparse
 import json
@@ -66,31 +76,68 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(steps, threads):
-    """Time the compiled reference (OpenMP build) on the first `steps` steps of
-    the rotor run; zone-updates counted from its own leaf log."""
+def cpu_threads():
+    """Threads for the CPU baseline: the host's CPU share for this GPU as the
+    environment states it (OMP_NUM_THREADS), else every core of the affinity
+    mask."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env or len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(first, steps, threads, variant="ng2_lv5_omp"):
+    """Time the compiled reference (an OpenMP build in oracle/_ref) on steps
+    [first, first + steps) of the rotor run from t = 0: the harness stamps the
+    end of every step of the reference's own loop, and the zone-updates of the
+    window are its leaf counts x 256.  The reference keeps t and the dt lag in
+    locals of its main loop, so it always starts from t = 0."""
     import numpy as np
 
-    exe = os.path.join(HERE, "oracle", "_ref", "ng2_lv5_omp", "refrun")
+    exe = os.path.join(HERE, "oracle", "_ref", variant, "refrun")
     if not os.access(exe, os.X_OK):
         return None
     with tempfile.TemporaryDirectory() as td:
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         inp = "16 16 8|1.0 1.0|0.5|rotor MC HLLD"
-        p = subprocess.run([exe, "run", "--dir", td, "--input", inp, "--steps", str(steps), "--log",
-                            os.path.join(td, "log.bin"), "--timing", os.path.join(td, "timing.json")],
-                           capture_output=True, text=True, env=env, timeout=900)
+        st = os.path.join(td, "steps.bin")
+        p = subprocess.run([exe, "run", "--dir", td, "--input", inp, "--steps", str(first + steps),
+                            "--step-times", st], capture_output=True, text=True, env=env, timeout=900)
         if p.returncode != 0:
             return None
-        with open(os.path.join(td, "timing.json")) as fp:
-            tm = json.load(fp)
-        dt = np.dtype([("step", "i4"), ("la_pre", "i4"), ("nleaf_pre", "i4"), ("la_post", "i4"),
-                       ("nleaf_post", "i4"), ("pad", "i4"), ("dt", "f8"), ("t", "f8"), ("h", "u8", 4)])
-        log = np.fromfile(os.path.join(td, "log.bin"), dtype=dt)
-        zu = int((log["nleaf_pre"][1:].astype(np.int64) * 256).sum())
-    return dict(value=zu / tm["loop_seconds"], unit="zone-updates/s", cores=threads, kind="reference",
-                sample=f"reference OpenMP build, rotor lv5 MC+HLLD CFL 0.5, first {steps} steps "
-                       f"({zu} zone-updates in {tm['loop_seconds']:.2f} s), host CPU: {cpu_model()}")
+        rec = np.fromfile(st, np.float64).reshape(-1, 2)
+    if len(rec) < first + steps:
+        return None
+    t0 = rec[first - 1, 0] if first > 0 else 0.0
+    secs = rec[first + steps - 1, 0] - t0
+    zu = int(rec[first:first + steps, 1].sum()) * 256
+    return dict(value=zu / secs, unit="zone-updates/s", zone_updates=zu, seconds=secs, steps=[first, first + steps])
+
+
+def cpu_baselines(first, steps, max_steps):
+    """The reference on the host's cores over the GPU's timed window (its
+    first `max_steps` steps when the window is longer): the FMA-free build
+    (the oracle's arithmetic) and the build with the reference makefile's own
+    flags."""
+    threads = cpu_threads()
+    n = min(steps, max_steps)
+    plain = cpu_baseline(first, n, threads, "ng2_lv5_omp")
+    if plain is None:
+        return None
+    native = cpu_baseline(first, n, threads, "ng2_lv5_omp_native")
+    window = "the GPU's timed window" if n == steps else f"the first {n} steps of the GPU's timed window"
+    out = dict(value=plain["value"], unit="zone-updates/s", cores=threads, kind="reference",
+               sample=f"reference OpenMP build (oracle/_ref/ng2_lv5_omp: -O3 -funroll-loops -fopenmp, no FMA "
+                      f"contraction), rotor lv5 MC+HLLD CFL 0.5, steps {first}..{first + n} of the run from t = 0 "
+                      f"({window}): {plain['zone_updates']} zone-updates in {plain['seconds']:.3f} s on {threads} "
+                      f"threads; host CPU: {cpu_model()}",
+               steps=plain["steps"], nproc=os.cpu_count(), affinity_cores=len(os.sched_getaffinity(0)),
+               threads_note="OMP_NUM_THREADS (the host's CPU share for one GPU) when set, else the affinity mask")
+    if native is not None:
+        out["reference_flags"] = dict(
+            value=native["value"], unit="zone-updates/s", zone_updates=native["zone_updates"],
+            seconds=native["seconds"],
+            build="reference makefile flags (makefile:9-19: -O3 -funroll-loops -fopenmp, FMA contraction on) "
+                  "with -march=x86-64-v3 for -march=native (built on another host)")
+    return out
 
 
 def ensemble_bench(steps, full, n_inst, plain=True, warmup=0, groups=2):
@@ -306,16 +353,61 @@ def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):
     return out
 
 
+def rotor_tuning():
+    """Stage-kernel tuner decisions of the whole rotor run, from an untimed
+    run (the timed runs import them, so no trial launch falls in a timed
+    region)."""
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(ROTOR)
+    sim.init()
+    sim.run(-1)
+    tun = {b: v[0] for b, v in sim.stage_tuning().items() if v[0] >= 0}
+    sim.close()
+    return tun
+
+
+def rotor_run(warmup, steps, tuning):
+    """`warmup` untimed steps of the rotor from t = 0, then `steps` timed
+    steps (-1: to the end of the run) with the stage kernel's launches timed
+    by HIP events on the engine's stream."""
+    import torch
+
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(ROTOR)
+    sim.set_stage_tuning(tuning)
+    sim.init()
+    if warmup:
+        sim.run(warmup)
+    torch.cuda.synchronize()
+    nleaf0 = sim.info()["nleaf"]
+    sim.profile(("stage",))
+    t0 = time.perf_counter()
+    n, zu = sim.run(steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    prof = sim.profile_read()
+    info = sim.info()
+    untuned = [b for b, v in sim.stage_tuning().items() if v[0] < 0]
+    sim.close()
+    stage_ms, stage_n = prof["stage"]
+    return dict(steps=n, zone_updates=zu, seconds=secs, value=zu / secs, ms_per_step=secs / n * 1e3,
+                t_reached=info["t"], leaves_at_start=nleaf0, leaves_at_end=info["nleaf"],
+                stage_ms=stage_ms, stage_launches=stage_n, untuned_buckets=untuned)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=ROTOR_STEPS - 57)
-    ap.add_argument("--warmup", type=int, default=57)
-    ap.add_argument("--cpu-steps", type=int, default=100)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=ROTOR_STEPS, help="timed rotor steps (-1: to the end)")
+    ap.add_argument("--warmup", type=int, default=0, help="untimed rotor steps from t = 0 before the timed ones")
+    ap.add_argument("--cpu-max-steps", type=int, default=200,
+                    help="the CPU baseline times at most this many steps of the GPU's window")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
-                    help="also time a batched ensemble of N seeded rotors against N plain runs")
+    ap.add_argument("--no-whole-run", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=256, metavar="N",
+                    help="batched ensemble of N seeded rotors against N plain runs (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-full", action="store_true", help="also run the batched ensemble to t = 0.2")
     ap.add_argument("--ens-warmup", type=int, default=0,
@@ -327,12 +419,12 @@ def main():
     ap.add_argument("--ot-steps", type=int, default=0,
                     help="also time the Orszag-Tang workload for this many steps (-1: to t = 0.5)")
     ap.add_argument("--ot-warmup", type=int, default=0)
-    ap.add_argument("--adjoint", type=int, default=0, metavar="N",
-                    help="also run the frozen-mesh adjoint on an N x N grid")
+    ap.add_argument("--adjoint", type=int, default=512, metavar="N",
+                    help="frozen-mesh adjoint on an N x N grid, HLLE and HLLD (0: skip)")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-fd", type=int, default=0, help="finite-difference check directions")
-    ap.add_argument("--adj-flux", default="HLLE", choices=["HLLE", "HLLD"])
+    ap.add_argument("--adj-flux", default="both", choices=["HLLE", "HLLD", "both"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -343,13 +435,10 @@ def main():
 
     import torch
 
-    import amr2d_mhd_amd as A
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.only_ensemble:
         print(json.dumps(ensemble_bench(args.ens_steps, args.ens_full, args.ensemble or 256, plain=False,
-                                        groups=args.ens_groups,
-                                        warmup=args.ens_warmup)))
+                                        groups=args.ens_groups, warmup=args.ens_warmup)))
         return 0
     if args.copy_calibration:
         x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -358,67 +447,68 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
 
-    sim = A.Simulation(ROTOR)
-    sim.init()
-    if args.warmup:
-        sim.run(args.warmup)
-    torch.cuda.synchronize()
-    sim.profile(("stage",))
-    t0 = time.perf_counter()
-    steps, zu = sim.run(args.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = sim.profile_read()
-    if steps != args.steps:
-        print(f"bench.py: run ended after {steps} of {args.steps} timed steps", file=sys.stderr)
+    tuning = rotor_tuning()
+    steps = args.steps if args.steps >= 0 else -1
+    win = rotor_run(args.warmup, steps, tuning)
+    if steps >= 0 and win["steps"] != steps:
+        print(f"bench.py: run ended after {win['steps']} of {steps} timed steps", file=sys.stderr)
         return 1
-    secs = t1 - t0
-    value = zu / secs
-    stage_ms, stage_n = prof["stage"]
-    zones_per_launch = zu / steps  # every step launches the stage kernel twice over the same leaves
-    achieved = STAGE_BYTES_PER_ZONE * zones_per_launch / (stage_ms / stage_n * 1e-3) / 1e9
-    info = sim.info()
+    steps = win["steps"]
+    zones_per_launch = win["zone_updates"] / steps  # each step launches the stage kernel twice over its leaves
+    mean_launch_s = win["stage_ms"] / win["stage_launches"] * 1e-3
+    achieved = STAGE_BYTES_PER_ZONE * zones_per_launch / mean_launch_s / 1e9
+    flops = STAGE_FP64_FLOPS_PER_ZONE * zones_per_launch / mean_launch_s / 1e12
     out = {
-        "metric": "zone-updates/s FP64 at 1 GPU (rotor, 5 AMR levels, MC+HLLD, CT)",
-        "value": value,
+        "metric": "zone-updates/s FP64 at 1 GPU, batched vs plain; adjoint/forward step cost",
+        "value": win["value"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": secs / steps * 1e3,
+        "ms_per_step": win["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (reference rotor initial condition)",
-        "config": {"workload": "MHD rotor, 4 root blocks of 16x16, 5 AMR levels, MC-MUSCL + HLLD, CT, "
-                               "RK2, CFL 0.5, gamma 1.4, from t=0 (whole run to t=0.295 by default)",
-                   "t_end_reached": info["t"], "leaves_at_end": info["nleaf"],
-                   "zone_updates": zu, "parallelism": "single GPU"},
+        "data": "synthetic (the reference's rotor initial condition, IC2Dtype.cpp:75-111)",
+        "config": {"workload": "MHD rotor (plain method), 4 root blocks of 16x16, 5 AMR levels, MC-MUSCL + HLLD, "
+                               "CT, RK2, CFL 0.5, gamma 1.4; value = zone-updates/s over steps "
+                               f"{args.warmup}..{args.warmup + steps} of the run from t = 0",
+                   "window_steps": [args.warmup, args.warmup + steps], "t_end_reached": win["t_reached"],
+                   "leaves_at_window_start": win["leaves_at_start"], "leaves_at_end": win["leaves_at_end"],
+                   "zone_updates": win["zone_updates"], "parallelism": "single GPU",
+                   "stage_tuner": "decisions imported from an untimed run; untuned buckets in the window: "
+                                  f"{win['untuned_buckets']}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STAGE_TRAFFIC_BYTES_PER_ZONE * zones_per_launch,
-                     "kernel": "stage_kernel<20,2>", "mean_launch_us": stage_ms / stage_n * 1e3,
-                     "launches": stage_n, "bytes_per_zone": STAGE_BYTES_PER_ZONE,
+                     "kernel": "stage_kernel<20,2>", "mean_launch_us": mean_launch_s * 1e6,
+                     "launches": win["stage_launches"], "bytes_per_zone": STAGE_BYTES_PER_ZONE,
                      "zones_per_launch": zones_per_launch},
-        "valu": {"bound": "fp64", "kernel": "stage_kernel<20,2>",
-                 "achieved": STAGE_FP64_FLOPS_PER_ZONE * zones_per_launch / (stage_ms / stage_n * 1e-3) / 1e12,
-                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": STAGE_FP64_FLOPS_PER_ZONE * zones_per_launch / (stage_ms / stage_n * 1e-3) / 1e12
-                 / FP64_PEAK_TFLOPS,
-                 "flops_per_zone": STAGE_FP64_FLOPS_PER_ZONE},
+        "valu": {"bound": "fp64", "kernel": "stage_kernel<20,2>", "achieved": flops, "peak": FP64_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": flops / FP64_PEAK_TFLOPS, "flops_per_zone": STAGE_FP64_FLOPS_PER_ZONE},
         "cpu_baseline": None,
     }
+    if not args.no_whole_run and not (args.warmup == 0 and args.steps in (-1, ROTOR_STEPS)):
+        w = rotor_run(0, -1, tuning)
+        out["whole_run"] = {"workload": "the same rotor run, every step from t = 0 to t = 0.295", "steps": w["steps"],
+                            "zone_updates": w["zone_updates"], "seconds": w["seconds"], "value": w["value"],
+                            "ms_per_step": w["ms_per_step"], "t_reached": w["t_reached"]}
+    if args.ensemble:
+        out["batched_vs_plain"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble,
+                                                 warmup=args.ens_warmup, groups=args.ens_groups)
+    if args.adjoint:
+        fluxes = ["HLLE", "HLLD"] if args.adj_flux == "both" else [args.adj_flux]
+        out["adjoint_over_forward"] = {f: adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd, f)
+                                       for f in fluxes}
     if args.ot_steps:
         out["orszag_tang"] = ot_bench(args.ot_steps, args.ot_warmup)
-    if args.adjoint:
-        out["adjoint"] = adjoint_bench(args.adjoint, args.adj_steps, args.adj_every, args.adj_fd, args.adj_flux)
-    if args.ensemble:
-        out["ensemble"] = ensemble_bench(args.ens_steps, args.ens_full, args.ensemble, warmup=args.ens_warmup,
-                                         groups=args.ens_groups)
     if not args.no_cpu:
-        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)
+        out["cpu_baseline"] = cpu_baselines(args.warmup, steps, args.cpu_max_steps)
+        if out["cpu_baseline"]:
+            cb = out["cpu_baseline"]
+            if cb["steps"] == [args.warmup, args.warmup + steps]:
+                out["gpu_over_cpu_same_window"] = out["value"] / cb["value"]
     print(json.dumps(out))
     return 0
 

@@ -176,16 +176,26 @@ int amr_riemann_batch(int solver, int dir, double gamma, int n, const double *wL
 int amr_recon_batch(int NT, int nghosts, const char *limiter, double dx, int nt, const double *W, double *out,
                     void *stream);
 
-/* Graph replay of the step (no counterpart in the reference): with `on`
- * amr_step/amr_run capture the device work of one step (the parameter
- * upload, ghost fills, both RK stages, CT, the CFL/criteria kernel and the
- * downloads that follow it) into a HIP graph once a topology has lasted one
- * step, and replay it until the tree changes; the first step after a change
- * is launched directly.  Off by default: on ROCm 7.2 direct launches are as
- * fast for these workloads (the 5-level rotor: 1.456 s direct, 1.478 s with
- * replay).  Device-side kernel timing (amr_profile_enable bits 0-4) uses
- * direct launches. */
+/* Graph replay of the step (no counterpart in the reference).  Disabled in
+ * this build: amr_set_graph(ctx, 1) returns AMR_ERR_CONFIG; 0 (direct
+ * launches, the default) is accepted. */
 int amr_set_graph(amr_ctx *ctx, int on);
+
+/* Persistent step kernel (no counterpart in the reference): a step of
+ * amr_step / amr_run / amr_run_group runs as ONE cooperative launch over the
+ * device work lists of every running instance -- the ghost fills, RK stages,
+ * CT kernels and the CFL/criteria reduction of main.cpp:128-230 in order,
+ * separated by grid-wide barriers -- instead of one launch per phase.  Same
+ * results bit for bit.  mode 1: 256 lanes per workgroup, 2: 512 lanes,
+ * 0 and -1 (the default): per-phase launches (graph replay and the
+ * stage-kernel tuner apply to these only) -- on MI355X a grid-wide barrier
+ * over the 8 XCDs costs more than the kernel boundary it replaces, so per-phase
+ * launches are faster for these workloads.  The step kernel runs only where
+ * the device keeps its whole grid resident (else per-phase launches); a
+ * barrier wait bounded at 2 s that runs out fails the step with AMR_ERR_HIP
+ * and switches the context to per-phase launches.  amr_run_group launches per
+ * phase, and so do phase-level calls (amr_fill, amr_stage, ...). */
+int amr_set_step_kernel(amr_ctx *ctx, int mode);
 
 /* Stage-kernel variant (no counterpart in the reference; all variants give
  * identical results): lanes per leaf block and where the face fluxes live
@@ -205,6 +215,11 @@ int amr_set_stage_variant(amr_ctx *ctx, int variant);
  * the mean measured microseconds per leaf block of each variant,
  * us_per_leaf[b][6]. */
 int amr_stage_tuning(amr_ctx *ctx, int nbuckets, int32_t *chosen, double *us_per_leaf);
+/* Import tuner decisions (e.g. from an earlier run of the same workload):
+ * chosen[b] >= 0 fixes bucket b's variant for the specialisation the
+ * context's running instances select, so later launches in it run no trials;
+ * -1 leaves a bucket as it is. */
+int amr_set_stage_tuning(amr_ctx *ctx, int nbuckets, const int32_t *chosen);
 
 /* Fill-program cache (no counterpart in the reference): after a tree change
  * an instance reuses a program compiled earlier for the same tree (the key is
@@ -223,8 +238,12 @@ int amr_set_fill_cache(amr_ctx *ctx, int entries);
  * fill program), of the regrid decisions and of the per-step wait for the
  * device; bit 8 the host time of fill-program compiles, counting compiled
  * programs (cache misses) as launches; bit 9 counts graph replays of the
- * step (no time).   amr_profile_read returns the summed milliseconds and the number of
- * launches (or host scopes) of one family since the last amr_profile_enable. */
+ * step (no time); bit 11 the host time of graph captures; bit 10 times the persistent
+ * step kernel's launches (HIP events around
+ * them); with the step kernel, bits 0-3 time its phases from device-clock stamps
+ * taken after each phase's closing barrier.  amr_profile_read returns the summed
+ * milliseconds and the number of launches (phases, host scopes) of one family
+ * since the last amr_profile_enable. */
 int amr_profile_enable(amr_ctx *ctx, int mask);
 int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
 

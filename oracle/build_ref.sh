@@ -9,7 +9,13 @@
 # the end, so no reference source ever sits in the tree or travels with it.
 #
 # Variants: ng<G>_lv<L>      serial, -O3 -ffp-contract=off (the oracle build)
-#           ng<G>_lv<L>_omp  -fopenmp -DOPENMP, generic x86-64 (CPU baseline)
+#           ng<G>_lv<L>_omp  -fopenmp -DOPENMP, generic x86-64, no FMA contraction
+#                            (CPU baseline with the oracle's arithmetic)
+#           ng<G>_lv<L>_omp_native  the reference makefile's own flags
+#                            (makefile:9-19: -O3 -march -funroll-loops -fopenmp,
+#                            FMAs contracted) with -march=x86-64-v3 in place of
+#                            -march=native: built here, it runs on other hosts
+#                            (timing only: its FMAs change the results)
 #           ng<G>_lv<L>_bs   serial; its "rotor" is the Balsara-Spicer rotor of
 #                            the rotor workload's text (p = 1, v0 = 2,
 #                            Bx = 5/sqrt(4 pi)): a temporary copy of
@@ -20,7 +26,7 @@ HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${REF:-/root/reference}"
 OUT="$HERE/_ref"
 JOBS="${JOBS:-8}"
-VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp ng2_lv5_bs}"
+VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp ng2_lv5_omp_native ng2_lv5_bs}"
 
 if [ ! -f "$REF/defs.hpp" ]; then
   echo "build_ref: reference not present at $REF; skipping" >&2
@@ -53,6 +59,7 @@ build_variant() {
   ng="$(echo "$v" | sed -E 's/^ng([0-9]+)_.*/\1/')"
   lv="$(echo "$v" | sed -E 's/^ng[0-9]+_lv([0-9]+).*/\1/')"
   omp=0; [[ "$v" == *_omp ]] && omp=1
+  native=0; [[ "$v" == *_omp_native ]] && native=1
   bs=0; [[ "$v" == *_bs ]] && bs=1
   local inc="$TMP/$v" obj="$OUT/$v/obj" bin="$OUT/$v/refrun"
   mkdir -p "$inc" "$obj"
@@ -76,6 +83,7 @@ build_variant() {
   fi
   local flags="-O3 -std=c++17 -ffp-contract=off -DDOUBLE_PREC -w"
   if [ "$omp" = 1 ]; then flags="-O3 -std=c++17 -funroll-loops -ffp-contract=off -DDOUBLE_PREC -DOPENMP -fopenmp -w"; fi
+  if [ "$native" = 1 ]; then flags="-O3 -std=c++17 -march=x86-64-v3 -funroll-loops -DDOUBLE_PREC -DOPENMP -fopenmp -w"; fi
   # skip when up to date
   if [ -x "$bin" ] && [ "$bin" -nt "$HERE/refharness/harness.cpp" ] && [ "$bin" -nt "$HERE/build_ref.sh" ] \
      && [ -z "$(find "$REF" -newer "$bin" -name '*.[ch]pp' -print -quit)" ]; then

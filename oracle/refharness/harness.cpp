@@ -62,6 +62,7 @@ struct Opts {
   std::vector<int> dumps;  // dump full state at the end of these steps
   int trace = -1;          // dump state around every phase of this step
   std::string timingf;     // wall-clock of the time loop
+  std::string steptimesf;  // per step: seconds since the loop started at its end, leaves it advanced
   // random-tree mode (criteria replaced by seeded random marks)
   unsigned long long seed = 0;
   double p_ref = 0.0, p_coarse = 0.0;
@@ -77,6 +78,7 @@ bool g_zeroed = false;
 int g_muscl_stage_seen[3] = {-1, -1, -1};
 double g_dt_applied = 0.0;
 FILE *g_log = nullptr;
+FILE *g_steptimes = nullptr;
 std::chrono::steady_clock::time_point g_t0;
 bool g_t0_set = false;
 
@@ -328,6 +330,12 @@ void admesh(meshblock &dom) {
   ref_admesh(dom);
   trace("regrid", "all", "post");
   r.lastActive_post = dom.lastActive; r.nleaf_post = count_leaves();
+  if (g_steptimes) {
+    // end of the loop iteration (main.cpp:128-247): the regrid was its last work
+    const double rec[2] = {std::chrono::duration<double>(std::chrono::steady_clock::now() - g_t0).count(),
+                           (double)r.nleaf_pre};
+    std::fwrite(rec, sizeof rec, 1, g_steptimes);
+  }
   if (g_log) { r.hf_post = hash_fields(); r.ht_post = hash_tree(); std::fwrite(&r, sizeof r, 1, g_log); std::fflush(g_log); }
   for (int s : g.dumps)
     if (s == g_step) {
@@ -530,7 +538,7 @@ int main(int argc, char **argv) {
     return vec_main(argv[2], argv[3], argv[4]);
   }
   if (argc < 2 || std::string(argv[1]) != "run") {
-    std::fprintf(stderr, "usage: run --dir D --input 'nx ny nvar|lx ly|cfl|IC lim flux' [--steps K] [--log F] [--dump a,b] [--trace s] [--timing F] [--randtree seed,pref,pcoarse,rounds]\n");
+    std::fprintf(stderr, "usage: run --dir D --input 'nx ny nvar|lx ly|cfl|IC lim flux' [--steps K] [--log F] [--dump a,b] [--trace s] [--timing F] [--step-times F] [--randtree seed,pref,pcoarse,rounds]\n");
     return 2;
   }
   std::string input;
@@ -545,6 +553,7 @@ int main(int argc, char **argv) {
     else if (s == "--dump") { std::stringstream ss(next()); std::string t; while (std::getline(ss, t, ',')) g.dumps.push_back(std::atoi(t.c_str())); }
     else if (s == "--trace") g.trace = std::atoi(next().c_str());
     else if (s == "--timing") g.timingf = next();
+    else if (s == "--step-times") g.steptimesf = next();
     else if (s == "--verbose") quiet = false;
     else if (s == "--randtree") {
       std::string v = next();
@@ -557,6 +566,7 @@ int main(int argc, char **argv) {
   g.dir = ".";
   { FILE *fp = std::fopen("input.txt", "w"); std::fprintf(fp, "%s\n", input.c_str()); std::fclose(fp); }
   if (!g.logf.empty()) g_log = std::fopen(g.logf.c_str(), "wb");
+  if (!g.steptimesf.empty()) g_steptimes = std::fopen(g.steptimesf.c_str(), "wb");
   if (quiet) { std::FILE *r = std::freopen("/dev/null", "w", stdout); (void)r; }
   std::stringstream answers("2\n0\n");
   std::streambuf *old = std::cin.rdbuf(answers.rdbuf());
@@ -565,6 +575,7 @@ int main(int argc, char **argv) {
   std::cin.rdbuf(old);
   auto t1 = std::chrono::steady_clock::now();
   if (g_log) std::fclose(g_log);
+  if (g_steptimes) std::fclose(g_steptimes);
   if (!g.timingf.empty()) {
     FILE *fp = std::fopen(g.timingf.c_str(), "w");
     double s = g_t0_set ? std::chrono::duration<double>(t1 - g_t0).count() : 0.0;

@@ -65,3 +65,38 @@ def lockstep(sim, dom, steps, full_every=10, label=""):
             d = compare_states(gpu_state(sim), oracle_state(dom))
             assert not d, f"{label}: state differs after step {s}: " + "; ".join(d[:4])
     return done
+
+
+def hashes_via_oracle(sim, dom, fields=True):
+    """FNV-1a hashes of the GPU's quadtree and (with `fields`) of every slot
+    field, computed by the oracle's hash functions over the downloaded state
+    (dom: an oracle Domain of the same configuration, used for hashing only)."""
+    lp, ic, lf = sim.get_tree()
+    dom.set_tree(lp, ic, lf)
+    if not fields:
+        return None, dom.hash_tree()
+    for name, f in FIELD_MAP.items():
+        dom.set_field(f, sim.download(name))
+    return dom.hash_fields(), dom.hash_tree()
+
+
+def follow_oracle_log(sim, dom, rows, label):
+    """Step the GPU through an oracle-written log (tests/golden/
+    make_oracle_logs.py): per step the applied dt, t, slot and leaf counts and
+    the tree hash; the field hash wherever the log holds one."""
+    r = rows[0]
+    hf, ht = hashes_via_oracle(sim, dom, fields=bool(r["hash_fields"]))
+    assert ht == int(r["hash_tree"]), f"{label}: initial tree differs"
+    if r["hash_fields"]:
+        assert hf == int(r["hash_fields"]), f"{label}: initial fields differ"
+    for r in rows[1:]:
+        applied = sim.info()["dt"]
+        sim.step()
+        inf = sim.info()
+        got = (applied, inf["t"], inf["last_active"], inf["nleaf"])
+        want = (r["dt"], r["t"], r["last_active"], r["nleaf"])
+        assert got == want, f"{label}: step {int(r['step'])}: gpu {got} oracle {want}"
+        hf, ht = hashes_via_oracle(sim, dom, fields=bool(r["hash_fields"]))
+        assert ht == int(r["hash_tree"]), f"{label}: tree differs after step {int(r['step'])}"
+        if r["hash_fields"]:
+            assert hf == int(r["hash_fields"]), f"{label}: fields differ after step {int(r['step'])}"

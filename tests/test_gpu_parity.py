@@ -178,27 +178,37 @@ def test_pressure_error_reported():
     assert codes[:2] == [0, 0] and codes[2] == 1 and ei.value.code == 1
 
 
+@pytest.mark.skip(reason="graph replay is disabled: replaying one captured step over changing trees "
+                         "faulted the GPU twice in a long rotor run; its cause is not yet shown, so the case "
+                         "stays off the GPU (amr_set_graph(ctx, 1) returns AMR_ERR_CONFIG)")
 def test_graph_replay_equals_direct_launches():
-    """The step replayed as a HIP graph (captured once a topology has lasted
-    a step, re-captured after every tree change) equals direct launches bit
-    for bit, replay after replay."""
+    """The step replayed as a HIP graph equals direct launches bit for bit,
+    replay after replay, over hundreds of tree changes: the captured kernels
+    read the leaf and chunk counts from the device, so a graph is captured
+    again only when the counts outgrow (or shrink far below) its grids or the
+    stage-kernel variant changes -- not at every tree change."""
     import amr2d_mhd_amd as A
 
     cfg = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
     g, d = A.Simulation(cfg), A.Simulation(cfg)
+    for x in (g, d):  # graph replay serves per-phase launches
+        x.set_step_kernel(0)
     g.set_graph(True)
     d.set_graph(False)
-    g.profile(("graph",))
+    g.profile(("graph", "graph_capture", "host_topology"))
     g.init()
     d.init()
-    for s in range(120):
+    for s in range(600):
         g.step()
         d.step()
-        if s % 10 == 9:
+        assert g.info() == d.info(), f"step {s}"
+        if s % 25 == 24:
             diff = compare_states(gpu_state(g), gpu_state(d))
             assert not diff, f"after step {s}: " + "; ".join(diff[:4])
-    assert g.info() == d.info()
-    assert g.profile_read()["graph"][1] > 20  # replays happened
+    pr = g.profile_read()
+    replays, captures, rebuilds = pr["graph"][1], pr["graph_capture"][1], pr["host_topology"][1]
+    assert replays > 500, replays
+    assert rebuilds > 40 and captures * 4 < rebuilds, (rebuilds, captures)
 
 
 def test_fill_program_cache_equals_recompiling():
@@ -231,6 +241,7 @@ def test_stage_variant_bitwise(variant):
     import amr2d_mhd_amd as A
 
     sim = A.Simulation(dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=4))
+    sim.set_step_kernel(0)  # the variants are those of the per-phase stage launch
     sim.set_stage_variant(variant)
     dom = O.Domain(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=4)
     assert dom.init() == 0
@@ -331,3 +342,99 @@ def test_rotor_bs_aborts_like_reference():
     assert e.value.code == 1
     assert (e.value.diag.nb, e.value.diag.i, e.value.diag.j) == (39, 2, 2)
     assert sim.info()["status"] == 1
+
+
+OT_WORKLOAD = dict(ic="OT", limiter="WENO5", flux="HLLD", cfl=0.4, nx=16, nghosts=4, periodic=True, rk_stages=3)
+
+
+def test_orszag_tang_lv5_stops_where_the_oracle_does():
+    """Orszag-Tang, 5 levels, WENO5 + HLLD, SSP-RK3 (build-defined), against the
+    oracle's log (tests/golden/ot_WENO5_HLLD_rk3_lv5_abort.npz): every step's
+    applied dt, t, slot and leaf counts and tree hash, the hash of every slot
+    field every 100 steps, then the scheme's own failure -- p <= 0 in block
+    567, cell (19, 17) at step 686 -- with the same code at the same place.
+    (t = 0.5 is out of this scheme's reach at 5 and 6 levels.)"""
+    import os
+
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd import AmrError
+    from gpu_util import follow_oracle_log
+
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "ot_WENO5_HLLD_rk3_lv5_abort.npz"))
+    rows, abort = z["steps"], [int(v) for v in z["abort"]]
+    assert len(rows) == 686 and abort == [1, 567, 19, 17, 686]
+    cfg = dict(OT_WORKLOAD, maxlevs=5)
+    sim = A.Simulation(cfg)
+    sim.init()
+    dom = O.Domain(**cfg)  # hashes only
+    assert dom.init() == 0
+    follow_oracle_log(sim, dom, rows, "OT lv5")
+    with pytest.raises(AmrError) as e:
+        sim.step()
+    assert [e.value.code, e.value.diag.nb, e.value.diag.i, e.value.diag.j] == abort[:4]
+    sim.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("case", ["BWx lv4 full run", "rotor lv5", "OT WENO5 rk3 lv4", "rotor HLLC lv4 generic"])
+def test_step_kernel_modes_bitwise(case, mode):
+    """Per-phase launches (mode 0) and the persistent step kernel with 256 or
+    512 lanes (modes 1, 2) all reproduce the oracle bit for bit, through the
+    specialised (HLLD + MUSCL, HLLD + WENO5) and generic stage code; the step
+    kernel really runs in modes 1 and 2."""
+    import amr2d_mhd_amd as A
+
+    cfgs = {"BWx lv4 full run": (dict(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, maxlevs=4), 1000),
+            "rotor lv5": (dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, maxlevs=5), 150),
+            "OT WENO5 rk3 lv4": (dict(ic="OT", limiter="WENO5", flux="HLLD", cfl=0.4, maxlevs=4, nghosts=4,
+                                      periodic=True, rk_stages=3), 40),
+            "rotor HLLC lv4 generic": (dict(ic="rotor", limiter="MC", flux="HLLC", cfl=0.5, maxlevs=4), 60)}
+    cfg, steps = cfgs[case]
+    cfg = dict(dict(nx=16, nghosts=2), **cfg)
+    sim = A.Simulation(cfg)
+    sim.set_step_kernel(mode)
+    sim.profile(("step",))
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    n = lockstep(sim, dom, steps, full_every=20, label=f"{case} step kernel mode {mode}")
+    launched = sim.profile_read()["step"][1]
+    assert launched == (0 if mode == 0 else n), (launched, n)
+
+
+def test_orszag_tang_lv6_first_1000_steps_equal_oracle_log():
+    """The Orszag-Tang workload at its 6 levels (WENO5 + HLLD, SSP-RK3,
+    periodic, build-defined), first 1000 steps (to t = 0.158, ~4000 leaves),
+    against the oracle's log (tests/golden/ot_WENO5_HLLD_rk3_lv6_1000.npz):
+    every step's applied dt, t, slot and leaf counts and tree hash, and the
+    hash of every slot field every 100 steps."""
+    import os
+
+    import amr2d_mhd_amd as A
+    from gpu_util import follow_oracle_log
+
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ot_WENO5_HLLD_rk3_lv6_1000.npz"))
+    rows = z["steps"]
+    assert len(rows) == 1001 and int(z["abort"][0]) == -1
+    cfg = dict(OT_WORKLOAD, maxlevs=6)
+    sim = A.Simulation(cfg)
+    sim.init()
+    dom = O.Domain(**cfg)  # hashes only
+    assert dom.init() == 0
+    follow_oracle_log(sim, dom, rows, "OT lv6")
+    assert sim.info()["nleaf"] == int(rows[-1]["nleaf"]) > 3000
+    sim.close()
+
+
+def test_graph_replay_is_disabled():
+    """amr_set_graph(ctx, 1) is refused while graph replay is disabled; direct
+    launches (0) stay accepted."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd import AmrError
+
+    sim = A.Simulation(dict(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=3))
+    with pytest.raises(AmrError) as e:
+        sim.set_graph(True)
+    assert e.value.code == 10
+    sim.set_graph(False)
