@@ -94,6 +94,8 @@ _SIGS = {
     "amr_adj_step_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_adj_tape_doubles": (C.c_int64, [C.c_void_p]),
+    "amr_adj_step_noalloc": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_adj_step_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_adj_vjp_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,

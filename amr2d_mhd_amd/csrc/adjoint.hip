@@ -19,6 +19,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/amr2d_mhd.h"
 #include "abi_guard.hpp"
@@ -564,9 +565,69 @@ int reverse(amr_adj* a, const Tape& T, const double* Bx, const double* By, doubl
   return 0;
 }
 
+// ------------------------------------------------- allocation-free step
+// Work memory of amr_adj_step_noalloc: the forward's face fluxes, corner EMF,
+// stage state and tape (8 + 8 + 1 + 8 + kTapePerCell doubles per cell) for
+// grids of up to kNoAllocMaxN^2 cells, in one file-scope device array that
+// every GPU the library runs on holds until the process ends (25.7 MB).  Host
+// code never passes the array's name to a kernel (a host-side stand-in):
+// each device's address comes from hipGetSymbolAddress, taken once per
+// device on its first call, and only that address is used.
+constexpr int kNoAllocMaxN = 256;
+constexpr size_t kNoAllocPerCell = 8 + 8 + 1 + 8 + kTapePerCell;
+static __device__ __align__(256) double scratch[kNoAllocPerCell * kNoAllocMaxN * kNoAllocMaxN];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the current device's address of `scratch`
+int scratch_address(double** out) {
+  int dev = -1;
+  ADJ_CHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return AMR_ERR_CONFIG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    ADJ_CHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return AMR_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int amr_adj_step_noalloc(int n, double dx, double gamma, const char* flux, const double* U, const double* Bx,
+                         const double* By, double dt, double* Uo, double* Bxo, double* Byo, void* stream) {
+  AMR_TRY
+  if (n < 4 || n > kNoAllocMaxN || dx <= 0 || !U || !Bx || !By || !Uo || !Bxo || !Byo) {
+    if (n > kNoAllocMaxN)
+      std::fprintf(stderr, "amr_adj_step_noalloc: n = %d exceeds the %d x %d cells of its static work memory\n", n,
+                   kNoAllocMaxN, kNoAllocMaxN);
+    return AMR_ERR_CONFIG;
+  }
+  int solver = kFluxHLLE;
+  if (flux && std::strcmp(flux, "HLLD") == 0) solver = kFluxHLLD;
+  else if (flux && std::strcmp(flux, "HLLE") != 0) return AMR_ERR_CONFIG;
+  double* w = nullptr;
+  if (int e = scratch_address(&w)) return e;
+  // a context on the stack whose work buffers are views into the array
+  amr_adj a{};
+  a.G = Grid{n, dx, gamma, (size_t)n * n};
+  a.solver = solver;
+  const size_t n2 = a.G.n2;
+  a.Fx = w;
+  a.Fy = w + 8 * n2;
+  a.E = w + 16 * n2;
+  a.Us = w + 17 * n2;
+  a.tape = w + 25 * n2;
+  return forward(&a, U, Bx, By, dt, Uo, Bxo, Byo, tape_at(a.tape, n2), (hipStream_t)stream);
+  AMR_CATCH
+}
 
 int amr_adj_create(int n, double dx, double gamma, const char* flux, amr_adj** out) {
   AMR_TRY

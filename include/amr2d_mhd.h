@@ -279,6 +279,18 @@ int amr_adj_step_vjp(amr_adj *adj, const double *U, const double *Bx, const doub
                      const double *Uo_b, const double *Bxo_b, const double *Byo_b, double *U_b, double *Bx_b,
                      double *By_b, void *stream);
 
+/* amr_adj_step without a context and without allocating: no counterpart in
+ * the reference.  For callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same step, same results as amr_adj_step on a context created
+ * with (n, dx, gamma, flux).  Its work memory is a static device array the
+ * library holds on every GPU it runs on until the process ends (25.7 MB), so
+ * n is limited to 4..256 (AMR_ERR_CONFIG beyond).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams or
+ * host threads) race; calls ordered on one stream do not. */
+int amr_adj_step_noalloc(int n, double dx, double gamma, const char *flux, const double *U, const double *Bx,
+                         const double *By, double dt, double *Uo, double *Bxo, double *Byo, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
