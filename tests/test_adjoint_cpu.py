@@ -192,3 +192,48 @@ def test_numpy_frozen_step_equals_oracle_amr_step(flux):
         U1, Bx1, By1 = glob()
         Un, Bxn, Byn = US.step(U0, Bx0, By0, dt, 1.0 / 32, 5.0 / 3.0, flux)
         assert np.array_equal(Un, U1) and np.array_equal(Bxn, Bx1) and np.array_equal(Byn, By1), f"step {s}"
+
+
+def _minmod_branches(W):
+    """Branch taken by each minmod evaluation of a step's reconstruction:
+    0 (r <= 0), 1 (0 < r <= 1: r), 2 (r > 1) for both edge states of every
+    x and y face and variable."""
+    import uniform_step as US
+
+    out = []
+    for ax in (2, 1):
+        vm1, v0, vp1, vp2 = (np.roll(W, s, axis=ax) for s in (1, 0, -1, -2))
+        a, b, d = v0 - vm1, vp1 - v0, vp2 - vp1
+        for r in (b / (a + US.EPS), b / (d + US.EPS)):
+            out.append(np.where(r > 1.0, 2, np.where(r > 0.0, 1, 0)))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("flux", ["HLLE", "HLLD"])
+def test_torch_restatement_forward_equals_numpy(flux):
+    """The torch restatement (oracle/uniform_autograd.py, the one the GPU
+    adjoint tests differentiate) against the numpy one (uniform_step.py,
+    bitwise equal to the oracle's AMR step): over 6 steps they take the same
+    minmod branches at every face and agree to a few ulps.  They are not
+    bitwise equal because torch's CPU sqrt is not correctly rounded (about 1
+    in 110 results is off by one ulp), while numpy's, the reference's and the
+    GPU's are."""
+    import torch
+
+    import uniform_autograd as UA
+    import uniform_step as US
+    from amr2d_mhd_amd import adjoint as AD
+
+    g, n = 5.0 / 3.0, 32
+    rho, v, p, Bx, By = AD.orszag_tang_controls(n, device="cpu", modulation=0.05, noise=1e-3)
+    U = AD.cons_from_controls(rho, v, p, Bx, By, g).contiguous()
+    dt = AD.cfl_dt(U, g, 1.0 / n, 0.4)
+    Un, Bxn, Byn = U.numpy().copy(), Bx.numpy().copy(), By.numpy().copy()
+    Ut, Bxt, Byt = U.clone(), Bx.clone(), By.clone()
+    for s in range(6):
+        assert np.array_equal(_minmod_branches(US.prim(Un, g)), _minmod_branches(UA.prim(Ut, g).numpy())), s
+        Un, Bxn, Byn = US.step(Un, Bxn, Byn, dt, 1.0 / n, g, flux)
+        Ut, Bxt, Byt = UA.step(Ut, Bxt, Byt, dt, 1.0 / n, g, flux)
+        for a, b in ((Ut.numpy(), Un), (Bxt.numpy(), Bxn), (Byt.numpy(), Byn)):
+            assert np.allclose(a, b, rtol=2e-15, atol=2e-15 * np.abs(b).max()), (s, np.abs(a - b).max())
+    assert torch.equal(torch.sqrt(torch.tensor([4.0, 9.0])), torch.tensor([2.0, 3.0]))
