@@ -306,6 +306,12 @@ __device__ inline void face_reverse(const Grid& G, int i, int j, const double* _
     swap_xy(wlb);  // adjoint of the input swap
     swap_xy(wrb);
   }
+  // Recompute the stencil cells from opaque copies of (i, j) rather than
+  // keep the four 64-bit offsets alive across the solver reverse.
+  int io = i, jo = j;
+  asm volatile("" : "+v"(io), "+v"(jo));
+  for (int s = 0; s < 4; s++)
+    cell[s] = DIR == 0 ? (size_t)jo * G.n + wrap(io + s - 2, G.n) : (size_t)wrap(jo + s - 2, G.n) * G.n + io;
 }
 
 // stencil contributions vb[s] of variable k (zero for the normal B, which the
@@ -397,13 +403,15 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
                                                   double* __restrict__ Qb) {
   static_assert(TH * (TW + 3) <= TT && (TH + 3) * TW <= TT, "one face per lane and direction");
   __shared__ double S[8][4][TT];  // [variable][stencil position][face of the tile]
+  // x-face sums of the tile's cells, kept in LDS across the y-face reverse
+  // rather than in 16 VGPRs
+  __shared__ double Sx[8][TW * TH];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
   // this lane's cell
   const int ci = tid % TW, cj = tid / TW;
   const int i = i0 + ci, j = j0 + cj;
   const bool has_cell = tid < TW * TH && i < G.n && j < G.n;
-  double wb[8];
   // ---- x faces: tile row r, face column q -> face i0 - 1 + q
   if (tid < TH * (TW + 3)) {
     const int r = tid / (TW + 3), q = tid % (TW + 3);
@@ -424,7 +432,7 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
     for (int k = 0; k < 8; k++) {
       double a = 0.;
       for (int s = 0; s < 4; s++) a += S[k][s][cj * (TW + 3) + ci + 3 - s];
-      wb[k] = a;
+      Sx[k][tid] = a;
     }
   }
   __syncthreads();
@@ -445,9 +453,9 @@ __global__ __launch_bounds__(TT) void k_stage_vjp(Grid G, const double* __restri
   __syncthreads();
   if (has_cell) {
     const size_t c = (size_t)j * G.n + i;
-    double w[8], ub[8];
+    double w[8], ub[8], wb[8];
     for (int k = 0; k < 8; k++) {
-      double a = wb[k];
+      double a = Sx[k][tid];
       for (int s = 0; s < 4; s++) a += S[k][s][(cj + 3 - s) * TW + ci];
       wb[k] = a;
       w[k] = W[k * G.n2 + c];

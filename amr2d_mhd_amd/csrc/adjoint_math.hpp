@@ -12,6 +12,15 @@
 
 namespace amr {
 
+// Hides x from the optimizer, so that values recomputed from it are not
+// merged with identical ones computed earlier and kept live in the
+// meantime (a register-pressure device; no effect in the host build).
+__host__ __device__ inline void opaque(double& x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+v"(x));
+#endif
+}
+
 // ------------------------------------------------------------ cons -> prim
 __host__ __device__ inline void prim_of(const double* u, double g, double* w) {
   w[0] = u[0];
@@ -239,11 +248,17 @@ __host__ __device__ inline void side_bwd(const double* w, double g, double e, do
   fast_bwd(w, g, f, iw0, cfb, wb);
 }
 
-// side reverse with general adjoints of q[8] and F[8], plus adjoints of e
+// side reverse with q and F adjoints sQ Nb and sF Nb, plus adjoints of e
 // and pt used outside q and F (HLLD star states), H and the fast speed
 __host__ __device__ inline void side_bwd_vec(const double* w, double g, double e, double H, const FastT& f,
-                                             const double* qb, const double* Fb, double Hb, double eb_x, double ptb_x,
-                                             double cfb, double* wb) {
+                                             const double* Nb, double sF, double sQ, double Hb, double eb_x,
+                                             double ptb_x, double cfb, double* wb) {
+  double Fb[8], qb[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    Fb[k] = sF * Nb[k];
+    qb[k] = sQ * Nb[k];
+  }
   const double iw0 = 1. / w[0];
   const double vn = w[1], Bn = w[5];
   const double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
@@ -559,35 +574,192 @@ __host__ __device__ inline void star_cons_bwd(const double* w, const double* ub,
   wb[7] += ub[7];
 }
 
+// star_cons reverse with ub = sc Nb
+__host__ __device__ inline void star_cons_bwd_s(const double* w, const double* Nb, double sc, double* wb,
+                                                double& eb) {
+  double ub[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) ub[k] = sc * Nb[k];
+  star_cons_bwd(w, ub, wb, eb);
+}
+
+// component k of star_cons(w, e)
+__host__ __device__ inline double star_cons_k(const double* w, double e, int k) {
+  switch (k) {
+    case 0: return w[0];
+    case 1: return w[0] * w[1];
+    case 2: return w[0] * w[2];
+    case 3: return w[0] * w[3];
+    case 4: return e;
+    default: return w[k];
+  }
+}
+
+// Reverse of the part of the HLLD flux every subsonic region shares: SM and
+// p* over den, SL = min(vL, vR) - cmax, SR = max(vL, vR) + cmax with
+// cmax = max(cfL, cfR), and both sides' state (the active side with its
+// F adjoint fb and q adjoint sq fb; the other with none).  The sides' fast
+// speeds, energies, enthalpies and total pressures are recomputed here
+// from opaque copies of the states (the same expressions, so the same
+// values) rather than held in registers across the star-state reverse.
+struct HlldSides {
+  double eL, HL, ptL, eR, HR, ptR;
+  FastT fL, fR;
+};
+__host__ __device__ inline void hlld_sides(const double* wL, const double* wR, double g, HlldSides& s) {
+  side_fwd(wL, g, s.fL, s.eL, s.HL);
+  side_fwd(wR, g, s.fR, s.eR, s.HR);
+  s.ptL = wL[4] + 0.5 * (wL[5] * wL[5] + wL[6] * wL[6] + wL[7] * wL[7]);
+  s.ptR = wR[4] + 0.5 * (wR[5] * wR[5] + wR[6] * wR[6] + wR[7] * wR[7]);
+}
+__host__ __device__ inline void hlld_tail_bwd(const double* wL_, const double* wR_, double g, double SL, double SR,
+                                              double SM, double pts, double den, bool left, const double* fb,
+                                              double sq, double SLb, double SRb, double SMb, double ptsb, double eLb,
+                                              double eRb, double ptLb, double ptRb, double* wLb, double* wRb) {
+  double wL[8], wR[8];
+  for (int k = 0; k < 8; k++) {
+    wL[k] = wL_[k];
+    wR[k] = wR_[k];
+    opaque(wL[k]);
+    opaque(wR[k]);
+  }
+  HlldSides s;
+  hlld_sides(wL, wR, g, s);
+  const double vL = wL[1], vR = wR[1];
+  const double dR = SR - vR, dL = SL - vL;
+  double dRb = 0., dLb = 0., denb = 0., vLb = 0., vRb = 0., wL0b = 0., wR0b = 0.;
+  {
+    const double Ns = SMb / den, Np = ptsb / den;
+    denb -= SMb * SM / den + ptsb * pts / den;
+    dRb += Ns * wR[0] * vR;
+    wR0b += Ns * dR * vR;
+    vRb += Ns * dR * wR[0];
+    dLb -= Ns * wL[0] * vL;
+    wL0b -= Ns * dL * vL;
+    vLb -= Ns * dL * wL[0];
+    ptRb -= Ns;
+    ptLb += Ns;
+    dRb += Np * wR[0] * s.ptL;
+    wR0b += Np * dR * s.ptL;
+    ptLb += Np * dR * wR[0];
+    dLb -= Np * wL[0] * s.ptR;
+    wL0b -= Np * dL * s.ptR;
+    ptRb -= Np * dL * wL[0];
+    const double dv = vR - vL, t = wL[0] * wR[0] * dR * dL;
+    wL0b += Np * wR[0] * dR * dL * dv;
+    wR0b += Np * wL[0] * dR * dL * dv;
+    dRb += Np * wL[0] * wR[0] * dL * dv;
+    dLb += Np * wL[0] * wR[0] * dR * dv;
+    vRb += Np * t;
+    vLb -= Np * t;
+  }
+  dRb += denb * wR[0];
+  wR0b += denb * dR;
+  dLb -= denb * wL[0];
+  wL0b -= denb * dL;
+  SRb += dRb;
+  vRb -= dRb;
+  SLb += dLb;
+  vLb -= dLb;
+  const double cmaxb = -SLb + SRb;
+  if (vR < vL) vRb += SLb;
+  else vLb += SLb;
+  if (vL < vR) vRb += SRb;
+  else vLb += SRb;
+  double cfLb = 0., cfRb = 0.;
+  if (s.fL.cf < s.fR.cf) cfRb += cmaxb;
+  else cfLb += cmaxb;
+  wLb[0] += wL0b;
+  wRb[0] += wR0b;
+  wLb[1] += vLb;
+  wRb[1] += vRb;
+  side_bwd_vec(wL, g, s.eL, s.HL, s.fL, fb, left ? 1. : 0., left ? sq : 0., 0., eLb, ptLb, cfLb, wLb);
+  side_bwd_vec(wR, g, s.eR, s.HR, s.fR, fb, left ? 0. : 1., left ? 0. : sq, 0., eRb, ptRb, cfRb, wRb);
+}
+
 // HLLD flux (flux_hlld, Miyoshi & Kusano 2005 with the reference's branch
 // conditions) reverse: fb -> wLb, wRb (accumulated).  The forward values are
-// recomputed with the forward's own functions; f receives the flux.
+// recomputed with the forward's own expressions; f receives the flux.  The
+// region is decided first, from the wave speeds and the star densities, and
+// only that region's states are formed and reversed: the supersonic regions
+// (1, 6) reverse one side's flux, regions 2 and 5 one star state, regions 3
+// and 4 both star states and the double-star state.  Star conserved vectors
+// are formed per component where they are used.
 __host__ __device__ inline void hlld_vjp(const double* wL, const double* wR, double g, const double* fb, double* wLb,
                                          double* wRb, double* f = nullptr) {
-  // sides: fast speed, energy, enthalpy and total pressure as side_state
-  // computes them; q and F per component on demand (side_qF)
-  struct SideV {
-    double e, H, pt, cf;
-  } L, R;
-  FastT fL, fR;
-  side_fwd(wL, g, fL, L.e, L.H);
-  side_fwd(wR, g, fR, R.e, R.H);
-  L.cf = fL.cf;
-  R.cf = fR.cf;
-  L.pt = wL[4] + 0.5 * (wL[5] * wL[5] + wL[6] * wL[6] + wL[7] * wL[7]);
-  R.pt = wR[4] + 0.5 * (wR[5] * wR[5] + wR[6] * wR[6] + wR[7] * wR[7]);
+  HlldSides s;
+  hlld_sides(wL, wR, g, s);
   const double vL = wL[1], vR = wR[1];
-  const double cmax = std_max(L.cf, R.cf);
+  const double cmax = std_max(s.fL.cf, s.fR.cf);
   const double SL = std_min(vL, vR) - cmax;
   const double SR = std_max(vL, vR) + cmax;
   const double den = (SR - vR) * wR[0] - (SL - vL) * wL[0];
-  const double SM = ((SR - vR) * wR[0] * vR - (SL - vL) * wL[0] * vL - R.pt + L.pt) / den;
-  const double pts = ((SR - vR) * wR[0] * L.pt - (SL - vL) * wL[0] * R.pt +
+  const double SM = ((SR - vR) * wR[0] * vR - (SL - vL) * wL[0] * vL - s.ptR + s.ptL) / den;
+  const double pts = ((SR - vR) * wR[0] * s.ptL - (SL - vL) * wL[0] * s.ptR +
                       wL[0] * wR[0] * (SR - vR) * (SL - vL) * (vR - vL)) / den;
+  // star densities (hlld_star's ws[0]) and the Alfven speeds they give
+  const double rl = sqrt(wL[0] * (SL - wL[1]) / (SL - SM)), rr = sqrt(wR[0] * (SR - wR[1]) / (SR - SM));
+  const double SLs = SM - fabs(wL[5]) / rl;
+  const double SRs = SM + fabs(wR[5]) / rr;
+  int region = 0;
+  if (SL > 0) region = 1;
+  else if (SL <= 0. && SLs >= 0.) region = 2;
+  else if (SLs <= 0. && SM >= 0.) region = 3;
+  else if (SM <= 0. && SRs >= 0.) region = 4;
+  else if (SRs <= 0. && SR >= 0.) region = 5;
+  else if (SR < 0.) region = 6;
+
+  if (region == 0) {  // no region holds (NaN state): the forward's flux is 0
+    if (f)
+      for (int k = 0; k < 8; k++) f[k] = 0.;
+    return;
+  }
+  const bool left = region <= 3;
+  const double* w = left ? wL : wR;
+  double* wb = left ? wLb : wRb;
+  const double ee = left ? s.eL : s.eR, HH = left ? s.HL : s.HR;
+  if (region == 1 || region == 6) {  // F of one side
+    if (f)
+      for (int k = 0; k < 8; k++) {
+        double q, F;
+        side_qF(w, ee, HH, k, q, F);
+        f[k] = F;
+      }
+    side_bwd_vec(w, g, ee, HH, left ? s.fL : s.fR, fb, 1., 0., 0., 0., 0., 0., wb);
+    return;
+  }
+  const double S = left ? SL : SR;
+  double SLb = 0., SRb = 0., SMb = 0., ptsb = 0., eLb = 0., eRb = 0., ptLb = 0., ptRb = 0.;
+  if (region == 2 || region == 5) {
+    // f = F + S (q* - q) of the active side
+    double ws[8], es;
+    hlld_star(w, S, SM, pts, ee, left ? s.ptL : s.ptR, ws, es);
+    double Sb = 0.;
+    for (int k = 0; k < 8; k++) {
+      double q, F;
+      side_qF(w, ee, HH, k, q, F);
+      const double qs = star_cons_k(ws, es, k);
+      if (f) f[k] = F + S * qs - S * q;
+      Sb += fb[k] * (qs - q);
+    }
+    double wsb[8] = {0., 0., 0., 0., 0., 0., 0., 0.}, esb = 0.;
+    star_cons_bwd_s(ws, fb, S, wsb, esb);
+    if (left) {
+      hlld_star_bwd(wL, SL, SM, pts, s.eL, s.ptL, ws, es, wsb, esb, wLb, Sb, SMb, ptsb, eLb, ptLb);
+      SLb = Sb;
+    } else {
+      hlld_star_bwd(wR, SR, SM, pts, s.eR, s.ptR, ws, es, wsb, esb, wRb, Sb, SMb, ptsb, eRb, ptRb);
+      SRb = Sb;
+    }
+    hlld_tail_bwd(wL, wR, g, SL, SR, SM, pts, den, left, fb, -S, SLb, SRb, SMb, ptsb, eLb, eRb, ptLb, ptRb, wLb,
+                  wRb);
+    return;
+  }
+  // regions 3 and 4: both star states and the double-star state
   double wLs[8], wRs[8], eLs, eRs;
-  hlld_star(wL, SL, SM, pts, L.e, L.pt, wLs, eLs);
-  hlld_star(wR, SR, SM, pts, R.e, R.pt, wRs, eRs);
-  const double rl = sqrt(wLs[0]), rr = sqrt(wRs[0]), rlr = sqrt(wLs[0] * wRs[0]);
+  hlld_star(wL, SL, SM, pts, s.eL, s.ptL, wLs, eLs);
+  hlld_star(wR, SR, SM, pts, s.eR, s.ptR, wRs, eRs);
+  const double rlr = sqrt(wLs[0] * wRs[0]);
   const double sg = sign0(wL[5]), sgR = sign0(wR[5]);
   double wss[8];
   wss[0] = wLs[0];
@@ -601,88 +773,34 @@ __host__ __device__ inline void hlld_vjp(const double* wL, const double* wR, dou
   const double dss = vdotb(wss);
   const double eLss = eLs - rl * (vdotb(wLs) - dss) * sg;
   const double eRss = eRs + rr * (vdotb(wRs) - dss) * sgR;
-  const double SLs = SM - fabs(wL[5]) / rl;
-  const double SRs = SM + fabs(wR[5]) / rr;
-  double qLs[8], qRs[8], qLss[8], qRss[8];
-  star_cons(wLs, eLs, qLs);
-  star_cons(wRs, eRs, qRs);
-  star_cons(wss, eLss, qLss);
-  star_cons(wss, eRss, qRss);
-  int region = 0;
-  if (SL > 0) region = 1;
-  else if (SL <= 0. && SLs >= 0.) region = 2;
-  else if (SLs <= 0. && SM >= 0.) region = 3;
-  else if (SM <= 0. && SRs >= 0.) region = 4;
-  else if (SRs <= 0. && SR >= 0.) region = 5;
-  else if (SR < 0.) region = 6;
-  if (f) {
-    for (int k = 0; k < 8; k++) {
-      double Lq, LF, Rq, RF;
-      side_qF(wL, L.e, L.H, k, Lq, LF);
-      side_qF(wR, R.e, R.H, k, Rq, RF);
-      switch (region) {
-        case 1: f[k] = LF; break;
-        case 2: f[k] = LF + SL * qLs[k] - SL * Lq; break;
-        case 3: f[k] = LF + SLs * qLss[k] - (SLs - SL) * qLs[k] - SL * Lq; break;
-        case 4: f[k] = RF + SRs * qRss[k] - (SRs - SR) * qRs[k] - SR * Rq; break;
-        case 5: f[k] = RF + SR * qRs[k] - SR * Rq; break;
-        case 6: f[k] = RF; break;
-        default: f[k] = 0.; break;
-      }
-    }
-  }
-  // ---- reverse
-  double LFb[8] = {0}, RFb[8] = {0}, Lqb[8] = {0}, Rqb[8] = {0};
-  double qLsb[8] = {0}, qRsb[8] = {0}, qssb[8] = {0};
-  double SLb = 0., SRb = 0., SMb = 0., SLsb = 0., SRsb = 0., eLssb = 0., eRssb = 0.;
+  // region 3: f = FL + SL* q** - (SL* - SL) qL* - SL qL (region 4 the same on the right)
+  const double Ss = left ? SLs : SRs;
+  const double* wsa = left ? wLs : wRs;
+  const double esa = left ? eLs : eRs, essa = left ? eLss : eRss;
+  double Sb = 0., Ssb = 0.;
   for (int k = 0; k < 8; k++) {
-    const double b = fb[k];
-    double Lqk, LFk, Rqk, RFk;
-    side_qF(wL, L.e, L.H, k, Lqk, LFk);
-    side_qF(wR, R.e, R.H, k, Rqk, RFk);
-    (void)LFk;
-    (void)RFk;
-    switch (region) {
-      case 1: LFb[k] += b; break;
-      case 2:
-        LFb[k] += b;
-        qLsb[k] += SL * b;
-        Lqb[k] -= SL * b;
-        SLb += b * (qLs[k] - Lqk);
-        break;
-      case 3:
-        LFb[k] += b;
-        qssb[k] += SLs * b;
-        qLsb[k] -= (SLs - SL) * b;
-        Lqb[k] -= SL * b;
-        SLsb += b * (qLss[k] - qLs[k]);
-        SLb += b * (qLs[k] - Lqk);
-        break;
-      case 4:
-        RFb[k] += b;
-        qssb[k] += SRs * b;
-        qRsb[k] -= (SRs - SR) * b;
-        Rqb[k] -= SR * b;
-        SRsb += b * (qRss[k] - qRs[k]);
-        SRb += b * (qRs[k] - Rqk);
-        break;
-      case 5:
-        RFb[k] += b;
-        qRsb[k] += SR * b;
-        Rqb[k] -= SR * b;
-        SRb += b * (qRs[k] - Rqk);
-        break;
-      case 6: RFb[k] += b; break;
-      default: break;
-    }
+    double q, F;
+    side_qF(w, ee, HH, k, q, F);
+    const double qs = star_cons_k(wsa, esa, k), qss = star_cons_k(wss, essa, k);
+    if (f) f[k] = F + Ss * qss - (Ss - S) * qs - S * q;
+    Ssb += fb[k] * (qss - qs);
+    Sb += fb[k] * (qs - q);
   }
-  double wLsb[8] = {0}, wRsb[8] = {0}, wssb[8] = {0};
-  double eLsb = 0., eRsb = 0., rlb = 0., rrb = 0., rlrb = 0.;
-  // q** = star_cons(w**, e_L** or e_R**)
-  if (region == 3) star_cons_bwd(wss, qssb, wssb, eLssb);
-  if (region == 4) star_cons_bwd(wss, qssb, wssb, eRssb);
-  star_cons_bwd(wLs, qLsb, wLsb, eLsb);
-  star_cons_bwd(wRs, qRsb, wRsb, eRsb);
+  double wLsb[8] = {0., 0., 0., 0., 0., 0., 0., 0.}, wRsb[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  double wssb[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  double eLsb = 0., eRsb = 0., eLssb = 0., eRssb = 0., rlb = 0., rrb = 0., rlrb = 0.;
+  double SLsb = 0., SRsb = 0.;
+  if (left) {
+    star_cons_bwd_s(wss, fb, Ss, wssb, eLssb);
+    star_cons_bwd_s(wLs, fb, -(Ss - S), wLsb, eLsb);
+    SLsb = Ssb;
+    SLb = Sb;
+  } else {
+    star_cons_bwd_s(wss, fb, Ss, wssb, eRssb);
+    star_cons_bwd_s(wRs, fb, -(Ss - S), wRsb, eRsb);
+    SRsb = Ssb;
+    SRb = Sb;
+  }
   // S_L* = SM - |BxL| / rl, S_R* = SM + |BxR| / rr
   SMb += SLsb + SRsb;
   wLb[5] -= SLsb * sign0(wL[5]) / rl;
@@ -776,61 +894,10 @@ __host__ __device__ inline void hlld_vjp(const double* wL, const double* wR, dou
     wLsb[0] += pb * wRs[0];
     wRsb[0] += pb * wLs[0];
   }
-  // star states
-  double ptsb = 0., eLb = 0., eRb = 0., ptLb = 0., ptRb = 0.;
-  hlld_star_bwd(wL, SL, SM, pts, L.e, L.pt, wLs, eLs, wLsb, eLsb, wLb, SLb, SMb, ptsb, eLb, ptLb);
-  hlld_star_bwd(wR, SR, SM, pts, R.e, R.pt, wRs, eRs, wRsb, eRsb, wRb, SRb, SMb, ptsb, eRb, ptRb);
-  // SM, pts over den
-  const double dR = SR - vR, dL = SL - vL;
-  double dRb = 0., dLb = 0., denb = 0., vLb = 0., vRb = 0., wL0b = 0., wR0b = 0.;
-  {
-    const double Ns = SMb / den, Np = ptsb / den;
-    denb -= SMb * SM / den + ptsb * pts / den;
-    dRb += Ns * wR[0] * vR;
-    wR0b += Ns * dR * vR;
-    vRb += Ns * dR * wR[0];
-    dLb -= Ns * wL[0] * vL;
-    wL0b -= Ns * dL * vL;
-    vLb -= Ns * dL * wL[0];
-    ptRb -= Ns;
-    ptLb += Ns;
-    dRb += Np * wR[0] * L.pt;
-    wR0b += Np * dR * L.pt;
-    ptLb += Np * dR * wR[0];
-    dLb -= Np * wL[0] * R.pt;
-    wL0b -= Np * dL * R.pt;
-    ptRb -= Np * dL * wL[0];
-    const double dv = vR - vL, t = wL[0] * wR[0] * dR * dL;
-    wL0b += Np * wR[0] * dR * dL * dv;
-    wR0b += Np * wL[0] * dR * dL * dv;
-    dRb += Np * wL[0] * wR[0] * dL * dv;
-    dLb += Np * wL[0] * wR[0] * dR * dv;
-    vRb += Np * t;
-    vLb -= Np * t;
-  }
-  dRb += denb * wR[0];
-  wR0b += denb * dR;
-  dLb -= denb * wL[0];
-  wL0b -= denb * dL;
-  SRb += dRb;
-  vRb -= dRb;
-  SLb += dLb;
-  vLb -= dLb;
-  // SL = min(vL, vR) - cmax, SR = max(vL, vR) + cmax, cmax = max(cfL, cfR)
-  double cmaxb = -SLb + SRb;
-  if (vR < vL) vRb += SLb;
-  else vLb += SLb;
-  if (vL < vR) vRb += SRb;
-  else vLb += SRb;
-  double cfLb = 0., cfRb = 0.;
-  if (L.cf < R.cf) cfRb += cmaxb;
-  else cfLb += cmaxb;
-  wLb[0] += wL0b;
-  wRb[0] += wR0b;
-  wLb[1] += vLb;
-  wRb[1] += vRb;
-  side_bwd_vec(wL, g, L.e, L.H, fL, Lqb, LFb, 0., eLb, ptLb, cfLb, wLb);
-  side_bwd_vec(wR, g, R.e, R.H, fR, Rqb, RFb, 0., eRb, ptRb, cfRb, wRb);
+  hlld_star_bwd(wL, SL, SM, pts, s.eL, s.ptL, wLs, eLs, wLsb, eLsb, wLb, SLb, SMb, ptsb, eLb, ptLb);
+  hlld_star_bwd(wR, SR, SM, pts, s.eR, s.ptR, wRs, eRs, wRsb, eRsb, wRb, SRb, SMb, ptsb, eRb, ptRb);
+  hlld_tail_bwd(wL, wR, g, SL, SR, SM, pts, den, left, fb, -S, SLb, SRb, SMb, ptsb, eLb, eRb, ptLb, ptRb, wLb,
+                wRb);
 }
 
 }  // namespace amr
