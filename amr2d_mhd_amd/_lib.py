@@ -77,6 +77,7 @@ _SIGS = {
     "amr_recon_batch": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "amr_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_use_stream": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "amr_set_step_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_set_fill_cache": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_step_begin": (C.c_int, [C.c_void_p]),

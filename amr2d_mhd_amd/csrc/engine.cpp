@@ -363,7 +363,8 @@ struct amr_ctx {
   std::vector<Instance> inst;
   FieldPtrs F{};
   int64_t cap_slots = 0, used_slots = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every operation of the context runs on
+  hipStream_t own_stream = nullptr;  // the context's own stream (created with it)
   // device work lists and buffers (capacities in elements)
   // topology block, rebuilt at each tree change: the fill-program segment
   // tables and the leaf work list in one device allocation, filled by one
@@ -2604,7 +2605,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
   // error records, then the step kernel's barrier-failure flag
   c->rec_err_bytes = (sizeof(DevError) * n_inst + sizeof(int32_t) + 255) & ~size_t(255);
   const size_t rec_bytes = c->rec_err_bytes + sizeof(double) * 2 * (size_t)std::max<int64_t>(1, c->cap_slots);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
       hipMalloc((void**)&c->d_rec, rec_bytes) != hipSuccess ||
       hipMemset(c->d_rec, 0, rec_bytes) != hipSuccess ||
@@ -2619,6 +2620,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     amr_destroy(c);
     return AMR_ERR_HIP;
   }
+  c->stream = c->own_stream;
   std::memset(c->h_rec, 0, rec_bytes);
   c->d_err = (DevError*)c->d_rec;
   c->h_err = (DevError*)c->h_rec;
@@ -2650,6 +2652,7 @@ int amr_destroy(amr_ctx* c) {
   AMR_TRY
   if (!c) return AMR_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->own_stream) hipStreamSynchronize(c->own_stream);
   for (int f = 0; f < kNumFields; f++)
     if (c->F.f[f]) hipFree(c->F.f[f]);
   hipFree(c->d_topo);
@@ -2677,7 +2680,7 @@ int amr_destroy(amr_ctx* c) {
   hipFree(c->d_stamps);
   if (c->h_stamps) hipHostFree(c->h_stamps);
   if (c->h_span) hipHostFree(c->h_span);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
   return AMR_OK;
   AMR_CATCH
@@ -3263,6 +3266,20 @@ int amr_set_step_kernel(amr_ctx* c, int mode) {
   AMR_TRY
   if (!c || mode < -1 || mode > 2 || c->in_flight) return AMR_ERR_CONFIG;
   c->step_mode = mode < 0 ? 0 : mode;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_use_stream(amr_ctx* c, void* stream, int caller) {
+  AMR_TRY
+  if (!c || c->in_flight || c->in_group) return AMR_ERR_CONFIG;
+  hipStream_t next = caller ? (hipStream_t)stream : c->own_stream;
+  if (next == c->stream) return AMR_OK;
+  // what the context enqueued so far comes before what it enqueues next
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  HIPCHK(hipStreamWaitEvent(next, c->ev0, 0));
+  c->stream = next;
+  c->graph_dirty = true;
   return AMR_OK;
   AMR_CATCH
 }

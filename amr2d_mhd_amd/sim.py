@@ -170,6 +170,15 @@ class Simulation:
         as one launch, 0 launches per phase, -1 restores the default."""
         _check(self.h, lib().amr_set_step_kernel(self.h, int(mode)))
 
+    def use_stream(self, stream):
+        """Run the context on `stream` (a torch.cuda.Stream or a raw hipStream_t
+        handle as an int), or on its own stream again when `stream` is None."""
+        if stream is None:
+            _check(self.h, lib().amr_use_stream(self.h, None, 0))
+        else:
+            handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+            _check(self.h, lib().amr_use_stream(self.h, C.c_void_p(handle), 1))
+
     def set_graph(self, on=True):
         """Replay each step's device work as a HIP graph, or launch it directly (the default)."""
         _check(self.h, lib().amr_set_graph(self.h, 1 if on else 0))

@@ -176,6 +176,17 @@ int amr_riemann_batch(int solver, int dir, double gamma, int n, const double *wL
 int amr_recon_batch(int NT, int nghosts, const char *limiter, double dx, int nt, const double *W, double *out,
                     void *stream);
 
+/* Stream of the context (no counterpart in the reference, which runs on the
+ * host).  caller = 1: every later operation of the context (steps, phases,
+ * uploads, downloads, checkpoints) is enqueued on `stream`, a hipStream_t of
+ * the caller (NULL is the null stream), e.g. PyTorch's current stream, so
+ * that the caller orders its own work against the context's; caller = 0:
+ * back to the context's own stream (the default).  Work enqueued before the
+ * switch completes before work enqueued after it.  The caller keeps the
+ * stream alive while the context uses it.  Not between amr_step_begin and
+ * amr_step_end, nor inside amr_run_group (AMR_ERR_CONFIG). */
+int amr_use_stream(amr_ctx *ctx, void *stream, int caller);
+
 /* Graph replay of the step (no counterpart in the reference).  Disabled in
  * this build: amr_set_graph(ctx, 1) returns AMR_ERR_CONFIG; 0 (direct
  * launches, the default) is accepted. */

@@ -278,10 +278,10 @@ def test_rk3_mirror_boundaries_bitwise():
     lockstep(sim, dom, 40, full_every=10, label="BWx rk3")
 
 
-def test_rotor_lv5_first_2400_steps_equal_reference_log():
+def test_rotor_lv5_whole_run_equals_reference_log():
     """The bench workload against the compiled reference's own log
-    (tests/golden/run_rotor_MC_HLLD_lv5_2400.npy, serial build): the rotor, 5
-    levels, MC + HLLD, CFL 0.5, first 2400 steps (to t = 0.140, half the run).  Every step's
+    (tests/golden/run_rotor_MC_HLLD_lv5_full.npy, serial build): the rotor, 5
+    levels, MC + HLLD, CFL 0.5, the whole run, 4763 steps to t = 0.295.  Every step's
     applied dt, time, slot count and leaf count equal the reference's; the
     FNV-1a hashes of every slot field (U Us Bi Bis ff gg EMF) and of the
     quadtree -- the oracle's hash over the state downloaded from the GPU --
@@ -290,7 +290,7 @@ def test_rotor_lv5_first_2400_steps_equal_reference_log():
 
     import amr2d_mhd_amd as A
 
-    log = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "run_rotor_MC_HLLD_lv5_2400.npy"))
+    log = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "run_rotor_MC_HLLD_lv5_full.npy"))
     cfg = dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
     sim = A.Simulation(cfg)
     sim.init()
@@ -306,7 +306,7 @@ def test_rotor_lv5_first_2400_steps_equal_reference_log():
 
     assert hashes() == (int(log[0]["hf_post"]), int(log[0]["ht_post"]))
     n = len(log) - 1
-    assert n == 2400
+    assert n == 4763
     for s in range(n):
         applied = sim.info()["dt"]
         sim.step()
@@ -314,6 +314,8 @@ def test_rotor_lv5_first_2400_steps_equal_reference_log():
         assert (applied, inf["t"], inf["last_active"], inf["nleaf"]) == (r["dt"], r["t"], r["la_post"], r["nleaf_post"]), s
         if (s + 1) % 200 == 0:
             assert hashes() == (int(r["hf_post"]), int(r["ht_post"])), f"state hash differs after step {s}"
+    assert hashes() == (int(log[-1]["hf_post"]), int(log[-1]["ht_post"])), "final state hash differs"
+    assert sim.info()["t"] >= 0.295 and log[-2]["t"] < 0.295  # the run ends at the same step
     sim.close()
 
 

@@ -5,6 +5,7 @@ import ctypes as C
 import glob
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -38,6 +39,101 @@ def test_python_binding_covers_header():
     from amr2d_mhd_amd import _lib
 
     assert set(_declared_symbols()) <= set(_lib.EXPORTED)
+
+
+def _prototypes():
+    """name -> (return kind, [argument kinds]) of every entry point declared in
+    include/*.h; kinds: i32, i64, f64, str (char pointer), ptr, void."""
+    def kind(decl):
+        d = " ".join(decl.replace("const ", "").split())
+        if "*" in d:
+            return "str" if re.match(r"char\s*\*\s*\w*$", d) else "ptr"
+        t = d.split()[0]
+        return {"int": "i32", "int32_t": "i32", "int64_t": "i64", "double": "f64", "void": "void"}[t]
+
+    out = {}
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for ret, name, args in re.findall(r"([\w\s\*]+?)\b(amr_[a-z_0-9]+)\s*\(([^)]*)\)\s*;", text):
+            a = [x.strip() for x in args.split(",") if x.strip() and x.strip() != "void"]
+            out[name] = (kind(ret.strip() + (" *" if ret.strip().endswith("*") else "")), [kind(x) for x in a])
+    return out
+
+
+def _ctypes_kind(t):
+    if t is None:
+        return "void"
+    if t in (C.c_int, C.c_int32):
+        return "i32"
+    if t is C.c_int64:
+        return "i64"
+    if t is C.c_double:
+        return "f64"
+    if t is C.c_char_p:
+        return "str"
+    if t is C.c_void_p or (isinstance(t, type) and issubclass(t, C._Pointer)):
+        return "ptr"
+    raise AssertionError(f"unexpected ctypes type {t}")
+
+
+def test_ctypes_prototypes_match_header():
+    """The Python binding's prototypes agree with the header's declarations
+    (return and argument kinds, argument count) for every declared entry point."""
+    from amr2d_mhd_amd import _lib
+
+    protos = _prototypes()
+    assert len(protos) >= 40
+    for name, (ret, args) in protos.items():
+        cret, cargs = _lib._SIGS[name]
+        assert _ctypes_kind(cret) == ret, name
+        assert [_ctypes_kind(a) for a in cargs] == args, name
+
+
+def _build(tmp_path, cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True, cwd=REPO)
+    assert p.returncode == 0, p.stderr
+
+
+def test_header_is_c11_and_every_entry_point_links(tmp_path):
+    """tests/abi/abi_check.c: the header compiles as strict C11, every declared
+    entry point resolves against the built library, the struct layouts equal
+    the Python binding's, and the host-only tree validation accepts the base
+    tree and rejects a bad leaf flag."""
+    from amr2d_mhd_amd import _lib
+
+    src = os.path.join(REPO, "tests", "abi", "abi_check.c")
+    listed = set(re.findall(r"ENTRY\((amr_[a-z_0-9]+)\)", open(src).read()))
+    assert listed == set(_prototypes()), "abi_check.c must list every declared entry point"
+    exe = str(tmp_path / "abi_check")
+    libdir = os.path.join(REPO, "amr2d_mhd_amd")
+    _build(tmp_path, ["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", src, "-I" + os.path.join(REPO, "include"),
+                      "-L" + libdir, "-l:libamr2d_mhd.so", "-Wl,-rpath," + libdir, "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines()}
+    assert int(lines["entries"][0]) == len(listed)
+    for cname, S in (("amr_sim_config", _lib.SimConfig), ("amr_inst_info", _lib.InstInfo), ("amr_diag", _lib.Diag)):
+        v = lines[cname]
+        assert int(v[0]) == C.sizeof(S), cname
+        fields = {v[k]: (int(v[k + 1]), int(v[k + 2])) for k in range(1, len(v), 3)}
+        assert set(fields) == {f[0] for f in S._fields_}, cname
+        for f, _t in S._fields_:
+            d = getattr(S, f)
+            assert fields[f] == (d.offset, d.size), (cname, f)
+    assert lines["validate_base"] == ["0"] and lines["validate_bad"] == ["10"]
+
+
+def test_reference_loop_example_builds(tmp_path):
+    """The reference's time loop bound to the C ABI
+    (amr2d_mhd_amd/examples/reference_loop.cpp) compiles as C++17 without
+    warnings and links against the library."""
+    exe = str(tmp_path / "reference_loop")
+    libdir = os.path.join(REPO, "amr2d_mhd_amd")
+    _build(tmp_path, ["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror",
+                      os.path.join(REPO, "amr2d_mhd_amd", "examples", "reference_loop.cpp"),
+                      "-I" + os.path.join(REPO, "include"), "-L" + libdir, "-l:libamr2d_mhd.so",
+                      "-Wl,-rpath," + libdir, "-o", exe])
+    assert os.access(exe, os.X_OK)
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
