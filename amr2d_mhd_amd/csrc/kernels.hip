@@ -70,6 +70,91 @@ struct StageLds {
   static constexpr int kDoubles = 8 * N2 + 2 * NFL;
 };
 
+// Primitives of tile cell c into sWc[k * VS] (k = 0..7) with the checks of
+// meshblock.cpp:135-142 / 163-176, cell by cell: pressure, then velocity
+// NaN; a NaN primitive sets *sNan where the limiter's edge check applies.
+template <int NT, int NG>
+__device__ inline void stage_prim_cell(const double* __restrict__ Qin, double g, int lim, int step, const LeafItem& L,
+                                       DevError* err, int c, double* sWc, int VS, int* sNan) {
+  constexpr int N2 = NT * NT, LO = NG - 1, HI = NT - 1 - NG;
+  double u[8], w[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) u[k] = Qin[k * N2 + c];
+  cons_to_prim(u, g, w);
+#pragma unroll
+  for (int k = 0; k < 8; k++) sWc[k * VS] = w[k];
+  const int i = c % NT, j = c / NT;
+  if (w[4] <= 0 && i > LO && i < HI + 1 && j > LO && j < HI + 1)
+    report(err, L.inst, error_key(step, L.nb, 0, i, j, 0, L.nb, kErrPressure));
+  if (step == 2 && (isnan(w[1]) || isnan(w[2]) || isnan(w[3])))
+    report(err, L.inst, error_key(step, L.nb, 0, i, j, 1, L.nb, kErrNanVel));
+  if (lim < kLimWENO3) {
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) bad |= isnan(w[k]);
+    if (bad) *sNan = 1;
+  }
+}
+
+// Flux fl of the x face (ydir = 0: between cells (i, j) and (i + 1, j)) or
+// y face (between (i, j) and (i, j + 1)) from the primitives in sWb (cell
+// (i, j) of variable k at sWb[k * VS + j * NT + i]) and the face B, with the
+// solver-failure and NaN-flux checks of MUSCL2D.cpp:62-115.
+template <int NT, int SPEC>
+__device__ inline void stage_face(const double* sWb, int VS, const double* __restrict__ B, int lim, int solver,
+                                  double g, double dxl, int i, int j, int ydir, int step, const LeafItem& L,
+                                  DevError* err, double* fl) {
+  constexpr int N2 = NT * NT;
+  constexpr bool muscl_only = SPEC == kStageHLLDMuscl, weno5_only = SPEC == kStageHLLDWeno5;
+  const int s = ydir ? NT : 1;  // stride along the face normal
+  const int c = j * NT + i;     // left cell; right cell is c + s
+  double wl[8], wr[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const double* Wk = sWb + k * VS;
+    if (weno5_only || (!muscl_only && lim == kLimWENO5)) {
+      wl[k] = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
+      wr[k] = weno5_edge(Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], Wk[c + 3 * s], false);
+    } else if (!muscl_only && !weno5_only && lim == kLimWENO3) {
+      wl[k] = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
+      wr[k] = weno3_edge(Wk[c], Wk[c + s], Wk[c + 2 * s], false);
+    } else {
+      const double a = Wk[c] - Wk[c - s], b = Wk[c + s] - Wk[c], d = Wk[c + 2 * s] - Wk[c + s];
+      wl[k] = muscl_plus(lim, Wk[c], a, b, dxl);
+      wr[k] = muscl_minus(lim, Wk[c + s], b, d, dxl);
+    }
+  }
+  int st;
+  if (!ydir) {
+    wl[5] = B[0 * N2 + c + 1];
+    wr[5] = B[0 * N2 + c + 1];
+    st = normal_flux(solver, wl, wr, g, fl);
+  } else {
+    wl[6] = B[1 * N2 + c + NT];
+    wr[6] = B[1 * N2 + c + NT];
+    swap_xy(wl);
+    swap_xy(wr);
+    st = normal_flux(solver, wl, wr, g, fl);
+    swap_xy(fl);
+  }
+  // per (i, j) the x face (solver, NaN), then the y face
+  if (st) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 3 : 0, L.nb, kErrHLLC));
+  bool nan = false;
+#pragma unroll
+  for (int k = 0; k < 8; k++) nan |= isnan(fl[k]);
+  if (nan) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 4 : 1, L.nb, kErrNanFlux));
+}
+
+// RK update of component k of cell e of the tile from its residual
+// (main.cpp:147-157 / 179-189; SSP-RK3 build-defined)
+__device__ inline void stage_rk(int step, int rk, double dt, double res, const double* U, double* Us, double* Uw,
+                                int e) {
+  if (step == 1) Us[e] = U[e] - dt * res;
+  else if (rk == 2) Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
+  else if (step == 2) Us[e] = 0.75 * U[e] + 0.25 * (Us[e] - dt * res);
+  else Uw[e] = (1. / 3.) * U[e] + (2. / 3.) * (Us[e] - dt * res);
+}
+
 // One leaf, all TPB lanes of the workgroup; sW/sF/sG/sNan: the workgroup's
 // LDS (StageLds layout).  The caller separates two calls on the same LDS by a
 // workgroup barrier.
@@ -80,8 +165,7 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
   constexpr int N2 = S::N2, LO = S::LO, HI = S::HI, NF = S::NF;
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
-  constexpr bool muscl_only = SPEC == kStageHLLDMuscl, weno5_only = SPEC == kStageHLLDWeno5;
-  const int lim = weno5_only ? (int)kLimWENO5 : P.limiter;
+  const int lim = SPEC == kStageHLLDWeno5 ? (int)kLimWENO5 : P.limiter;
   const int solver = SPEC != kStageGeneric ? (int)kFluxHLLD : P.solver;
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
@@ -92,26 +176,7 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
   if (tid == 0) *sNan = 0;
   __syncthreads();
 
-  for (int c = tid; c < N2; c += TPB) {
-    double u[8], w[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) u[k] = Qin[k * N2 + c];
-    cons_to_prim(u, g, w);
-#pragma unroll
-    for (int k = 0; k < 8; k++) sW[k * N2 + c] = w[k];
-    const int i = c % NT, j = c / NT;
-    // meshblock.cpp:135-142 / 163-176, cell by cell: pressure, then velocity NaN
-    if (w[4] <= 0 && i > LO && i < HI + 1 && j > LO && j < HI + 1)
-      report(err, L.inst, error_key(step, L.nb, 0, i, j, 0, L.nb, kErrPressure));
-    if (step == 2 && (isnan(w[1]) || isnan(w[2]) || isnan(w[3])))
-      report(err, L.inst, error_key(step, L.nb, 0, i, j, 1, L.nb, kErrNanVel));
-    if (lim < kLimWENO3) {
-      bool bad = false;
-#pragma unroll
-      for (int k = 0; k < 8; k++) bad |= isnan(w[k]);
-      if (bad) *sNan = 1;
-    }
-  }
+  for (int c = tid; c < N2; c += TPB) stage_prim_cell<NT, NG>(Qin, g, lim, step, L, err, c, sW + c, N2, sNan);
   if (FLDS) {
     // stale edge fluxes ff[LO][j], gg[i][LO] enter the residual of ghost cells
     for (int q = tid; q < 8 * NF; q += TPB) {
@@ -130,43 +195,9 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
     const int ydir = q >= NF * NF;
     const int r = ydir ? q - NF * NF : q;
     const int i = LO + r % NF, j = LO + r / NF;
-    const int s = ydir ? NT : 1;  // stride along the face normal
-    const int c = j * NT + i;     // left cell; right cell is c + s
-    double wl[8], wr[8], fl[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double* Wk = sW + k * N2;
-      if (weno5_only || (!muscl_only && lim == kLimWENO5)) {
-        wl[k] = weno5_edge(Wk[c - 2 * s], Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], true);
-        wr[k] = weno5_edge(Wk[c - s], Wk[c], Wk[c + s], Wk[c + 2 * s], Wk[c + 3 * s], false);
-      } else if (!muscl_only && !weno5_only && lim == kLimWENO3) {
-        wl[k] = weno3_edge(Wk[c - s], Wk[c], Wk[c + s], true);
-        wr[k] = weno3_edge(Wk[c], Wk[c + s], Wk[c + 2 * s], false);
-      } else {
-        const double a = Wk[c] - Wk[c - s], b = Wk[c + s] - Wk[c], d = Wk[c + 2 * s] - Wk[c + s];
-        wl[k] = muscl_plus(lim, Wk[c], a, b, dxl);
-        wr[k] = muscl_minus(lim, Wk[c + s], b, d, dxl);
-      }
-    }
-    int st;
-    if (!ydir) {
-      wl[5] = B[0 * N2 + c + 1];
-      wr[5] = B[0 * N2 + c + 1];
-      st = normal_flux(solver, wl, wr, g, fl);
-    } else {
-      wl[6] = B[1 * N2 + c + NT];
-      wr[6] = B[1 * N2 + c + NT];
-      swap_xy(wl);
-      swap_xy(wr);
-      st = normal_flux(solver, wl, wr, g, fl);
-      swap_xy(fl);
-    }
-    // MUSCL2D.cpp:62-115: per (i, j) the x face (solver, NaN), then the y face
-    if (st) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 3 : 0, L.nb, kErrHLLC));
-    bool nan = false;
-#pragma unroll
-    for (int k = 0; k < 8; k++) nan |= isnan(fl[k]);
-    if (nan) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 4 : 1, L.nb, kErrNanFlux));
+    const int c = j * NT + i;
+    double fl[8];
+    stage_face<NT, SPEC>(sW, N2, B, lim, solver, g, dxl, i, j, ydir, step, L, err, fl);
     if (!ydir) {
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -205,12 +236,7 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
           res = (Fk[c + 1] - Fk[c]) / dxl + (Gk[c + NT] - Gk[c]) / dxl;
         }
       }
-      const int e = k * N2 + c;
-      if (step == 1) Us[e] = U[e] - dt * res;
-      else if (P.rk == 2) Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
-      // SSP-RK3 (Shu-Osher), build-defined
-      else if (step == 2) Us[e] = 0.75 * U[e] + 0.25 * (Us[e] - dt * res);
-      else Uw[e] = (1. / 3.) * U[e] + (2. / 3.) * (Us[e] - dt * res);
+      stage_rk(step, P.rk, dt, res, U, Us, Uw, k * N2 + c);
     }
   }
 }
@@ -226,6 +252,142 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
   if (span && threadIdx.x == 0) stamp_start(span);
   stage_block<NT, NG, TPB, FLDS, SPEC>(F, leaves[blockIdx.x], ip, step, err, lds, lds + 8 * S::N2,
                                        lds + 8 * S::N2 + S::NFL, &sNan);
+  if (span) {
+    __syncthreads();
+    if (threadIdx.x == 0) stamp_end(span);
+  }
+}
+
+// ------------------------------------------------------------ stage strips
+// The same stage with each leaf split into K row strips, one workgroup per
+// strip, so that small trees still fill the device and a CU holds several
+// strips whose load, face and update phases overlap.  The interior rows
+// [LO, HI] are divided into K nearly equal parts; the first strip also
+// updates the ghost rows below LO, the last those above HI.  A strip forms
+// the primitives of its rows and NG rows either side, the x faces of its
+// rows and the y faces below and above them -- the y faces on its lower
+// edge are also formed by the strip below, with the same operations and so
+// the same values, and written to gg by that strip only.  Results equal the
+// whole-leaf kernel's bit for bit.  The workgroup of item q = leaf * K +
+// strip is launched with id b such that q = (b % 8) * per + b / 8: the
+// strips of one leaf run on the same XCD (workgroups are dealt to the 8 XCDs
+// round robin) and share the rows they both read in its L2.
+template <int NT, int NG, int K>
+struct StripGeom {
+  static constexpr int LO = NG - 1, HI = NT - 1 - NG, NF = HI - LO + 1;
+  static constexpr int kRows = (NF + K - 1) / K;            // interior rows of the largest strip
+  static constexpr int kPrimRows = kRows + 2 * NG;           // primitive rows it needs
+  static constexpr int kFaces = (2 * kRows + 1) * NF;        // x faces + y faces, at most
+  static constexpr int kLanes = (kFaces + 63) / 64 * 64;
+  // doubles: primitives, x fluxes (stale edge face + NF per row), y fluxes
+  static constexpr int kDoubles = 8 * kPrimRows * NT + 8 * kRows * (NF + 1) + 8 * (kRows + 1) * NF;
+  __device__ static int row0(int s) { return LO + (s * NF) / K; }
+};
+
+template <int NT, int NG, int K, int SPEC>
+__global__ __launch_bounds__((StripGeom<NT, NG, K>::kLanes)) void stage_strip_kernel(
+    FieldPtrs F, const LeafItem* __restrict__ leaves, int nleaf, const InstParams* __restrict__ ip, int step,
+    DevError* err, unsigned long long* span, const int32_t* __restrict__ counts) {
+  using Gm = StripGeom<NT, NG, K>;
+  constexpr int N2 = NT * NT, LO = Gm::LO, HI = Gm::HI, NF = Gm::NF, TPB = Gm::kLanes;
+  constexpr int PR = Gm::kPrimRows, XR = Gm::kRows;
+  __shared__ double lds[Gm::kDoubles];
+  __shared__ int sNan;
+  if (counts) nleaf = counts[kCountLeaves];
+  const int per = (nleaf * K + 7) / 8;
+  const int item = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if ((int)(blockIdx.x / 8) >= per || item >= nleaf * K) return;
+  if (span && threadIdx.x == 0) stamp_start(span);
+  const LeafItem L = leaves[item / K];
+  const int strip = item % K;
+  const int jr0 = Gm::row0(strip), jr1 = Gm::row0(strip + 1) - 1;  // interior rows
+  const int r0 = strip == 0 ? 0 : jr0, r1 = strip == K - 1 ? NT - 1 : jr1;  // updated rows
+  const int pr0 = jr0 - NG > 0 ? jr0 - NG : 0, pr1 = jr1 + NG < NT - 1 ? jr1 + NG : NT - 1;
+  const int ya = jr0 > LO + 1 ? jr0 : LO + 1;  // first computed y-face row (face jj: cells jj - 1, jj)
+  const int nx = (jr1 - jr0 + 1) * NF, ny = (jr1 + 1 - ya + 1) * NF;
+  double* sW = lds;                     // [k][PR][NT], row pr0 first
+  double* sF = lds + 8 * PR * NT;       // [k][XR][NF + 1]: x faces LO .. HI + 1 of rows jr0 ..
+  double* sG = sF + 8 * XR * (NF + 1);  // [k][XR + 1][NF]: y faces of rows jr0 .. jr1 + 1
+  const double* sWb = sW - pr0 * NT;    // cell (i, j), variable k at sWb[k * PR * NT + j * NT + i]
+
+  const InstParams& P = ip[L.inst];
+  const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
+  const int lim = SPEC == kStageHLLDWeno5 ? (int)kLimWENO5 : P.limiter;
+  const int solver = SPEC != kStageGeneric ? (int)kFluxHLLD : P.solver;
+  const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
+  const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
+  const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
+  double* ff = F.f[kFF] + t8;
+  double* gg = F.f[kGG] + t8;
+  const int tid = threadIdx.x;
+  if (tid == 0) sNan = 0;
+  __syncthreads();
+
+  for (int c = pr0 * NT + tid; c <= pr1 * NT + NT - 1; c += TPB)
+    stage_prim_cell<NT, NG>(Qin, g, lim, step, L, err, c, sW + (c - pr0 * NT), PR * NT, &sNan);
+  // stale edge fluxes ff[LO][j] of the strip's rows and, in the first strip, gg[i][LO]
+  for (int q = tid; q < 8 * (jr1 - jr0 + 1); q += TPB) {
+    const int k = q / (jr1 - jr0 + 1), r = q % (jr1 - jr0 + 1);
+    sF[k * XR * (NF + 1) + r * (NF + 1)] = ff[k * N2 + (jr0 + r) * NT + LO];
+  }
+  if (jr0 == LO)
+    for (int q = tid; q < 8 * NF; q += TPB) {
+      const int k = q / NF, r = q % NF;
+      sG[k * (XR + 1) * NF + r] = gg[k * N2 + LO * NT + LO + r];
+    }
+  __syncthreads();
+  if (sNan) {
+    // the whole-leaf kernel reports this (and writes nothing) when any
+    // primitive of the tile is NaN; the strip holding it reports the same key
+    if (tid == 0) report(err, L.inst, error_key(step, L.nb, 1, 0, 0, 0, L.nb, kErrNanEdge));
+    return;
+  }
+
+  for (int q = tid; q < nx + ny; q += TPB) {
+    const int ydir = q >= nx;
+    const int r = ydir ? q - nx : q;
+    const int i = LO + r % NF;
+    // x face (i, i + 1) of row j, or y face jj = ya + r / NF between rows j = jj - 1 and jj
+    const int j = ydir ? ya + r / NF - 1 : jr0 + r / NF;
+    const int c = j * NT + i;
+    double fl[8];
+    stage_face<NT, SPEC>(sWb, PR * NT, B, lim, solver, g, dxl, i, j, ydir, step, L, err, fl);
+    if (!ydir) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        ff[k * N2 + c + 1] = fl[k];
+        sF[k * XR * (NF + 1) + (j - jr0) * (NF + 1) + (i + 1 - LO)] = fl[k];
+      }
+    } else {
+      const bool own = j >= jr0;  // the face below the strip belongs to the strip below
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (own) gg[k * N2 + c + NT] = fl[k];
+        sG[k * (XR + 1) * NF + (j + 1 - jr0) * NF + (i - LO)] = fl[k];
+      }
+    }
+  }
+  __syncthreads();
+
+  const double* U = F.f[kU] + t8;
+  double* Us = F.f[kUs] + t8;
+  double* Uw = F.f[kU] + t8;
+  for (int c = r0 * NT + tid; c <= r1 * NT + NT - 1; c += TPB) {
+    const int i = c % NT, j = c / NT;
+    const bool in = i >= LO && i <= HI && j >= LO && j <= HI;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      double res = 0.;
+      if (in) {
+        const double* Fk = sF + k * XR * (NF + 1);
+        const double* Gk = sG + k * (XR + 1) * NF;
+        const int fi = (j - jr0) * (NF + 1) + (i - LO);
+        const int gi = (j - jr0) * NF + (i - LO);
+        res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+      }
+      stage_rk(step, P.rk, dt, res, U, Us, Uw, k * N2 + c);
+    }
+  }
   if (span) {
     __syncthreads();
     if (threadIdx.x == 0) stamp_end(span);
@@ -882,6 +1044,20 @@ void launch_stage_variant(const FieldPtrs& F, const LeafItem* leaves, int nleaf,
   stage_kernel<T, G, TPB, FLDS, WPE, SPEC><<<nleaf, TPB, 0, s>>>(F, leaves, ip, step, err, span, counts);
 }
 
+template <int T, int G, int K, int SPEC>
+void launch_stage_strips(const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
+                         DevError* err, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                         unsigned long long* span, const int32_t* counts) {
+  constexpr int TPB = StripGeom<T, G, K>::kLanes;
+  const unsigned grid = 8u * (unsigned)((nleaf * K + 7) / 8);
+  if (ev_start && ev_stop) {
+    hipExtLaunchKernelGGL((stage_strip_kernel<T, G, K, SPEC>), dim3(grid), dim3(TPB), 0, s, ev_start, ev_stop, 0, F,
+                          leaves, nleaf, ip, step, err, span, counts);
+    return;
+  }
+  stage_strip_kernel<T, G, K, SPEC><<<grid, TPB, 0, s>>>(F, leaves, nleaf, ip, step, err, span, counts);
+}
+
 // WENO5 needs nghosts >= 3: the 24x24 tile only
 #define AMR_DISPATCH_TILE_WENO5(NT, NG, CALL)                         \
   do {                                                                \
@@ -901,7 +1077,17 @@ void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const Le
     if constexpr (SPEC == kStageHLLDWeno5) AMR_DISPATCH_TILE_WENO5(NT, ng, AMR_STAGE_CALL); \
     else AMR_DISPATCH_TILE(NT, ng, AMR_STAGE_CALL);                            \
   } while (0)
+#define AMR_STRIPS_CALL (launch_stage_strips<T, G, K, SPEC>(F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, \
+                                                            span, counts))
+#define AMR_STRIPS(K_)                                                          \
+  do {                                                                          \
+    constexpr int K = K_;                                                       \
+    if constexpr (SPEC == kStageHLLDWeno5) AMR_DISPATCH_TILE_WENO5(NT, ng, AMR_STRIPS_CALL); \
+    else AMR_DISPATCH_TILE(NT, ng, AMR_STRIPS_CALL);                            \
+  } while (0)
   switch (variant) {
+    case 6: AMR_STRIPS(2); break;
+    case 7: AMR_STRIPS(4); break;
     case 1: AMR_STAGE(512, true, 1); break;
     case 2: AMR_STAGE(768, true, 1); break;
     case 3: AMR_STAGE(256, false, 4); break;
@@ -911,6 +1097,8 @@ void launch_stage_spec(int NT, int ng, int variant, const FieldPtrs& F, const Le
   }
 #undef AMR_STAGE
 #undef AMR_STAGE_CALL
+#undef AMR_STRIPS
+#undef AMR_STRIPS_CALL
 }
 
 void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,

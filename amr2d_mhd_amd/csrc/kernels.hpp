@@ -41,11 +41,14 @@ bool tile_supported(int NT, int ng);
 // the face fluxes live for the RK update --
 //   0: 256 lanes, fluxes in LDS      1: 512 lanes, LDS      2: 768 lanes, LDS
 //   3: 256 lanes, fluxes read back from ff/gg, compiled for 4 waves per SIMD
-//   4: 512 lanes, read back, 4 waves per SIMD
+//   4: 512 lanes, read back, 4 waves per SIMD   5: 512 lanes, LDS, 4 waves per SIMD
+//   6: each leaf split into 2 row strips, one workgroup per strip (fluxes in LDS)
+//   7: 4 row strips per leaf
 // The fewer-lane variants suit large leaf counts (throughput), the wide ones
-// small leaf counts (per-block latency).  A non-null `span` receives the
-// launch's device wall-clock span (see stamp_start/stamp_end).
-constexpr int kStageVariants = 6;
+// and the strips small leaf counts (per-block latency, filling the device).
+// A non-null `span` receives the launch's device wall-clock span (see
+// stamp_start/stamp_end).
+constexpr int kStageVariants = 8;
 // stage-kernel specialisations: generic (solver and limiter read per leaf), or
 // HLLD with a MUSCL-family limiter, or HLLD with WENO5 (24x24 tiles, the only
 // ones with the ghost depth WENO5 needs), fixed at compile time

@@ -12,7 +12,7 @@ import numpy as np
 from ._lib import FIELD_VARS, FIELDS, FILL_KIND, SOLVERS, AmrError, Diag, InstInfo, lib, make_config
 
 # stage-kernel variants of the library (kStageVariants in csrc/kernels.hpp)
-STAGE_VARIANTS = 6
+STAGE_VARIANTS = 8
 
 
 def _check(ctx, rc):

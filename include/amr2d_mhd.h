@@ -213,7 +213,8 @@ int amr_set_step_kernel(amr_ctx *ctx, int mode);
  * for the RK update -- 0: 256 lanes, fluxes in LDS; 1: 512 lanes, LDS;
  * 2: 768 lanes, LDS; 3: 256 lanes, fluxes read back from ff/gg, 4 waves per
  * SIMD; 4: 512 lanes, read back, 4 waves per SIMD; 5: 512 lanes, LDS, 4 waves
- * per SIMD.  Each variant exists generic (solver and limiter read per
+ * per SIMD; 6: each leaf split into 2 row strips, one workgroup per strip;
+ * 7: 4 row strips.  Each variant exists generic (solver and limiter read per
  * instance), specialised to HLLD with a MUSCL-family limiter, and, for
  * nghosts 4, specialised to HLLD with WENO5; launches use a specialisation
  * when every running instance has its pair.  -1 (the default)

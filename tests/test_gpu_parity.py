@@ -235,7 +235,7 @@ def test_fill_program_cache_equals_recompiling():
     assert compiled < compiled_plain <= rebuilds, (compiled, compiled_plain, rebuilds)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_stage_variant_bitwise(variant):
     """Every stage-kernel variant reproduces the oracle bit for bit."""
     import amr2d_mhd_amd as A
@@ -247,6 +247,27 @@ def test_stage_variant_bitwise(variant):
     assert dom.init() == 0
     sim.init()
     lockstep(sim, dom, 40, full_every=10, label=f"stage variant {variant}")
+
+
+@pytest.mark.parametrize("variant", [6, 7])
+@pytest.mark.parametrize("cfg", [
+    dict(ic="rotor", limiter="MC", flux="HLLC", cfl=0.5, nx=16, nghosts=2, maxlevs=4),  # generic kernel
+    dict(ic="OT", limiter="WENO5", flux="HLLD", cfl=0.4, nx=16, nghosts=4, maxlevs=4, periodic=True, rk_stages=3),
+    dict(ic="BWx", limiter="Koren", flux="HLLD", cfl=0.4, nx=8, nghosts=2, maxlevs=4),  # 12x12 tiles
+], ids=["rotor-HLLC", "OT-WENO5-rk3", "BWx-nx8"])
+def test_stage_strips_bitwise(cfg, variant):
+    """The leaf split into 2 or 4 row strips (one workgroup each) reproduces
+    the oracle bit for bit: generic kernel, WENO5 on 24x24 tiles with RK3,
+    and 12x12 tiles."""
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(cfg)
+    sim.set_step_kernel(0)
+    sim.set_stage_variant(variant)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    lockstep(sim, dom, 30, full_every=10, label=f"strips {variant}")
 
 
 @pytest.mark.parametrize("lim,flux,rk,maxlevs,ng", [("WENO5", "HLLD", 3, 4, 4), ("MC", "HLLC", 2, 3, 2),
