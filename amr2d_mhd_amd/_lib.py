@@ -17,11 +17,12 @@ ERRORS = {0: "ok", 1: "non-positive pressure", 2: "NaN velocity", 3: "NaN flux",
           5: "NaN edge state", 6: "HLLC wave-speed failure", 7: "invalid dt", 8: "maxblocks exceeded",
           9: "NaN in refined face field", 10: "bad configuration", 11: "HIP runtime error",
           12: "slot arena exhausted", 13: "host-side failure (out of host memory)",
-          14: "checkpoint file missing, unwritable, truncated or corrupt"}
+          14: "checkpoint file missing, unwritable, truncated or corrupt",
+          15: "CTtype 2 EMF weight outside [0, 1]"}
 
 
 class SimConfig(C.Structure):
-    _fields_ = [("nx", C.c_int32), ("nghosts", C.c_int32), ("maxlevs", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("nx", C.c_int32), ("nghosts", C.c_int32), ("maxlevs", C.c_int32), ("ct_type", C.c_int32),
                 ("lenx", C.c_double), ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16),
                 ("flux", C.c_char * 8), ("ic_params", C.c_double * 8), ("periodic", C.c_int32),
                 ("rk_stages", C.c_int32)]
@@ -150,8 +151,9 @@ def lib():
 
 
 def make_config(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0,
-                ic_params=None, periodic=False, rk_stages=2):
+                ic_params=None, periodic=False, rk_stages=2, ct_type=1):
     cfg = SimConfig()
+    cfg.ct_type = int(ct_type)
     cfg.periodic = 1 if periodic else 0
     cfg.rk_stages = int(rk_stages)
     cfg.nx, cfg.nghosts, cfg.maxlevs, cfg.lenx, cfg.cfl = nx, nghosts, maxlevs, lenx, cfl

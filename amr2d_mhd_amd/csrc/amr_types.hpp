@@ -42,7 +42,8 @@ enum {
   kErrRefineNan = 9,  // refineBi_CT.cpp:47
   kErrConfig = 10,
   kErrHip = 11,
-  kErrCapacity = 12
+  kErrCapacity = 12,
+  kErrCtPsi = 15      // CT2D.cpp:60-61 (CTtype 2)
 };
 
 // one leaf block in a device work list
@@ -62,6 +63,8 @@ struct InstParams {
   int32_t solver;
   int32_t maxlevs;
   int32_t rk;         // RK stages: 2 (timeIntegral.cpp) or 3 (SSP-RK3)
+  int32_t ct_type;    // corner EMF: 1 (CTtype 1) or 2 (CTtype 2, CT2D.cpp:28-75)
+  int32_t pad;
 };
 
 // Ghost-fill program.  A reference to a value is either a pre-fill element
@@ -115,6 +118,9 @@ struct DevError {
   int32_t pad[6];
 };
 enum { kOrdCfl = 4, kOrdRefine = 5, kOrdCoarsen = 6 };
+// sequence of the CT phase of a stage: after every block's stage errors
+// (block ids stay below 2^15 for maxlevs <= 7), before the next stage's
+constexpr int kSeqCt = 0x8000;
 
 inline __host__ __device__ unsigned long long error_key(int ord, int seq, int phase, int i, int j, int chk, int nb,
                                                         int code) {

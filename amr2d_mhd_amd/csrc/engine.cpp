@@ -328,6 +328,7 @@ struct Instance {
   double t = 0, dt_main = 0, dt_dom = 0;
   int count = 0, status = 0;
   bool periodic = false;  // build-defined: wrapped root neighbours, domain [0, lenx]^2
+  int ct_type = 1;        // corner EMF: CTtype 1 or 2 (CT2D.cpp:15-75)
   amr_diag err{};
   bool done = false;
   // ghost-fill program of this instance (global slot ids); recompiled only
@@ -1767,6 +1768,8 @@ void fill_params(amr_ctx* c) {
     P.solver = I.solver;
     P.maxlevs = I.maxlevs;
     P.rk = c->rk;
+    P.ct_type = I.ct_type;
+    P.pad = 0;
   }
 }
 
@@ -2100,7 +2103,7 @@ void tune_collect(amr_ctx* c) {
 void run_ct(amr_ctx* c, int step) {
   {
     Timed tm(c, PF_CT);
-    launch_emf(c->NT, c->F, c->d_leaves, c->grid_leaves, c->stream, c->launch_counts);
+    launch_emf(c->NT, c->F, c->d_leaves, c->grid_leaves, c->d_ip, step, c->rk, c->d_err, c->stream, c->launch_counts);
   }
   run_fill(c, make_set(c, {{kEMF, kFillEdge}}));
   {
@@ -2569,7 +2572,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     I.maxlevs = g.maxlevs;
     I.cfl = g.cfl;
     if (g.nx != c->nxi || g.nghosts != c->ng || I.lim < 0 || I.solver < 0 || I.ic < 0 || g.maxlevs < 1 ||
-        g.maxlevs > 8 || ((I.lim == kLimWENO5) && c->ng < 3)) {
+        g.maxlevs > 8 || ((I.lim == kLimWENO5) && c->ng < 3) || g.ct_type < 0 || g.ct_type > 2) {
       delete c;
       return AMR_ERR_CONFIG;
     }
@@ -2578,6 +2581,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     // meshblock.cpp:90-96; a periodic domain needs its exact length, so it
     // uses lenx / (sqrt(nbroots) * nx) instead
     I.periodic = g.periodic != 0;
+    I.ct_type = g.ct_type == 2 ? 2 : 1;
     I.dx[0] = g.lenx / (std::sqrt((double)kNbRoots) * (c->NT - 2 * c->ng - (I.periodic ? 0 : 1)));
     for (int l = 1; l < kMaxLev; l++) I.dx[l] = I.dx[l - 1] / 2.;
     // IC2Dtype.cpp:18-80 (gamma, tEnd per problem)
@@ -2972,7 +2976,7 @@ struct CkptHeader {
   char magic[8];
   int32_t version, header_bytes;
   int32_t nx, nghosts, maxlevs, NT, rk, periodic, lim, solver, ic, last_active, count, status;
-  int32_t nfields, pad;
+  int32_t nfields, ct_type;
   double t, dt, dt_next, cfl, lenx, gamma, t_end;
   double ic_params[8];
 };
@@ -3007,6 +3011,7 @@ CkptHeader ckpt_header(const amr_ctx* c, const Instance& I) {
   h.count = I.count;
   h.status = I.status;
   h.nfields = kNumFields;
+  h.ct_type = I.ct_type;
   h.t = I.t;
   h.dt = I.dt_dom;
   h.dt_next = I.dt_main;
@@ -3022,7 +3027,8 @@ CkptHeader ckpt_header(const amr_ctx* c, const Instance& I) {
 bool ckpt_matches(const CkptHeader& a, const CkptHeader& b) {
   return a.nx == b.nx && a.nghosts == b.nghosts && a.maxlevs == b.maxlevs && a.NT == b.NT && a.rk == b.rk &&
          a.periodic == b.periodic && a.lim == b.lim && a.solver == b.solver && a.ic == b.ic &&
-         a.nfields == b.nfields && a.cfl == b.cfl && a.lenx == b.lenx && a.gamma == b.gamma &&
+         a.nfields == b.nfields && (a.ct_type == 2) == (b.ct_type == 2) && a.cfl == b.cfl && a.lenx == b.lenx &&
+         a.gamma == b.gamma &&
          a.t_end == b.t_end && std::memcmp(a.ic_params, b.ic_params, sizeof(a.ic_params)) == 0;
 }
 

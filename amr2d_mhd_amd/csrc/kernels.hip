@@ -395,23 +395,74 @@ __global__ __launch_bounds__((StripGeom<NT, NG, K>::kLanes)) void stage_strip_ke
 }
 
 // --------------------------------------------------------------------- CT
-// CT2D.cpp:15-27: corner EMF from the face fluxes for i,j in [1,NT-1]
+// CT2D.cpp:28-75 (CTtype 2) gas pressure and fast speed of one cell of the
+// stage state q (the reference's per-block p and C arrays, formed here per
+// corner for the four cells around it, by the same expressions)
+__device__ inline void ct2_pc(const double* q, int N2, int c, double g, double& p, double& C) {
+  const double q0 = q[c], q1 = q[N2 + c], q2 = q[2 * N2 + c], q3 = q[3 * N2 + c], q4 = q[4 * N2 + c],
+               q5 = q[5 * N2 + c], q6 = q[6 * N2 + c], q7 = q[7 * N2 + c];
+  p = (g - 1) * (q4 - 0.5 * ((q1 * q1 + q2 * q2 + q3 * q3) / q0 + (q5 * q5 + q6 * q6 + q7 * q7)));
+  C = sqrt((g * p + (q5 * q5 + q6 * q6 + q7 * q7)) / q0);
+}
+
+// CT2D.cpp:15-75: corner EMF from the face fluxes for i,j in [1,NT-1].
+// CTtype 1: the mean of the four face values.  CTtype 2 ("upwind"): the
+// y-flux and x-flux pairs weighted 1 - psi and psi, psi = dPx / (dPx + dPy)
+// where the pressure jump around the corner exceeds half the smallest
+// pressure (sw1) and the flow is compressive beyond a tenth of the smallest
+// fast speed (sw2), else psi = 0.5; the velocity divergence keeps the
+// reference's terms as written (two y-velocity quotients divide q2 by
+// itself, CT2D.cpp:45).  q is the stage state (Us, or U on the last stage).
 template <int NT>
-__device__ inline void emf_cell(const FieldPtrs& F, const LeafItem L, int c) {
+__device__ inline void emf_cell(const FieldPtrs& F, const LeafItem L, const InstParams* __restrict__ ip, int last,
+                                DevError* err, int step, int c) {
   constexpr int N2 = NT * NT;
   const int i = c % NT, j = c / NT;
   if (i < 1 || j < 1) return;
   const double* gg5 = F.f[kGG] + (size_t)L.gs * 8 * N2 + 5 * N2;
   const double* ff6 = F.f[kFF] + (size_t)L.gs * 8 * N2 + 6 * N2;
-  F.f[kEMF][(size_t)L.gs * N2 + c] = 0.25 * ((gg5[c - 1] + gg5[c]) - (ff6[c - NT] + ff6[c]));
+  const InstParams& P = ip[L.inst];
+  if (P.ct_type != 2) {
+    F.f[kEMF][(size_t)L.gs * N2 + c] = 0.25 * ((gg5[c - 1] + gg5[c]) - (ff6[c - NT] + ff6[c]));
+    return;
+  }
+  const double* q = F.f[last ? kU : kUs] + (size_t)L.gs * 8 * N2;
+  // cells (i, j), (i, j-1), (i-1, j), (i-1, j-1) of the reference's [i][j]
+  const int c00 = c, c0m = c - NT, cm0 = c - 1, cmm = c - 1 - NT;
+  double p00, p0m, pm0, pmm, C00, C0m, Cm0, Cmm;
+  ct2_pc(q, N2, c00, P.gamma, p00, C00);
+  ct2_pc(q, N2, c0m, P.gamma, p0m, C0m);
+  ct2_pc(q, N2, cm0, P.gamma, pm0, Cm0);
+  ct2_pc(q, N2, cmm, P.gamma, pmm, Cmm);
+  const double beta = 0.5, delta = 0.1;
+  const double dPx = 0.5 * fabs(p00 + p0m - pm0 - pmm);
+  const double dPy = 0.5 * fabs(p00 + pm0 - p0m - pmm);
+  const double* q0 = q;
+  const double* q1 = q + N2;
+  const double* q2 = q + 2 * N2;
+  const double divV = 0.5 * (q1[c00] / q0[c00] + q1[c0m] / q0[c0m] - q1[cm0] / q0[cm0] - q1[cmm] / q0[cmm]) +
+                      0.5 * (q2[c00] / q0[c00] + q2[cm0] / q2[cm0] - q2[c0m] / q2[c0m] - q2[cmm] / q0[cmm]);
+  double tP = pmm < p0m ? pmm : p0m;
+  const double tP1 = pm0 < p00 ? pm0 : p00;
+  tP = tP < tP1 ? tP : tP1;
+  const bool sw1 = (dPx + dPy) > beta * tP;
+  double tC = Cmm < C0m ? Cmm : C0m;
+  const double tC1 = Cm0 < C00 ? Cm0 : C00;
+  tC = tC < tC1 ? tC : tC1;
+  const bool sw2 = (-delta * tC) > divV;
+  double psi = 0.5;
+  if (sw1 && sw2) psi = dPx / (dPx + dPy);
+  if (psi < 0 || psi > 1) report(err, L.inst, error_key(step, kSeqCt | L.nb, 0, i, j, 0, L.nb, kErrCtPsi));
+  F.f[kEMF][(size_t)L.gs * N2 + c] = 0.5 * (1. - psi) * (gg5[c - 1] + gg5[c]) - 0.5 * psi * (ff6[c - NT] + ff6[c]);
 }
 
 template <int NT>
 __global__ __launch_bounds__(kThreads) void emf_kernel(FieldPtrs F, const LeafItem* __restrict__ leaves,
-                                                       const int32_t* __restrict__ counts) {
+                                                       const InstParams* __restrict__ ip, int last, DevError* err,
+                                                       int step, const int32_t* __restrict__ counts) {
   if (counts && (int)blockIdx.y >= counts[kCountLeaves]) return;
   const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c < NT * NT) emf_cell<NT>(F, leaves[blockIdx.y], c);
+  if (c < NT * NT) emf_cell<NT>(F, leaves[blockIdx.y], ip, last, err, step, c);
 }
 
 // CT2D.cpp:78-92: face-B RK update from the edge EMF.  Stage input B (Bi for
@@ -823,7 +874,7 @@ __global__ __launch_bounds__(TPB) void step_kernel(StepArgs A) {
       for (int e = blockIdx.x * TPB + tid; e < total; e += nwg * TPB) {
         const LeafItem L = A.leaves[e / N2];
         const int c = e % N2;
-        if (ph.kind == kPhaseEmf) emf_cell<NT>(A.F, L, c);
+        if (ph.kind == kPhaseEmf) emf_cell<NT>(A.F, L, A.ip, ph.step == A.ip[L.inst].rk, A.err, ph.step, c);
         else if (ph.kind == kPhaseBupdate) bupdate_cell<NT>(A.F, L, A.ip, ph.step, A.ip[L.inst].rk, c);
         else bsync_cell<NT>(A.F, L, ph.step, c);
       }
@@ -1116,10 +1167,12 @@ void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, con
                                      counts);
 }
 
-void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s, const int32_t* counts) {
+void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
+                int rk, DevError* err, hipStream_t s, const int32_t* counts) {
   if (nleaf <= 0) return;
   dim3 grid((NT * NT + kThreads - 1) / kThreads, nleaf);
-  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2), (emf_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, counts)));
+  AMR_DISPATCH_TILE(NT, (NT == 24 ? 4 : 2),
+                    (emf_kernel<T><<<grid, kThreads, 0, s>>>(F, leaves, ip, step == rk, err, step, counts)));
 }
 
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,

@@ -61,8 +61,9 @@ enum { kCountLeaves = 0, kCountChunks = 1 };
 void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, const LeafItem* leaves, int nleaf,
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start = nullptr,
                   hipEvent_t ev_stop = nullptr, unsigned long long* span = nullptr, const int32_t* counts = nullptr);
-void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, hipStream_t s,
-                const int32_t* counts = nullptr);
+// corner EMF of RK stage `step` of an `rk`-stage scheme (CTtype per instance)
+void launch_emf(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,
+                int rk, DevError* err, hipStream_t s, const int32_t* counts = nullptr);
 // face-B update of RK stage `step` of an `rk`-stage scheme; cell sync into U
 // (last stage) or Us
 void launch_bupdate(int NT, const FieldPtrs& F, const LeafItem* leaves, int nleaf, const InstParams* ip, int step,

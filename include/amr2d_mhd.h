@@ -33,7 +33,8 @@ enum {
   AMR_ERR_CAPACITY = 12,  /* slot arena of the context exhausted */
   AMR_ERR_HOST = 13,      /* host-side failure (out of host memory): no exception
                              crosses this interface; destroy the context */
-  AMR_ERR_IO = 14         /* checkpoint file missing, unwritable, truncated or corrupt */
+  AMR_ERR_IO = 14,        /* checkpoint file missing, unwritable, truncated or corrupt */
+  AMR_ERR_CT_PSI = 15     /* CT2D.cpp:60-61: CTtype 2 EMF weight outside [0, 1] */
 };
 
 /* slot fields (meshblock arrays, defs.hpp:69-83); layout [slot][var][j][i] */
@@ -46,7 +47,8 @@ typedef struct {
   int32_t nx;          /* interior cells per block side: 16 (or 8) */
   int32_t nghosts;     /* 2, or 4 for WENO5 */
   int32_t maxlevs;     /* number of AMR levels, 1..7 */
-  int32_t pad;
+  int32_t ct_type;     /* corner EMF of CT2D.cpp: 0 or 1 = CTtype 1 (the reference's setting,
+                          defs.hpp:34), 2 = CTtype 2, the upwind EMF of CT2D.cpp:28-75 */
   double lenx;         /* domain length parameter (lenx = leny) */
   double cfl;
   char ic[16];         /* SST SST1 SSTy BWx BWx1 BWy BWy1 rotor | rotor_p (parameterised rotor), OT */

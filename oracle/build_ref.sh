@@ -16,6 +16,8 @@
 #                            FMAs contracted) with -march=x86-64-v3 in place of
 #                            -march=native: built here, it runs on other hosts
 #                            (timing only: its FMAs change the results)
+#           ng<G>_lv<L>_ct2  serial, CTtype = 2 (defs.hpp:34, the upwind corner
+#                            EMF of CT2D.cpp:28-75) instead of 1
 #           ng<G>_lv<L>_bs   serial; its "rotor" is the Balsara-Spicer rotor of
 #                            the rotor workload's text (p = 1, v0 = 2,
 #                            Bx = 5/sqrt(4 pi)): a temporary copy of
@@ -26,7 +28,7 @@ HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${REF:-/root/reference}"
 OUT="$HERE/_ref"
 JOBS="${JOBS:-8}"
-VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp ng2_lv5_omp_native ng2_lv5_bs}"
+VARIANTS="${VARIANTS:-ng2_lv1 ng2_lv3 ng2_lv4 ng2_lv5 ng2_lv6 ng4_lv3 ng4_lv5 ng2_lv4_omp ng2_lv5_omp ng2_lv5_omp_native ng2_lv5_bs ng2_lv4_ct2 ng4_lv3_ct2}"
 
 if [ ! -f "$REF/defs.hpp" ]; then
   echo "build_ref: reference not present at $REF; skipping" >&2
@@ -61,12 +63,17 @@ build_variant() {
   omp=0; [[ "$v" == *_omp ]] && omp=1
   native=0; [[ "$v" == *_omp_native ]] && native=1
   bs=0; [[ "$v" == *_bs ]] && bs=1
+  ct2=0; [[ "$v" == *_ct2 ]] && ct2=1
   local inc="$TMP/$v" obj="$OUT/$v/obj" bin="$OUT/$v/refrun"
   mkdir -p "$inc" "$obj"
   sed -e "s/^const int nghosts=2;/const int nghosts=${ng};/" \
       -e "s/^const int maxlevs=5;/const int maxlevs=${lv};/" "$REF/defs.hpp" > "$inc/defs.hpp"
   grep -q "^const int nghosts=${ng};" "$inc/defs.hpp" || { echo "patch nghosts failed" >&2; return 1; }
   grep -q "^const int maxlevs=${lv};" "$inc/defs.hpp" || { echo "patch maxlevs failed" >&2; return 1; }
+  if [ "$ct2" = 1 ]; then
+    sed -i -e "s/^const int CTtype=1;/const int CTtype=2;/" "$inc/defs.hpp"
+    grep -q "^const int CTtype=2;" "$inc/defs.hpp" || { echo "patch CTtype failed" >&2; return 1; }
+  fi
   if [ "$bs" = 1 ]; then
     # rotor constants of IC2Dtype.cpp:79-80, 100-101, 104-105 (v0 = 2: both
     # velocity components doubled, an exact scaling)

@@ -83,6 +83,7 @@ struct ora_dom {
   double icp[8];
   int periodic; /* build-defined: wrapped root neighbours, exact geometry */
   int rk;       /* 2 (timeIntegral.cpp) or 3 (build-defined SSP-RK3) */
+  int ct;       /* CT2D.cpp corner EMF: 1 (CTtype 1) or 2 (CTtype 2, upwind) */
 };
 
 #define TILE(d, fld, nb) ((d)->f[fld] + (size_t)(nb) * NVAR[fld] * (d)->NT * (d)->NT)
@@ -593,6 +594,8 @@ ora_dom *ora_create(const ora_config *c) {
   memcpy(d->icp, c->ic_params, sizeof d->icp);
   d->periodic = c->periodic != 0;
   d->rk = c->rk_stages == 3 ? 3 : 2;
+  if (c->ct_type < 0 || c->ct_type > 2) { free(d); return NULL; }
+  d->ct = c->ct_type == 2 ? 2 : 1;
   if (c->rk_stages != 0 && c->rk_stages != 2 && c->rk_stages != 3) { free(d); return NULL; }
   if (d->lim == L_BAD || d->flux == F_BAD || d->ic == IC_BAD) { free(d); return NULL; }
   /* meshblock.cpp:90-96; a periodic domain needs its exact length, so it
@@ -1143,21 +1146,83 @@ static double face_rk(int rk, int step, double B, double Bn, double B0, double d
   return (1. / 3.) * B0 + (2. / 3.) * (B - dt * res);
 }
 
-/* CT2D.cpp:11-124 (CTtype 1): corner EMF, edge fill, face-B update, face
- * fill, then cell-centred B from faces with the energy correction. */
+/* CT2D.cpp:28-75 (CTtype 2, "upwind"): per block, gas pressure p and fast
+ * speed C of every cell of the stage state q, then each corner (i, j) in
+ * [1, NT-1]^2 weighs the x-flux and y-flux averages by psi = dPx / (dPx +
+ * dPy) where a strong pressure jump (sw1) meets a compressive flow (sw2),
+ * else 0.5.  The velocity divergence keeps the reference's terms as written
+ * (two of its y-velocity quotients divide q2 by itself, CT2D.cpp:45). */
+static int ct_upwind_block(ora_dom *d, int nb, const double *q, const double *ff, const double *gg, double *E,
+                           double *p, double *C) {
+  const int NT = d->NT;
+  const double g = d->gamma;
+  for (int i = 0; i < NT; i++)
+    for (int j = 0; j < NT; j++) {
+      const double q0 = AT(q, NT, 0, i, j), q1 = AT(q, NT, 1, i, j), q2 = AT(q, NT, 2, i, j),
+                   q3 = AT(q, NT, 3, i, j), q4 = AT(q, NT, 4, i, j), q5 = AT(q, NT, 5, i, j),
+                   q6 = AT(q, NT, 6, i, j), q7 = AT(q, NT, 7, i, j);
+      p[i * NT + j] = (g - 1) * (q4 - 0.5 * ((q1 * q1 + q2 * q2 + q3 * q3) / q0 + (q5 * q5 + q6 * q6 + q7 * q7)));
+      C[i * NT + j] = sqrt((g * p[i * NT + j] + (q5 * q5 + q6 * q6 + q7 * q7)) / q0);
+    }
+#define P_(a, b) p[(a) * NT + (b)]
+#define C_(a, b) C[(a) * NT + (b)]
+#define Q_(k, a, b) AT(q, NT, k, a, b)
+  for (int i = 1; i < NT; i++)
+    for (int j = 1; j < NT; j++) {
+      const double beta = 0.5, delta = 0.1;
+      const double dPx = 0.5 * fabs(P_(i, j) + P_(i, j - 1) - P_(i - 1, j) - P_(i - 1, j - 1));
+      const double dPy = 0.5 * fabs(P_(i, j) + P_(i - 1, j) - P_(i, j - 1) - P_(i - 1, j - 1));
+      const double divV = 0.5 * (Q_(1, i, j) / Q_(0, i, j) + Q_(1, i, j - 1) / Q_(0, i, j - 1) -
+                                 Q_(1, i - 1, j) / Q_(0, i - 1, j) - Q_(1, i - 1, j - 1) / Q_(0, i - 1, j - 1)) +
+                          0.5 * (Q_(2, i, j) / Q_(0, i, j) + Q_(2, i - 1, j) / Q_(2, i - 1, j) -
+                                 Q_(2, i, j - 1) / Q_(2, i, j - 1) - Q_(2, i - 1, j - 1) / Q_(0, i - 1, j - 1));
+      double tP = P_(i - 1, j - 1) < P_(i, j - 1) ? P_(i - 1, j - 1) : P_(i, j - 1);
+      const double tP1 = P_(i - 1, j) < P_(i, j) ? P_(i - 1, j) : P_(i, j);
+      tP = tP < tP1 ? tP : tP1;
+      const int sw1 = (dPx + dPy) > beta * tP;
+      double tC = C_(i - 1, j - 1) < C_(i, j - 1) ? C_(i - 1, j - 1) : C_(i, j - 1);
+      const double tC1 = C_(i - 1, j) < C_(i, j) ? C_(i - 1, j) : C_(i, j);
+      tC = tC < tC1 ? tC : tC1;
+      const int sw2 = (-delta * tC) > divV;
+      double psi = 0.5;
+      if (sw1 && sw2) psi = dPx / (dPx + dPy);
+      if (psi < 0 || psi > 1) return fail(d, ORA_ERR_CT_PSI, nb, i, j);
+      AT(E, NT, 0, i, j) = 0.5 * (1. - psi) * (AT(gg, NT, 5, i - 1, j) + AT(gg, NT, 5, i, j)) -
+                           0.5 * psi * (AT(ff, NT, 6, i, j - 1) + AT(ff, NT, 6, i, j));
+    }
+#undef P_
+#undef C_
+#undef Q_
+  return 0;
+}
+
+/* CT2D.cpp:11-124: corner EMF (CTtype 1, or CTtype 2 above), edge fill,
+ * face-B update, face fill, then cell-centred B from faces with the energy
+ * correction. */
 int ora_ct(ora_dom *d, int step) {
   const int NT = d->NT;
   /* last stage updates Bi and U; earlier stages Bis and Us */
   const int last = step == d->rk;
   const int qf = last ? ORA_U : ORA_US, bf = step == 1 ? ORA_BI : ORA_BIS, nf = last ? ORA_BI : ORA_BIS;
+  double *pc = d->ct == 2 ? (double *)malloc(sizeof(double) * 2 * NT * NT) : NULL;
+  if (d->ct == 2 && !pc) return fail(d, ORA_ERR_CONFIG, -1, 0, 0);
   for (int nb = 0; nb < d->lastActive; nb++) {
     if (!d->leafs[nb]) continue;
     double *E = TILE(d, ORA_EMF, nb), *ff = TILE(d, ORA_FF, nb), *gg = TILE(d, ORA_GG, nb);
+    if (d->ct == 2) {
+      const int e = ct_upwind_block(d, nb, TILE(d, qf, nb), ff, gg, E, pc, pc + NT * NT);
+      if (e) {
+        free(pc);
+        return e;
+      }
+      continue;
+    }
     for (int i = 1; i < NT; i++)
       for (int j = 1; j < NT; j++)
         AT(E, NT, 0, i, j) = 0.25 * ((AT(gg, NT, 5, i - 1, j) + AT(gg, NT, 5, i, j)) -
                                      (AT(ff, NT, 6, i, j - 1) + AT(ff, NT, 6, i, j)));
   }
+  free(pc);
   ora_fill(d, ORA_EMF, ORA_FILL_EC);
   for (int nb = 0; nb < d->lastActive; nb++) {
     if (!d->leafs[nb]) continue;

@@ -34,7 +34,8 @@ enum {
   ORA_ERR_DT = 7,       /* main.cpp:235-238 */
   ORA_ERR_OVERFLOW = 8, /* admesh.cpp:42-45 */
   ORA_ERR_REFINE_NAN = 9, /* refineBi_CT.cpp:47 */
-  ORA_ERR_CONFIG = 10
+  ORA_ERR_CONFIG = 10,
+  ORA_ERR_CT_PSI = 15     /* CT2D.cpp:60: upwind EMF weight outside [0, 1] */
 };
 
 typedef struct {
@@ -50,6 +51,8 @@ typedef struct {
                           rho_in, v0, p, Bx, r0, r1, gamma, t_end */
   int periodic;        /* build-defined: wrapped root neighbours, domain [0,lenx]^2 */
   int rk_stages;       /* 0 or 2: timeIntegral.cpp RK2; 3: SSP-RK3 (build-defined) */
+  int ct_type;         /* CT2D.cpp corner EMF: 0 or 1 = CTtype 1 (defs.hpp:34, the reference's
+                          setting), 2 = CTtype 2 (upwind, CT2D.cpp:28-75) */
 } ora_config;
 
 typedef struct ora_dom ora_dom;

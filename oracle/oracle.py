@@ -20,7 +20,8 @@ FILL_CC, FILL_FC, FILL_EC = 0, 1, 2
 class OraConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("nghosts", C.c_int), ("maxlevs", C.c_int), ("lenx", C.c_double),
                 ("cfl", C.c_double), ("ic", C.c_char * 16), ("limiter", C.c_char * 16), ("flux", C.c_char * 8),
-                ("ic_params", C.c_double * 8), ("periodic", C.c_int), ("rk_stages", C.c_int)]
+                ("ic_params", C.c_double * 8), ("periodic", C.c_int), ("rk_stages", C.c_int),
+                ("ct_type", C.c_int)]
 
 
 _lib = None
@@ -83,10 +84,11 @@ class Domain:
     """One oracle simulation (the reference's meshblock + main loop)."""
 
     def __init__(self, ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=4, lenx=1.0,
-                 ic_params=None, periodic=False, rk_stages=2):
+                 ic_params=None, periodic=False, rk_stages=2, ct_type=1):
         cfg = OraConfig(nx, nghosts, maxlevs, lenx, cfl, ic.encode(), limiter.encode(), flux.encode())
         cfg.periodic = 1 if periodic else 0
         cfg.rk_stages = int(rk_stages)
+        cfg.ct_type = int(ct_type)
         if ic_params is not None:
             for q, v in enumerate(ic_params):
                 cfg.ic_params[q] = float(v)

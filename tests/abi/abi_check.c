@@ -36,7 +36,7 @@ int main(void) {
 
   printf("amr_sim_config %zu", sizeof(amr_sim_config));
   FIELD(amr_sim_config, nx); FIELD(amr_sim_config, nghosts); FIELD(amr_sim_config, maxlevs);
-  FIELD(amr_sim_config, pad); FIELD(amr_sim_config, lenx); FIELD(amr_sim_config, cfl);
+  FIELD(amr_sim_config, ct_type); FIELD(amr_sim_config, lenx); FIELD(amr_sim_config, cfl);
   FIELD(amr_sim_config, ic); FIELD(amr_sim_config, limiter); FIELD(amr_sim_config, flux);
   FIELD(amr_sim_config, ic_params); FIELD(amr_sim_config, periodic); FIELD(amr_sim_config, rk_stages);
   printf("\namr_inst_info %zu", sizeof(amr_inst_info));

@@ -24,6 +24,11 @@ RUNS = [
     ("run_BWy_minmod_RUSA_lv3", dict(ng=2, lv=3, ic="BWy", lim="minmod", flux="RUSA", cfl=0.4), 60),
     ("run_SST_Superbee_RUSA_lv3", dict(ng=2, lv=3, ic="SST", lim="Superbee", flux="RUSA", cfl=0.4), 60),
     ("run_rotor_WENO5_HLLD_lv3_ng4", dict(ng=4, lv=3, ic="rotor", lim="WENO5", flux="HLLD", cfl=0.4), 40),
+    # CTtype 2 (the upwind corner EMF of CT2D.cpp:28-75), reference built with CTtype = 2
+    ("run_rotor_MC_HLLD_lv4_ct2", dict(ng=2, lv=4, ic="rotor", lim="MC", flux="HLLD", cfl=0.5, ct2=True), 300),
+    ("run_rotor_minmod_HLLE_lv4_ct2", dict(ng=2, lv=4, ic="rotor", lim="minmod", flux="HLLE", cfl=0.5, ct2=True), 200),
+    ("run_rotor_WENO5_HLLD_lv3_ng4_ct2", dict(ng=4, lv=3, ic="rotor", lim="WENO5", flux="HLLD", cfl=0.4, ct2=True),
+     60),
     # the bench workload's whole run, 4763 steps to t = 0.295 (GPU test against the reference
     # directly; about 2 h of one core)
     ("run_rotor_MC_HLLD_lv5_full", dict(ng=2, lv=5, ic="rotor", lim="MC", flux="HLLD", cfl=0.5), None),

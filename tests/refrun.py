@@ -17,12 +17,12 @@ LOG_DTYPE = np.dtype([("step", "i4"), ("la_pre", "i4"), ("nleaf_pre", "i4"), ("l
 STATE_FIELDS = [("U", 8), ("Us", 8), ("W", 8), ("Bi", 2), ("Bis", 2), ("ff", 8), ("gg", 8), ("EMF", 1)]
 
 
-def refrun_path(ng, lv, omp=False):
-    return os.path.join(REF_DIR, f"ng{ng}_lv{lv}{'_omp' if omp else ''}", "refrun")
+def refrun_path(ng, lv, omp=False, ct2=False):
+    return os.path.join(REF_DIR, f"ng{ng}_lv{lv}{'_omp' if omp else ''}{'_ct2' if ct2 else ''}", "refrun")
 
 
-def have_ref(ng=2, lv=4, omp=False):
-    return os.access(refrun_path(ng, lv, omp), os.X_OK)
+def have_ref(ng=2, lv=4, omp=False, ct2=False):
+    return os.access(refrun_path(ng, lv, omp, ct2), os.X_OK)
 
 
 def input_line(nx, lenx, cfl, ic, lim, flux):
@@ -30,8 +30,8 @@ def input_line(nx, lenx, cfl, ic, lim, flux):
 
 
 def run(workdir, ng, lv, nx=16, lenx=1.0, cfl=0.4, ic="BWx", lim="MC", flux="RUSA", steps=None, dumps=(),
-        trace=None, randtree=None, omp=False, timing=False, env=None, timeout=3600):
-    exe = refrun_path(ng, lv, omp)
+        trace=None, randtree=None, omp=False, timing=False, env=None, timeout=3600, ct2=False):
+    exe = refrun_path(ng, lv, omp, ct2)
     args = [exe, "run", "--dir", workdir, "--input", input_line(nx, lenx, cfl, ic, lim, flux),
             "--log", os.path.join(workdir, "log.bin")]
     if steps is not None:

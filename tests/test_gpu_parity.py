@@ -400,7 +400,8 @@ def test_orszag_tang_lv5_stops_where_the_oracle_does():
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
-@pytest.mark.parametrize("case", ["BWx lv4 full run", "rotor lv5", "OT WENO5 rk3 lv4", "rotor HLLC lv4 generic"])
+@pytest.mark.parametrize("case", ["BWx lv4 full run", "rotor lv5", "OT WENO5 rk3 lv4", "rotor HLLC lv4 generic",
+                                  "rotor CTtype 2 lv4"])
 def test_step_kernel_modes_bitwise(case, mode):
     """Per-phase launches (mode 0) and the persistent step kernel with 256 or
     512 lanes (modes 1, 2) all reproduce the oracle bit for bit, through the
@@ -412,7 +413,8 @@ def test_step_kernel_modes_bitwise(case, mode):
             "rotor lv5": (dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, maxlevs=5), 150),
             "OT WENO5 rk3 lv4": (dict(ic="OT", limiter="WENO5", flux="HLLD", cfl=0.4, maxlevs=4, nghosts=4,
                                       periodic=True, rk_stages=3), 40),
-            "rotor HLLC lv4 generic": (dict(ic="rotor", limiter="MC", flux="HLLC", cfl=0.5, maxlevs=4), 60)}
+            "rotor HLLC lv4 generic": (dict(ic="rotor", limiter="MC", flux="HLLC", cfl=0.5, maxlevs=4), 60),
+            "rotor CTtype 2 lv4": (dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, maxlevs=4, ct_type=2), 60)}
     cfg, steps = cfgs[case]
     cfg = dict(dict(nx=16, nghosts=2), **cfg)
     sim = A.Simulation(cfg)
@@ -461,3 +463,72 @@ def test_graph_replay_is_disabled():
         sim.set_graph(True)
     assert e.value.code == 10
     sim.set_graph(False)
+
+
+CT2_CASES = {
+    "rotor MC HLLD lv4": (dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=4), 300,
+                          "run_rotor_MC_HLLD_lv4_ct2"),
+    "rotor minmod HLLE lv4": (dict(ic="rotor", limiter="minmod", flux="HLLE", cfl=0.5, nx=16, nghosts=2, maxlevs=4),
+                              200, "run_rotor_minmod_HLLE_lv4_ct2"),
+    "rotor WENO5 HLLD lv3": (dict(ic="rotor", limiter="WENO5", flux="HLLD", cfl=0.4, nx=16, nghosts=4, maxlevs=3),
+                             60, "run_rotor_WENO5_HLLD_lv3_ng4_ct2"),
+}
+
+
+@pytest.mark.parametrize("case", list(CT2_CASES))
+def test_ct_type2_equals_reference_log(case):
+    """CTtype 2 (the upwind corner EMF of CT2D.cpp:28-75) against the log of
+    the reference built with CTtype = 2: every step's dt, time and tree sizes,
+    and the hashes of every slot field and of the tree every 20 steps and at
+    the end.  On the rotor the upwind weight dPx / (dPx + dPy) is taken at a
+    fraction of the corners, so both branches of the weight are exercised."""
+    import os
+
+    import amr2d_mhd_amd as A
+
+    cfg, steps, name = CT2_CASES[case]
+    log = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".npy"))
+    assert len(log) - 1 == steps
+    sim = A.Simulation(dict(cfg, ct_type=2))
+    sim.init()
+    dom = O.Domain(**dict(cfg, ct_type=2))  # used for its hash only
+    assert dom.init() == 0
+
+    def hashes():
+        st = sim.state()
+        dom.set_tree(st["lp"], st["icoord"], st["leafs"])
+        for fname, f in FIELD_MAP.items():
+            dom.set_field(f, st[fname])
+        return dom.hash_fields(), dom.hash_tree()
+
+    assert hashes() == (int(log[0]["hf_post"]), int(log[0]["ht_post"]))
+    for s in range(steps):
+        applied = sim.info()["dt"]
+        sim.step()
+        r, inf = log[s + 1], sim.info()
+        assert (applied, inf["t"], inf["last_active"], inf["nleaf"]) == (r["dt"], r["t"], r["la_post"], r["nleaf_post"]), s
+        if (s + 1) % 20 == 0 or s + 1 == steps:
+            assert hashes() == (int(r["hf_post"]), int(r["ht_post"])), f"state hash differs after step {s}"
+    sim.close()
+
+
+def test_ct_type2_lockstep_with_oracle_rk3_periodic():
+    """CTtype 2 with the build-defined SSP-RK3 and periodic Orszag-Tang (no
+    reference counterpart): lockstep with the oracle."""
+    import amr2d_mhd_amd as A
+
+    cfg = dict(ic="OT", limiter="MC", flux="HLLD", cfl=0.4, nx=16, nghosts=2, maxlevs=4, periodic=True,
+               rk_stages=3, ct_type=2)
+    sim = A.Simulation(cfg)
+    dom = O.Domain(**cfg)
+    assert dom.init() == 0
+    sim.init()
+    lockstep(sim, dom, 40, full_every=10, label="OT CTtype 2 rk3")
+
+
+def test_ct_type_rejected_outside_range():
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd import AmrError
+
+    with pytest.raises(AmrError):
+        A.Simulation(dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=4, ct_type=3))

@@ -324,6 +324,14 @@ def test_brio_wu_lv4_full_run_golden():
     ("run_BWy_minmod_RUSA_lv3", dict(ic="BWy", limiter="minmod", flux="RUSA", cfl=0.4, maxlevs=3, nghosts=2), 60),
     ("run_SST_Superbee_RUSA_lv3", dict(ic="SST", limiter="Superbee", flux="RUSA", cfl=0.4, maxlevs=3, nghosts=2), 60),
     ("run_rotor_WENO5_HLLD_lv3_ng4", dict(ic="rotor", limiter="WENO5", flux="HLLD", cfl=0.4, maxlevs=3, nghosts=4), 40),
+    # CTtype 2 (CT2D.cpp:28-75), against the reference built with CTtype = 2; on the rotor the
+    # upwind weight dPx / (dPx + dPy) is taken at ~0.4% of the corners, the rest use 0.5
+    ("run_rotor_MC_HLLD_lv4_ct2", dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, maxlevs=4, nghosts=2,
+                                       ct_type=2), 300),
+    ("run_rotor_minmod_HLLE_lv4_ct2", dict(ic="rotor", limiter="minmod", flux="HLLE", cfl=0.5, maxlevs=4, nghosts=2,
+                                           ct_type=2), 200),
+    ("run_rotor_WENO5_HLLD_lv3_ng4_ct2", dict(ic="rotor", limiter="WENO5", flux="HLLD", cfl=0.4, maxlevs=3,
+                                              nghosts=4, ct_type=2), 60),
 ])
 def test_runs_golden(name, kw, steps):
     log = np.load(os.path.join(GOLDEN, name + ".npy"))
