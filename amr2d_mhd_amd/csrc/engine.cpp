@@ -2572,7 +2572,7 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
     I.maxlevs = g.maxlevs;
     I.cfl = g.cfl;
     if (g.nx != c->nxi || g.nghosts != c->ng || I.lim < 0 || I.solver < 0 || I.ic < 0 || g.maxlevs < 1 ||
-        g.maxlevs > 8 || ((I.lim == kLimWENO5) && c->ng < 3) || g.ct_type < 0 || g.ct_type > 2) {
+        g.maxlevs > 7 || ((I.lim == kLimWENO5) && c->ng < 3) || g.ct_type < 0 || g.ct_type > 2) {
       delete c;
       return AMR_ERR_CONFIG;
     }
@@ -2938,7 +2938,7 @@ int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* 
 
 int amr_validate_tree(int nx, int maxlevs, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs) {
   AMR_TRY
-  if (nx <= 0 || maxlevs < 1 || maxlevs > 8) return AMR_ERR_CONFIG;
+  if (nx <= 0 || maxlevs < 1 || maxlevs > 7) return AMR_ERR_CONFIG;
   return validate_tree(n, maxblocks_of(maxlevs), maxlevs, nx, lp, icoord, leafs);
   AMR_CATCH
 }
