@@ -122,6 +122,8 @@ _SIGS = {
     "amr_debug_compile": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_void_p]),
     "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "amr_debug_stage_flops": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "amr_debug_count_flux": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = sorted(_SIGS)

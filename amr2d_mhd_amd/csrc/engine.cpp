@@ -29,6 +29,7 @@
 #include "../../include/amr2d_mhd.h"
 #include "abi_guard.hpp"
 #include "amr_types.hpp"
+#include "flop_count.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
 
@@ -3561,6 +3562,38 @@ int amr_debug_coarsen_maps(int NT, int ng, int32_t* mU, int32_t* mB) {
 }
 
 // the locate_bounds schedule of a tree (no device work)
+// Algorithmic operations per zone of one RK2 stage (mean of stages 1 and 2)
+// over the instance's current leaves, counted on the host from the current
+// state: out[0..4] = add/sub, mul, div, sqrt, total.
+int amr_debug_stage_flops(amr_ctx* c, int inst, double* out) {
+  AMR_TRY
+  if (check_inst(c, inst) || !out || c->in_flight) return AMR_ERR_CONFIG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const Instance& I = c->inst[inst];
+  const size_t N2 = (size_t)c->NT * c->NT;
+  std::vector<double> U(8 * N2), B(2 * N2);
+  StageFlops acc;
+  int64_t leaves = 0;
+  for (int nb = 0; nb < I.lastActive; nb++) {
+    if (!I.leafs[nb]) continue;
+    HIPCHK(hipMemcpy(U.data(), c->F.f[kU] + 8 * N2 * I.gs[nb], 8 * N2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(B.data(), c->F.f[kBi] + 2 * N2 * I.gs[nb], 2 * N2 * sizeof(double), hipMemcpyDeviceToHost));
+    for (int step = 1; step <= 2; step++)
+      count_stage_tile(c->NT, c->ng, U.data(), B.data(), I.gamma, I.dx[I.lp[nb][0]], I.dt_dom, I.lim, I.solver, step,
+                       acc);
+    leaves++;
+  }
+  const double zones = 2.0 * (double)leaves * (c->NT - 2 * c->ng) * (c->NT - 2 * c->ng);
+  if (zones <= 0) return AMR_ERR_CONFIG;
+  out[0] = acc.add / zones;
+  out[1] = acc.mul / zones;
+  out[2] = acc.div / zones;
+  out[3] = acc.sqrt / zones;
+  out[4] = out[0] + out[1] + out[2] + out[3];
+  return AMR_OK;
+  AMR_CATCH
+}
+
 int amr_debug_schedule(int n, const int32_t* lp, const int32_t* leafs, int cap, int32_t* ib3) {
   AMR_TRY
   Instance I;

@@ -26,20 +26,6 @@ __device__ inline void report(DevError* err, int inst, unsigned long long key) {
   atomicMax(&err[inst].nkey, ~key);
 }
 
-// conserved -> primitive of one cell (meshblock.cpp:119-179); returns the
-// pressure so the caller can apply the reference's interior p<=0 check
-__device__ inline void cons_to_prim(const double* u, double g, double* w) {
-  w[0] = u[0];
-  w[1] = u[1] / u[0];
-  w[2] = u[2] / u[0];
-  w[3] = u[3] / u[0];
-  double p = (g - 1) * (u[4] - 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
-  w[5] = u[5];
-  w[6] = u[6];
-  w[7] = u[7];
-  w[4] = p - (g - 1) * (0.5 * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]));
-}
-
 // Kernel span from the device's constant-rate wall clock (100 MHz): span[0]
 // accumulates the complement of the earliest workgroup start, span[1] the
 // latest workgroup end (both start at zero; vector atomics).

@@ -22,6 +22,20 @@ __host__ __device__ inline double mac_max(double a, double b) { return a > b ? a
 __host__ __device__ inline double std_min(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ inline double std_max(double a, double b) { return (a < b) ? b : a; }
 
+// conserved -> primitive of one cell (meshblock.cpp:119-179); the caller
+// applies the reference's interior p <= 0 check to w[4]
+__host__ __device__ inline void cons_to_prim(const double* u, double g, double* w) {
+  w[0] = u[0];
+  w[1] = u[1] / u[0];
+  w[2] = u[2] / u[0];
+  w[3] = u[3] / u[0];
+  double p = (g - 1) * (u[4] - 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]));
+  w[5] = u[5];
+  w[6] = u[6];
+  w[7] = u[7];
+  w[4] = p - (g - 1) * (0.5 * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]));
+}
+
 // slopelimiter.cpp:6-54
 __host__ __device__ inline double limiter(int id, double r, double eta) {
   switch (id) {

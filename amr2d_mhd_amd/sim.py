@@ -179,6 +179,14 @@ class Simulation:
             handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
             _check(self.h, lib().amr_use_stream(self.h, C.c_void_p(handle), 1))
 
+    def stage_flops(self, inst=0):
+        """Operations per zone of one RK2 stage (mean of the two stages) on the
+        instance's current leaves, counted on the host by the op-counting
+        build of the stage arithmetic: {add, mul, div, sqrt, total}."""
+        out = np.zeros(5)
+        _check(self.h, lib().amr_debug_stage_flops(self.h, inst, out.ctypes.data))
+        return dict(zip(("add", "mul", "div", "sqrt", "total"), out.tolist()))
+
     def set_graph(self, on=True):
         """Replay each step's device work as a HIP graph, or launch it directly (the default)."""
         _check(self.h, lib().amr_set_graph(self.h, 1 if on else 0))

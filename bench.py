@@ -26,6 +26,9 @@ cost.  One line of JSON carries all three:
                 window, against the 8 TB/s HBM peak;
   valu          the same kernel's executed FP64 FLOPs per launch (counter-
                 measured per zone) / mean launch time, against 78.6 TFLOP/s;
+                op_counted: the algorithmic operations per zone (each +, -, *,
+                /, sqrt of the source once, counted on the host on the run's
+                own states) at the same launch time;
   cpu_baseline  the reference itself (compiled from its sources into
                 oracle/_ref with its OpenMP loops) over the same steps of the
                 same run, on the host's cores, in its FMA-free build and with
@@ -353,6 +356,27 @@ def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):
     return out
 
 
+def stage_op_count(at=(1000, 2500, 4000)):
+    """Algorithmic FP64 operations per zone of one stage (mean of the two RK2
+    stages) of the rotor workload, counted on the host by the op-counting
+    build of the stage arithmetic on the run's own states at the given steps
+    (untimed): the operations of the source, each division and square root
+    counted once, every branch as taken."""
+    import amr2d_mhd_amd as A
+
+    sim = A.Simulation(ROTOR)
+    sim.init()
+    done, rows = 0, []
+    for s in at:
+        sim.run(s - done)
+        done = s
+        rows.append(sim.stage_flops())
+    sim.close()
+    mean = {k: sum(r[k] for r in rows) / len(rows) for k in rows[0]}
+    return dict(flops_per_zone=mean["total"], add_sub=mean["add"], mul=mean["mul"], div=mean["div"], sqrt=mean["sqrt"],
+                states=f"rotor states after steps {', '.join(str(s) for s in at)}")
+
+
 def rotor_tuning():
     """Stage-kernel tuner decisions of the whole rotor run, from an untimed
     run (the timed runs import them, so no trial launch falls in a timed
@@ -430,6 +454,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-fd", type=int, default=0, help="finite-difference check directions")
+    ap.add_argument("--no-op-count", action="store_true",
+                    help="skip the host-side operation count of the stage arithmetic")
     ap.add_argument("--step-kernel", type=int, default=0, choices=[0, 1, 2],
                     help="rotor runs with per-phase launches (0) or the persistent step kernel, 256 (1) or 512 (2) "
                          "lanes; with 1 or 2 the stage figures time whole stage phases, barrier included")
@@ -502,6 +528,10 @@ def main():
                  "unit": "TFLOP/s", "frac": flops / FP64_PEAK_TFLOPS, "flops_per_zone": STAGE_FP64_FLOPS_PER_ZONE},
         "cpu_baseline": None,
     }
+    if not args.no_op_count:
+        oc = stage_op_count()
+        oc_rate = oc["flops_per_zone"] * zones_per_launch / mean_launch_s / 1e12
+        out["valu"]["op_counted"] = dict(oc, achieved=oc_rate, unit="TFLOP/s", frac=oc_rate / FP64_PEAK_TFLOPS)
     if not args.no_whole_run and not (args.warmup == 0 and args.steps in (-1, ROTOR_STEPS)):
         w = rotor_run(0, -1, tuning)
         out["whole_run"] = {"workload": "the same rotor run, every step from t = 0 to t = 0.295", "steps": w["steps"],

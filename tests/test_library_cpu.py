@@ -332,3 +332,38 @@ def test_hlld_host_build_equals_oracle(gamma):
             L.amr_debug_hlld(wL[q].ctypes.data, wR[q].ctypes.data, gamma, out[q].ctypes.data)
         same = (out.view(np.uint64) == ref.view(np.uint64)) | (np.isnan(out) & np.isnan(ref))
         assert same.all(), f"case {case}: {(~same.all(1)).sum()} states differ"
+
+
+def test_operation_counting_build_is_the_real_arithmetic():
+    """The op-counting build of the pointwise physics (flop_count.cpp: the
+    same physics.hpp with a counting scalar type) computes every solver's flux
+    bit for bit like the oracle, so the operations it counts are the ones the
+    kernels perform; the branch-free solvers (Rusanov, HLLE) take the same
+    number of operations on every state, HLLD the fewest in its supersonic
+    regions (one side's flux) and more in the subsonic ones (star states)."""
+    from amr2d_mhd_amd._lib import lib
+
+    L = lib()
+    rng = np.random.default_rng(11)
+    totals = {}
+    for solver, name in ((0, "RUSA"), (1, "HLLE"), (2, "HLLC"), (3, "HLLD")):
+        seen = []
+        for q in range(300):
+            wL = np.concatenate([[rng.uniform(0.2, 5)], rng.normal(0, 1, 3), [rng.uniform(0.1, 5)], rng.normal(0, 1, 3)])
+            wR = np.concatenate([[rng.uniform(0.2, 5)], rng.normal(0, 1, 3), [rng.uniform(0.1, 5)], rng.normal(0, 1, 3)])
+            wR[5] = wL[5]
+            if name == "HLLD" and q % 3 == 0:
+                wL[1] += 8.0  # supersonic to the right: region 1
+                wR[1] += 8.0
+            f, c = np.zeros(8), np.zeros(4)
+            L.amr_debug_count_flux(solver, wL.ctypes.data, wR.ctypes.data, 1.4, f.ctypes.data, c.ctypes.data)
+            ref, _ = O.riemann(name, "x", 1.4, wL[None], wR[None])
+            assert np.array_equal(f.view(np.uint64), ref[0].view(np.uint64)), (name, q)
+            seen.append((q % 3 == 0, c.sum()))
+        totals[name] = seen
+    for name in ("RUSA", "HLLE"):
+        assert len({t for _, t in totals[name]}) == 1, name
+    sup = {t for s, t in totals["HLLD"] if s}
+    sub = [t for s, t in totals["HLLD"] if not s]
+    # (some unshifted random states are supersonic too)
+    assert len(sup) == 1 and min(sub) >= max(sup) and max(sub) > max(sup)
