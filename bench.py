@@ -531,7 +531,10 @@ def main():
     if not args.no_op_count:
         oc = stage_op_count()
         oc_rate = oc["flops_per_zone"] * zones_per_launch / mean_launch_s / 1e12
-        out["valu"]["op_counted"] = dict(oc, achieved=oc_rate, unit="TFLOP/s", frac=oc_rate / FP64_PEAK_TFLOPS)
+        out["valu"]["op_counted"] = dict(oc, achieved=oc_rate, unit="TFLOP/s", frac=oc_rate / FP64_PEAK_TFLOPS,
+                                         stage_flops_per_zone_update=2 * oc["flops_per_zone"],
+                                         note="flops_per_zone: one stage; a zone-update (one step) runs two "
+                                              "stages, plus CT and the CFL reduction (not counted)")
     if not args.no_whole_run and not (args.warmup == 0 and args.steps in (-1, ROTOR_STEPS)):
         w = rotor_run(0, -1, tuning)
         out["whole_run"] = {"workload": "the same rotor run, every step from t = 0 to t = 0.295", "steps": w["steps"],
