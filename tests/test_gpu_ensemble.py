@@ -41,6 +41,35 @@ def test_rotor_p_vs_oracle_bitwise(inst):
     lockstep(sim, dom, 80, full_every=20, label=f"rotor_p {inst}")
 
 
+def test_rotor_p_at_depths_5_and_6_batched_vs_oracle():
+    """Seeded parameterised rotors at their drawn depths 5 and 6 (HLLC and
+    HLLD), advanced together in one context: each instance equals its own
+    oracle run -- dt and t every step, tree and every slot field every 20
+    steps -- for 60 steps."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    ids = [1, 6, 12, 13]
+    cfgs = [instance_config(i) for i in ids]
+    assert [(c["maxlevs"], c["flux"]) for c in cfgs] == [(5, "HLLD"), (5, "HLLC"), (6, "HLLC"), (6, "HLLD")]
+    batch = A.Simulation(cfgs)
+    batch.init()
+    doms = [O.Domain(**c) for c in cfgs]
+    for d in doms:
+        assert d.init() == 0
+    for s in range(60):
+        batch.step()
+        for q, d in enumerate(doms):
+            assert d.step() == 0
+            inf = batch.info(q)
+            assert (inf["t"], inf["dt"]) == (d.info()["t"], d.info()["dt"]), (ids[q], s)
+        if (s + 1) % 20 == 0:
+            for q, d in enumerate(doms):
+                diff = compare_states(gpu_state(batch, q), oracle_state(d))
+                assert not diff, f"instance {ids[q]} after step {s}: " + "; ".join(diff[:4])
+    batch.close()
+
+
 def test_batch_equals_alone_bitwise():
     """Fourteen seeded instances (levels 3-6, HLLC and HLLD) advanced together."""
     import amr2d_mhd_amd as A
