@@ -377,6 +377,13 @@ struct amr_ctx {
   LeafItem* d_leaves = nullptr;  // view into d_topo
   int nleaf = 0;
   std::vector<LeafItem> h_leaves;
+  // stage launches split by specialisation (running instances of different
+  // solver/limiter classes): the leaves ordered by class, split_off[k] ..
+  // split_off[k + 1] those of class split_spec[k]
+  bool stage_split = false;
+  std::vector<LeafItem> h_split_leaves;
+  int split_n = 0, split_spec[kStageSpecs] = {0}, split_off[kStageSpecs + 1] = {0};
+  LeafItem* d_split_leaves = nullptr;  // view into d_topo
   std::vector<int> leaf_inst_begin;  // per active instance, first leaf index
   // host copies kept alive until the stream has consumed them
   std::vector<RegridOp> h_refs, h_coars;
@@ -445,7 +452,7 @@ struct amr_ctx {
   // kernel boundary it replaces), 1 / 2 = one launch per step with
   // kStepLanes[0] / [1] lanes per workgroup
   int step_mode = 0;
-  int step_grid[kStageSpecs][2] = {{-1, -1}, {-1, -1}, {-1, -1}};  // co-resident grid; -1: not queried yet
+  int step_grid[kStageSpecs][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // co-resident grid; -1: not queried yet
   bool step_broken = false;  // a barrier wait ran out: per-phase launches from then on
   // inside amr_run_group: steps of several contexts may run at once on their
   // own streams, and two cooperative grids sized for the whole device could
@@ -1881,6 +1888,43 @@ int stage_spec_of(const amr_ctx* c) {
   return hlld_muscl ? kStageHLLDMuscl : hlld_weno5 ? kStageHLLDWeno5 : kStageGeneric;
 }
 
+// stage specialisation an instance's leaves can run with on their own
+int stage_class_of(const amr_ctx* c, const Instance& I) {
+  if (I.solver == kFluxHLLD && I.lim < kLimWENO3) return kStageHLLDMuscl;
+  if (I.solver == kFluxHLLD && I.lim == kLimWENO5 && c->NT == 24 && c->ng == 4) return kStageHLLDWeno5;
+  if (I.solver == kFluxHLLC && I.lim < kLimWENO3) return kStageHLLCMuscl;
+  return kStageGeneric;
+}
+
+// when the running instances fall into more than one class, order the leaf
+// list by class for per-class stage launches (each leaf's stage is the same
+// whichever launch runs it)
+void plan_stage_split(amr_ctx* c) {
+  c->stage_split = false;
+  c->split_n = 0;
+  if (c->stage_spec != kStageGeneric) return;
+  std::vector<int> cls(c->inst.size(), -1);
+  int present = 0;
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    if (c->inst[q].done) continue;
+    cls[q] = stage_class_of(c, c->inst[q]);
+    present |= 1 << cls[q];
+  }
+  if (__builtin_popcount(present) < 2) return;
+  static const int order[kStageSpecs] = {kStageHLLDMuscl, kStageHLLDWeno5, kStageHLLCMuscl, kStageGeneric};
+  c->h_split_leaves.clear();
+  for (int k = 0; k < kStageSpecs; k++) {
+    if (!(present & (1 << order[k]))) continue;
+    c->split_spec[c->split_n] = order[k];
+    c->split_off[c->split_n] = (int)c->h_split_leaves.size();
+    for (const LeafItem& L : c->h_leaves)
+      if (cls[L.inst] == order[k]) c->h_split_leaves.push_back(L);
+    c->split_n++;
+  }
+  c->split_off[c->split_n] = (int)c->h_split_leaves.size();
+  c->stage_split = true;
+}
+
 int rebuild_topology(amr_ctx* c) {
   HostTimed ht(c, PF_HOST_TOPO);
   HIPCHK(hipStreamSynchronize(c->stream));  // previous uploads consumed
@@ -1900,6 +1944,7 @@ int rebuild_topology(amr_ctx* c) {
   {
     c->stage_spec = stage_spec_of(c);
     c->tune = c->tune_by_spec[c->stage_spec];
+    plan_stage_split(c);
   }
   // programs of instances whose tree changed: the current one if the tree is
   // the same, a cached one if the tree recurs, else compiled (in parallel)
@@ -1973,7 +2018,8 @@ int rebuild_topology(amr_ctx* c) {
   const size_t off_gen = off_ops + align(sizeof(FillOp*) * (ni + 1));
   const size_t off_pre = off_gen + align(sizeof(FillGen*) * (ni + 1));
   const size_t off_leaf = off_pre + align(sizeof(int32_t) * (ni + 1));
-  const size_t topo_bytes = off_leaf + sizeof(LeafItem) * std::max(1, c->nleaf);
+  const size_t off_split = off_leaf + align(sizeof(LeafItem) * std::max(1, c->nleaf));
+  const size_t topo_bytes = off_split + sizeof(LeafItem) * std::max(1, c->nleaf);
   if (topo_bytes > c->cap_topo) {
     const size_t cap = topo_bytes + topo_bytes / 2;
     hipFree(c->d_topo);
@@ -2002,8 +2048,13 @@ int rebuild_topology(amr_ctx* c) {
   ((int32_t*)c->h_topo)[kCountLeaves] = c->nleaf;
   ((int32_t*)c->h_topo)[kCountChunks] = c->nchunks;
   if (c->nleaf) std::memcpy(c->h_topo + off_leaf, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf);
-  HIPCHK(hipMemcpyAsync(c->d_topo, c->h_topo, off_leaf + sizeof(LeafItem) * c->nleaf, hipMemcpyHostToDevice,
-                        c->stream));
+  size_t up = off_leaf + sizeof(LeafItem) * c->nleaf;
+  if (c->stage_split) {
+    std::memcpy(c->h_topo + off_split, c->h_split_leaves.data(), sizeof(LeafItem) * c->h_split_leaves.size());
+    up = off_split + sizeof(LeafItem) * c->h_split_leaves.size();
+  }
+  HIPCHK(hipMemcpyAsync(c->d_topo, c->h_topo, up, hipMemcpyHostToDevice, c->stream));
+  c->d_split_leaves = (LeafItem*)(c->d_topo + off_split);
   c->d_counts = (const int32_t*)c->d_topo;
   c->d_inst_ops = (const FillOp**)(c->d_topo + off_ops);
   c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
@@ -2057,12 +2108,27 @@ int stage_choice(amr_ctx* c, bool* timed, bool allow_trial = true) {
 }
 
 bool tuning(amr_ctx* c) {
+  if (c->stage_split) return false;  // split launches make no trials
   bool t;
   stage_choice(c, &t);
   return t;
 }
 
 void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
+  if (c->stage_split) {
+    // one launch per class of instances, each with its specialisation
+    // (the HLLC one exists for kSplitVariant only); no tuner trials
+    const int v = c->stage_variant >= 0 ? c->stage_variant : kSplitVariant;
+    c->span_variant[step - 1] = -1;
+    for (int k = 0; k < c->split_n; k++) {
+      const int n = c->split_off[k + 1] - c->split_off[k];
+      if (n <= 0) continue;
+      Timed tm(c, PF_STAGE, true);
+      launch_stage(c->NT, c->ng, v, c->split_spec[k], c->F, c->d_split_leaves + c->split_off[k], n, c->d_ip, step,
+                   c->d_err, c->stream, tm.start(), tm.stop(), nullptr, nullptr);
+    }
+    return;
+  }
   Timed tm(c, PF_STAGE, true);
   bool timed;
   const int v = stage_choice(c, &timed, allow_trial);

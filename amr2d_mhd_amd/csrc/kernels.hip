@@ -56,6 +56,12 @@ struct StageLds {
   static constexpr int kDoubles = 8 * N2 + 2 * NFL;
 };
 
+// the Riemann solver of a stage specialisation (generic: the instance's)
+template <int SPEC>
+__device__ inline int spec_solver(int inst_solver) {
+  return SPEC == kStageHLLCMuscl ? (int)kFluxHLLC : SPEC != kStageGeneric ? (int)kFluxHLLD : inst_solver;
+}
+
 // Primitives of tile cell c into sWc[k * VS] (k = 0..7) with the checks of
 // meshblock.cpp:135-142 / 163-176, cell by cell: pressure, then velocity
 // NaN; a NaN primitive sets *sNan where the limiter's edge check applies.
@@ -91,7 +97,7 @@ __device__ inline void stage_face(const double* sWb, int VS, const double* __res
                                   double g, double dxl, int i, int j, int ydir, int step, const LeafItem& L,
                                   DevError* err, double* fl) {
   constexpr int N2 = NT * NT;
-  constexpr bool muscl_only = SPEC == kStageHLLDMuscl, weno5_only = SPEC == kStageHLLDWeno5;
+  constexpr bool muscl_only = SPEC == kStageHLLDMuscl || SPEC == kStageHLLCMuscl, weno5_only = SPEC == kStageHLLDWeno5;
   const int s = ydir ? NT : 1;  // stride along the face normal
   const int c = j * NT + i;     // left cell; right cell is c + s
   double wl[8], wr[8];
@@ -152,7 +158,7 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
   const int lim = SPEC == kStageHLLDWeno5 ? (int)kLimWENO5 : P.limiter;
-  const int solver = SPEC != kStageGeneric ? (int)kFluxHLLD : P.solver;
+  const int solver = spec_solver<SPEC>(P.solver);
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
   const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
@@ -299,7 +305,7 @@ __global__ __launch_bounds__((StripGeom<NT, NG, K>::kLanes)) void stage_strip_ke
   const InstParams& P = ip[L.inst];
   const double g = P.gamma, dxl = P.dx[L.lev], dt = P.dt;
   const int lim = SPEC == kStageHLLDWeno5 ? (int)kLimWENO5 : P.limiter;
-  const int solver = SPEC != kStageGeneric ? (int)kFluxHLLD : P.solver;
+  const int solver = spec_solver<SPEC>(P.solver);
   const size_t t8 = (size_t)L.gs * 8 * N2, t2 = (size_t)L.gs * 2 * N2;
   const double* Qin = F.f[step == 1 ? kU : kUs] + t8;
   const double* __restrict__ B = F.f[step == 1 ? kBi : kBis] + t2;
@@ -1142,6 +1148,15 @@ void launch_stage(int NT, int ng, int variant, int spec, const FieldPtrs& F, con
                   const InstParams* ip, int step, DevError* err, hipStream_t s, hipEvent_t ev_start,
                   hipEvent_t ev_stop, unsigned long long* span, const int32_t* counts) {
   if (nleaf <= 0) return;
+  if (spec == kStageHLLCMuscl && variant == kSplitVariant) {
+    // the HLLC part of a launch split by specialisation: one variant
+    constexpr int SPEC = kStageHLLCMuscl, TPB = 512, WPE = 4;
+    constexpr bool FLDS = true;
+    AMR_DISPATCH_TILE(NT, ng, (launch_stage_variant<T, G, TPB, FLDS, WPE, SPEC>(F, leaves, nleaf, ip, step, err, s,
+                                                                                 ev_start, ev_stop, span, counts)));
+    return;
+  }
+  if (spec == kStageHLLCMuscl) spec = kStageGeneric;
   if (spec == kStageHLLDMuscl)
     launch_stage_spec<kStageHLLDMuscl>(NT, ng, variant, F, leaves, nleaf, ip, step, err, s, ev_start, ev_stop, span,
                                        counts);

@@ -52,7 +52,11 @@ constexpr int kStageVariants = 8;
 // stage-kernel specialisations: generic (solver and limiter read per leaf), or
 // HLLD with a MUSCL-family limiter, or HLLD with WENO5 (24x24 tiles, the only
 // ones with the ghost depth WENO5 needs), fixed at compile time
-enum { kStageGeneric = 0, kStageHLLDMuscl = 1, kStageHLLDWeno5 = 2, kStageSpecs = 3 };
+// (kStageHLLCMuscl: HLLC with a MUSCL-family limiter, used for the HLLC part of
+// a launch split by specialisation; variant 5 only)
+enum { kStageGeneric = 0, kStageHLLDMuscl = 1, kStageHLLDWeno5 = 2, kStageHLLCMuscl = 3, kStageSpecs = 4 };
+// the variant a launch split by specialisation uses unless one is fixed
+constexpr int kSplitVariant = 5;
 // Work-list counts on the device (counts[kCountLeaves], counts[kCountChunks]):
 // a launch given `counts` may have a larger grid than its work (a captured
 // step replayed over changing trees) and its surplus workgroups exit; with

@@ -70,6 +70,36 @@ def test_rotor_p_at_depths_5_and_6_batched_vs_oracle():
     batch.close()
 
 
+def test_stage_launches_split_by_solver_class():
+    """A batch mixing HLLD + MC, HLLC + MC and HLLE + MC instances runs each
+    RK stage as one launch per class (HLLD and HLLC specialised, HLLE on the
+    generic kernel), and every instance equals its alone run bit for bit (an
+    instance alone runs one class, unsplit)."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import instance_config
+
+    cfgs = [instance_config(i) for i in (1, 2, 3, 4)]
+    assert [c["flux"] for c in cfgs] == ["HLLD", "HLLC", "HLLD", "HLLC"]
+    cfgs.append(dict(instance_config(5), flux="HLLE"))
+    steps = 30
+    batch = A.Simulation(cfgs)
+    batch.init()
+    batch.profile(("stage",))
+    for _ in range(steps):
+        batch.step()
+    ms, launches = batch.profile_read()["stage"]
+    assert launches == 3 * 2 * steps, launches
+    for q, cfg in enumerate(cfgs):
+        alone = _alone(cfg, steps)
+        ib, ia = batch.info(q), alone.info()
+        assert ib["status"] == ia["status"] == 0
+        assert (ib["t"], ib["dt"], ib["nleaf"]) == (ia["t"], ia["dt"], ia["nleaf"]), q
+        d = compare_states(gpu_state(batch, q), gpu_state(alone))
+        assert not d, f"instance {q}: " + "; ".join(d[:4])
+        alone.close()
+    batch.close()
+
+
 def test_batch_equals_alone_bitwise():
     """Fourteen seeded instances (levels 3-6, HLLC and HLLD) advanced together."""
     import amr2d_mhd_amd as A
