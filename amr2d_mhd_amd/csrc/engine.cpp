@@ -359,6 +359,9 @@ enum { PF_STAGE = 0, PF_FILL, PF_CT, PF_CFL, PF_REGRID, PF_HOST_TOPO, PF_HOST_RE
 
 }  // namespace
 
+// stage-kernel tuner: trials per variant and leaf-count bucket
+constexpr int kTuneTrials = 3;
+
 struct amr_ctx {
   int NT = 20, ng = 2, nxi = 16;
   int rk = 2;  // RK stages of every instance: 2 (timeIntegral.cpp) or 3 (SSP-RK3, build-defined)
@@ -434,9 +437,10 @@ struct amr_ctx {
   // one with the fastest trial per leaf is kept for that bucket
   int stage_variant = -1;
   struct TuneBucket {
-    double us_per_leaf[kStageVariants] = {0};  // sum over the trials (reported as a mean)
-    double us_min[kStageVariants] = {0};       // fastest trial (decides: noise only adds time)
-    int n[kStageVariants] = {0};
+    double us_per_leaf[kStageVariants] = {0};          // sum over the trials (reported as a mean)
+    double trial_us[kStageVariants][kTuneTrials] = {};  // each timed trial, us per leaf (decides: median)
+    int nval[kStageVariants] = {0};                     // timed trials
+    int n[kStageVariants] = {0};                        // trials launched
     int chosen = -1;
   };
   TuneBucket tune_by_spec[kStageSpecs][24];  // per stage-kernel specialisation
@@ -2083,8 +2087,6 @@ void run_fill(amr_ctx* c, const FillSet& s) {
               c->launch_counts);
 }
 
-constexpr int kTuneTrials = 3;
-
 int tune_bucket(int nleaf) {
   int b = 0;
   while ((2 << b) <= nleaf && b < 23) b++;
@@ -2099,11 +2101,15 @@ int stage_choice(amr_ctx* c, bool* timed, bool allow_trial = true) {
   const auto& B = c->tune[tune_bucket(c->nleaf)];
   if (B.chosen >= 0) return B.chosen;
   if (!allow_trial) return 0;
+  // round robin over the variants (the one with the fewest trials), so that
+  // every variant's trials fall on about the same leaf counts
+  int next = -1;
   for (int v = 0; v < kStageVariants; v++)
-    if (B.n[v] < kTuneTrials) {
-      *timed = true;
-      return v;
-    }
+    if (B.n[v] < kTuneTrials && (next < 0 || B.n[v] < B.n[next])) next = v;
+  if (next >= 0) {
+    *timed = true;
+    return next;
+  }
   return 0;
 }
 
@@ -2153,16 +2159,28 @@ void tune_collect(amr_ctx* c) {
     if (t1 > t0 && c->wall_khz > 0) {
       const double us = (double)(t1 - t0) / (c->wall_khz * 1e-3) / c->nleaf;
       B.us_per_leaf[v] += us;
-      if (B.us_min[v] == 0. || us < B.us_min[v]) B.us_min[v] = us;
+      if (B.nval[v] < kTuneTrials) B.trial_us[v][B.nval[v]++] = us;
     }
   }
   bool all = true;
   for (int v = 0; v < kStageVariants; v++) all &= B.n[v] >= kTuneTrials;
   if (all && B.chosen < 0) {
-    int best = 0;
-    for (int v = 1; v < kStageVariants; v++)
-      if (B.us_min[v] > 0. && (B.us_min[best] == 0. || B.us_min[v] < B.us_min[best])) best = v;
-    B.chosen = best;
+    // the variant with the smallest median trial (one slow or fast outlier
+    // does not decide)
+    int best = -1;
+    double best_us = 0.;
+    for (int v = 0; v < kStageVariants; v++) {
+      if (!B.nval[v]) continue;
+      double t[kTuneTrials];
+      std::copy(B.trial_us[v], B.trial_us[v] + B.nval[v], t);
+      std::sort(t, t + B.nval[v]);
+      const double med = t[(B.nval[v] - 1) / 2];
+      if (best < 0 || med < best_us) {
+        best = v;
+        best_us = med;
+      }
+    }
+    B.chosen = best < 0 ? 0 : best;
   }
 }
 

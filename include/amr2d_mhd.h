@@ -221,9 +221,9 @@ int amr_set_step_kernel(amr_ctx *ctx, int mode);
  * nghosts 4, specialised to HLLD with WENO5; launches use a specialisation
  * when every running instance has its pair.  -1 (the default)
  * tunes per specialisation and leaf-count bucket (bucket b holds
- * 2^b <= nleaf < 2^(b+1)): the first stage launches in a bucket try every
- * variant three times, timed by the kernel's device-clock span, and the one
- * with the fastest trial per leaf block is kept. */
+ * 2^b <= nleaf < 2^(b+1)): the first stage launches in a bucket try the
+ * variants in turn, three times each, timed by the kernel's device-clock
+ * span, and the one with the smallest median time per leaf block is kept. */
 int amr_set_stage_variant(amr_ctx *ctx, int variant);
 /* Tuner state of the specialisation in use: chosen[b] (-1 = not decided) and
  * the mean measured microseconds per leaf block of each variant,
