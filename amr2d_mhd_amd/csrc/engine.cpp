@@ -2095,6 +2095,17 @@ int tune_bucket(int nleaf) {
 
 // variant of the next stage launch; sets *timed when the launch is a trial
 // (trials run only inside full steps, whose spans are collected)
+// The strip variants split a leaf over several workgroups, so they cannot run
+// a stage that reads and writes the same field (SSP-RK3's second stage reads
+// Us and writes Us: one strip's update would race with another's loads of
+// its halo rows).  The tuner leaves them out of RK3 contexts, and a fixed
+// strip variant runs such a stage as the whole-leaf 768-lane variant.
+bool strip_variant(int v) { return v == 6 || v == 7; }
+bool strips_allowed(const amr_ctx* c) { return c->rk != 3; }
+int stage_variant_for(const amr_ctx* c, int v, int step) {
+  return strip_variant(v) && c->rk == 3 && step == 2 ? 2 : v;
+}
+
 int stage_choice(amr_ctx* c, bool* timed, bool allow_trial = true) {
   *timed = false;
   if (c->stage_variant >= 0) return c->stage_variant;
@@ -2105,7 +2116,8 @@ int stage_choice(amr_ctx* c, bool* timed, bool allow_trial = true) {
   // every variant's trials fall on about the same leaf counts
   int next = -1;
   for (int v = 0; v < kStageVariants; v++)
-    if (B.n[v] < kTuneTrials && (next < 0 || B.n[v] < B.n[next])) next = v;
+    if ((strips_allowed(c) || !strip_variant(v)) && B.n[v] < kTuneTrials && (next < 0 || B.n[v] < B.n[next]))
+      next = v;
   if (next >= 0) {
     *timed = true;
     return next;
@@ -2124,7 +2136,7 @@ void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
   if (c->stage_split) {
     // one launch per class of instances, each with its specialisation
     // (the HLLC one exists for kSplitVariant only); no tuner trials
-    const int v = c->stage_variant >= 0 ? c->stage_variant : kSplitVariant;
+    const int v = stage_variant_for(c, c->stage_variant >= 0 ? c->stage_variant : kSplitVariant, step);
     c->span_variant[step - 1] = -1;
     for (int k = 0; k < c->split_n; k++) {
       const int n = c->split_off[k + 1] - c->split_off[k];
@@ -2137,7 +2149,7 @@ void run_stage(amr_ctx* c, int step, bool allow_trial = true) {
   }
   Timed tm(c, PF_STAGE, true);
   bool timed;
-  const int v = stage_choice(c, &timed, allow_trial);
+  const int v = stage_variant_for(c, stage_choice(c, &timed, allow_trial), step);
   unsigned long long* span = timed ? c->d_span + 2 * (step - 1) : nullptr;
   c->span_variant[step - 1] = timed ? v : -1;
   if (timed) {
@@ -2163,7 +2175,7 @@ void tune_collect(amr_ctx* c) {
     }
   }
   bool all = true;
-  for (int v = 0; v < kStageVariants; v++) all &= B.n[v] >= kTuneTrials;
+  for (int v = 0; v < kStageVariants; v++) all &= B.n[v] >= kTuneTrials || (!strips_allowed(c) && strip_variant(v));
   if (all && B.chosen < 0) {
     // the variant with the smallest median trial (one slow or fast outlier
     // does not decide)

@@ -216,7 +216,9 @@ int amr_set_step_kernel(amr_ctx *ctx, int mode);
  * 2: 768 lanes, LDS; 3: 256 lanes, fluxes read back from ff/gg, 4 waves per
  * SIMD; 4: 512 lanes, read back, 4 waves per SIMD; 5: 512 lanes, LDS, 4 waves
  * per SIMD; 6: each leaf split into 2 row strips, one workgroup per strip;
- * 7: 4 row strips.  Each variant exists generic (solver and limiter read per
+ * 7: 4 row strips (a stage that reads and writes the same field, SSP-RK3's
+ * second, runs as variant 2 instead, and the tuner does not try strips with
+ * RK3).  Each variant exists generic (solver and limiter read per
  * instance), specialised to HLLD with a MUSCL-family limiter, and, for
  * nghosts 4, specialised to HLLD with WENO5; launches use a specialisation
  * when every running instance has its pair.  -1 (the default)
