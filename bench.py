@@ -52,10 +52,10 @@ ROTOR_STEPS = 4763  # steps of the full rotor run to t = 0.295 without FMA contr
 # Us and U (128 B) and face B (16 B), writes U (64 B) -> mean 176 B
 STAGE_BYTES_PER_ZONE = 176.0
 # HBM bytes per leaf zone one stage launch actually moves, from rocprofv3 FETCH_SIZE
-# (x2.0, copy-calibrated) + WRITE_SIZE (x1.0) on this workload:
-# profiles/stage_traffic_rotor_lv5.md (267.6 read + 270.0 written, 768-lane variant
-# specialised to HLLD/MUSCL)
-STAGE_TRAFFIC_BYTES_PER_ZONE = 537.6
+# (x2.0, copy-calibrated) + WRITE_SIZE (x1.0) over the 9526 stage launches of the
+# whole run, with the variants the tuner picks: profiles/stage_traffic_rotor_lv5.md
+# (311.5 read + 286.5 written)
+STAGE_TRAFFIC_BYTES_PER_ZONE = 598.0
 HBM_PEAK_GBS = 8000.0
 # FP64 FLOPs one stage launch executes per leaf zone: rocprofv3 counters
 # (SQ_INSTS_VALU_FLOPS_FP64 + SQ_INSTS_VALU_FLOPS_FP64_TRANS) x 64 lanes per
