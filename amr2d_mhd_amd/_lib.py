@@ -88,6 +88,7 @@ _SIGS = {
     "amr_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_set_stage_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "amr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_profile_sample": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "amr_adj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.POINTER(C.c_void_p)]),
     "amr_adj_destroy": (C.c_int, [C.c_void_p]),

@@ -489,6 +489,10 @@ struct amr_ctx {
   // the context stream; elapsed times are collected after the next sync
   unsigned prof_mask = 0;
   Profile prof[PF_COUNT];
+  // kernel families are timed on one launch in prof_stride (the others run
+  // without events); prof_seen counts each family's launches
+  int prof_stride = 1;
+  int64_t prof_seen[PF_COUNT] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<std::array<hipEvent_t, 2>> ev_pool;
   std::vector<std::pair<int, int>> ev_pending;  // (family, pool index)
@@ -551,6 +555,7 @@ struct Timed {
   bool attached;
   Timed(amr_ctx* c_, int f, bool attached_ = false) : c(c_), fam(f), attached(attached_) {
     if (!(c->prof_mask & (1u << fam))) return;
+    if (c->prof_seen[fam]++ % c->prof_stride) return;
     slot = (int)c->ev_pending.size();
     if (slot >= (int)c->ev_pool.size()) {
       std::array<hipEvent_t, 2> e{};
@@ -3426,6 +3431,16 @@ int amr_profile_enable(amr_ctx* c, int mask) {
   flush_profile(c);
   c->prof_mask = (unsigned)mask;
   for (auto& p : c->prof) p = Profile{};
+  for (auto& n : c->prof_seen) n = 0;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_profile_sample(amr_ctx* c, int stride) {
+  AMR_TRY
+  if (!c || stride < 1) return AMR_ERR_CONFIG;
+  c->prof_stride = stride;
+  for (auto& n : c->prof_seen) n = 0;
   return AMR_OK;
   AMR_CATCH
 }

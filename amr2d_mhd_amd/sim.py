@@ -215,8 +215,10 @@ class Simulation:
             arr[int(b)] = v[0] if isinstance(v, (tuple, list)) else int(v)
         _check(self.h, lib().amr_set_stage_tuning(self.h, 24, arr.ctypes.data))
 
-    def profile(self, families=PROFILE_FAMILIES):
-        """Start HIP-event timing of the given kernel families (resets totals)."""
+    def profile(self, families=PROFILE_FAMILIES, stride=1):
+        """Start HIP-event timing of the given kernel families (resets totals);
+        stride n > 1 times one launch in n of each family."""
+        _check(self.h, lib().amr_profile_sample(self.h, int(stride)))
         mask = 0
         for f in families:
             mask |= 1 << self.PROFILE_FAMILIES.index(f)

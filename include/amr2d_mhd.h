@@ -262,6 +262,12 @@ int amr_set_fill_cache(amr_ctx *ctx, int entries);
  * since the last amr_profile_enable. */
 int amr_profile_enable(amr_ctx *ctx, int mask);
 int amr_profile_read(amr_ctx *ctx, int family, double *ms, int64_t *launches);
+/* Event sampling of the kernel families (no counterpart in the reference):
+ * with stride n > 1 only one launch in n of each family carries events (its
+ * time and count go to amr_profile_read), the others run without them; a
+ * whole-run mean launch time then costs 1/n of the events' overhead.  1 (the
+ * default) times every launch.  Host timers are not sampled. */
+int amr_profile_sample(amr_ctx *ctx, int stride);
 
 /* ---- Frozen-mesh step and its discrete adjoint (no counterpart in the
  * reference, which has no adjoint).  One uniform periodic N x N grid of cell

@@ -93,3 +93,22 @@ def test_context_on_caller_streams_is_bitwise_unchanged():
     sim.use_stream(None)
     sim.close()
     ref.close()
+
+
+def test_profile_event_sampling():
+    """amr_profile_sample: with stride 3 one stage launch in three is timed
+    (two launches per step), with a mean close to the every-launch mean."""
+    import amr2d_mhd_amd as A
+
+    means = {}
+    for stride in (1, 3):
+        sim = A.Simulation(ROTOR4)
+        sim.init()
+        sim.run(100)
+        sim.profile(("stage",), stride=stride)
+        sim.run(60)
+        ms, n = sim.profile_read()["stage"]
+        assert n == 120 // stride, (stride, n)
+        means[stride] = ms / n
+        sim.close()
+    assert abs(means[3] - means[1]) < 0.5 * means[1]
