@@ -24,7 +24,7 @@
 namespace amr {
 
 namespace {
-StageFlops g_tally;
+thread_local StageFlops g_tally;  // the counts of the calling thread's current count
 }
 
 struct Counted {

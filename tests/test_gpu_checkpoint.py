@@ -86,10 +86,12 @@ def test_bad_checkpoints_are_rejected(tmp_path):
         with pytest.raises(AmrError) as e:
             dst.load_checkpoint(tmp_path / name)
         assert e.value.code == 14, name
-    other = A.Simulation(dict(ROTOR, flux="HLLE"))
-    with pytest.raises(AmrError) as e:
-        other.load_checkpoint(path)
-    assert e.value.code == 10
+    for cfg in (dict(ROTOR, flux="HLLE"), dict(ROTOR, ct_type=2)):  # another solver, another corner EMF
+        other = A.Simulation(cfg)
+        with pytest.raises(AmrError) as e:
+            other.load_checkpoint(path)
+        assert e.value.code == 10, cfg
+        other.close()
     # the rejected loads left the instance as it was
     d = compare_states(gpu_state(dst), before)
     assert not d, "; ".join(d[:3])
