@@ -1040,9 +1040,11 @@ void for_ranges(FillScratch& S, int R, int nthreads, const F& f) {
 }
 
 // ring elements (blocks x ring cells) from which a compile is split over
-// threads: below, the helpers' wake-ups and the extra merge pass cost more
-// than they save
-constexpr size_t kSplitSlots = size_t(1) << 20;
+// threads (about 230 blocks of a 20x20 tile): below, the helpers' wake-ups
+// and the extra merge pass cost more than they save (whole rotor run on the
+// 16 host threads of a GPU: 1.385 s unsplit, 1.32-1.35 s split from 2^15
+// elements, 1.35-1.39 s from 2^14)
+constexpr size_t kSplitSlots = size_t(1) << 15;
 
 // Program of one instance: 16-byte destination ops plus the explicit-operand
 // table of restrictions that read an element already written in the same
