@@ -80,6 +80,7 @@ _SIGS = {
     "amr_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_use_stream": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "amr_set_step_kernel": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_set_regrid_device": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_set_fill_cache": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_step_begin": (C.c_int, [C.c_void_p]),
     "amr_step_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -124,6 +125,8 @@ _SIGS = {
                                     C.c_void_p, C.c_void_p]),
     "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "amr_debug_stage_flops": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "amr_debug_regrid_lds": (C.c_int, [C.c_void_p, C.c_int]),
+    "amr_debug_device_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_count_flux": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
 }
 

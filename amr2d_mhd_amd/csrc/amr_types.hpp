@@ -118,6 +118,33 @@ struct DevError {
   int32_t pad[6];
 };
 enum { kOrdCfl = 4, kOrdRefine = 5, kOrdCoarsen = 6 };
+
+// Device-side regrid (admesh.cpp:17-102), per instance: where its blocks sit
+// in the context's tree arrays and its leaves in the leaf work list.
+struct RegridInst {
+  int32_t off;          // first block of the instance in the tree arrays
+  int32_t maxblocks;    // defs.hpp:30
+  int32_t maxlevs;
+  int32_t last_active;  // meshblock::lastActive before the regrid
+  int32_t leaf_begin, leaf_end;
+  int32_t regrid;       // 1: the whole admesh; 0: only the CFL minimum
+  int32_t lds_rows;     // rows staged in LDS (covering every block the regrid
+                        // can reach), or 0: the tree is worked on in place
+};
+// most tree rows a regrid workgroup stages in LDS (16-bit ids and four flag
+// bytes per block, 159 KB)
+constexpr int kRgLdsRows = 8128;
+// What the host reads back per instance: the CFL minimum and the regrid's
+// decisions.  The op log holds the instance's refinements (parent ids), then
+// its coarsenings (first-son ids), in the reference's order, from the
+// instance's first leaf index on (at most one op per leaf).
+struct RegridOut {
+  double lambda;         // min over the leaves' CFL bounds, from 1e6
+  int32_t nref, ncoars;
+  int32_t last_active;   // after the regrid
+  int32_t status;        // 0, or kErrOverflow at op nref - 1
+  int32_t fail_nb, pad;
+};
 // sequence of the CT phase of a stage: after every block's stage errors
 // (block ids stay below 2^15 for maxlevs <= 7), before the next stage's
 constexpr int kSeqCt = 0x8000;

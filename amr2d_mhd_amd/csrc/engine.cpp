@@ -332,6 +332,9 @@ struct Instance {
   int ct_type = 1;        // corner EMF: CTtype 1 or 2 (CT2D.cpp:15-75)
   amr_diag err{};
   bool done = false;
+  // the device copy of this tree (device-side regrid) is stale: refreshed
+  // from this one at the next topology rebuild
+  bool dev_tree_dirty = true;
   // ghost-fill program of this instance (global slot ids); recompiled only
   // when its tree changed, kept in its own device segment (nops ops, ngen
   // explicit-operand entries); ops/gen are the host arrays of the last compile
@@ -481,6 +484,20 @@ struct amr_ctx {
   int fail_nb = -1;           // block of a host-detected regrid failure
   RegridOp* d_rops = nullptr;
   int cap_rops = 0;
+  // device-side regrid (regrid.hip): the device copy of every instance's
+  // tree (instance q's blocks from tree_off[q]), its per-instance parameters
+  // (view into d_topo) and results (views into d_rec / h_rec: per-instance
+  // records, then the op log indexed like the leaf list); off: the host
+  // decides from the downloaded per-leaf indicators
+  bool regrid_device = true;
+  bool regrid_lds = true;  // stage trees that fit in LDS (else every tree is worked on in place)
+  DeviceTree dtree{};
+  std::vector<int32_t> tree_off;
+  RegridInst* d_rgi = nullptr;
+  RegridOut *d_rgo = nullptr, *h_rgo = nullptr;
+  int32_t *d_oplog = nullptr, *h_oplog = nullptr;
+  size_t rec_off_log = 0, rec_off_red = 0;
+  std::vector<uint8_t> rg_on;  // instances whose regrid the device runs (set at each topology rebuild)
   int32_t *d_cmapU = nullptr, *d_cmapB = nullptr;
   double* d_W = nullptr;  // initial-condition primitives for the first regrid
   bool topo_dirty = true;
@@ -1561,8 +1578,69 @@ void grow_tree(Instance& I, int n) {
   }
 }
 
-// admesh.cpp:17-102 given the per-leaf indicator (criteria.cpp:151-163),
-// with critneighup.cpp / critneighdown.cpp.  Emits the data-transfer ops.
+// admesh.cpp:33-58 for one block marked for refinement: its son group (a
+// new one at lastActive, or the one it had before), the data-transfer op, the
+// sons' coordinates and updatelpup.  `first_ref`: the instance's first op in refs.
+int refine_block(amr_ctx* c, int ii, int nb, std::vector<RegridOp>& refs, size_t first_ref) {
+  Instance& I = c->inst[ii];
+  auto& lp = I.lp;
+  const int side = c->NT - 2 * c->ng;
+  int s[4];
+  if (lp[nb][2] == 0) {
+    s[0] = I.lastActive;
+    s[1] = s[0] + 1; s[2] = s[0] + 2; s[3] = s[0] + 3;
+    lp[nb][2] = s[0];
+    I.lastActive += 4;
+    if (I.lastActive - 1 >= I.maxblocks) { c->fail_nb = nb; return AMR_ERR_OVERFLOW; }
+    grow_tree(I, I.lastActive);
+  } else {
+    s[0] = lp[nb][2]; s[1] = lp[s[0]][5]; s[2] = lp[s[0]][7]; s[3] = lp[s[2]][5];
+  }
+  RegridOp op{};
+  op.parent_gs = slot_of(c, I, nb);
+  for (int q = 0; q < 4; q++) {
+    op.son_gs[q] = slot_of(c, I, s[q]);
+    if (op.son_gs[q] < 0) { c->fail_nb = s[q]; return AMR_ERR_CAPACITY; }
+  }
+  op.inst = ii;
+  op.nb = nb;
+  op.son_nb = s[0];
+  op.seq = (int32_t)(refs.size() - first_ref);
+  refs.push_back(op);
+  // refine.cpp:61-84
+  const int x0 = I.icoord[nb][0] * 2, y0 = I.icoord[nb][1] * 2;
+  for (int q = 0; q < 4; q++) {
+    I.icoord[s[q]][0] = x0 + ((q & 1) ? side : 0);
+    I.icoord[s[q]][1] = y0 + ((q & 2) ? side : 0);
+  }
+  link_sons(I, nb, s);
+  return 0;
+}
+
+// admesh.cpp:69-92 for one group of four siblings coarsened into their dad
+// (first son nb): the data-transfer op and updatelpdown.  `nrefs`: the
+// instance's refinements of this regrid (coarsenings are numbered after them).
+int coarsen_group(amr_ctx* c, int ii, int nb, size_t nrefs, std::vector<RegridOp>& coars, size_t first_coars) {
+  Instance& I = c->inst[ii];
+  auto& lp = I.lp;
+  const int dad = lp[nb][1];
+  int s[4] = {nb, lp[nb][5], lp[nb][7], 0};
+  s[3] = lp[s[1]][7];
+  RegridOp op{};
+  op.parent_gs = slot_of(c, I, dad);
+  for (int q = 0; q < 4; q++) op.son_gs[q] = slot_of(c, I, s[q]);
+  op.inst = ii;
+  op.nb = dad;
+  op.son_nb = s[0];
+  op.seq = (int32_t)(nrefs + coars.size() - first_coars);
+  if (int e = unlink_sons(I, dad, s)) { c->fail_nb = dad; return e; }
+  coars.push_back(op);
+  return 0;
+}
+
+// admesh.cpp:17-102 on the host given the per-leaf indicator
+// (criteria.cpp:8-43), with critneighup.cpp / critneighdown.cpp.  Emits the
+// data-transfer ops.
 int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std::vector<RegridOp>& refs,
                   std::vector<RegridOp>& coars) {
   Instance& I = c->inst[ii];
@@ -1585,40 +1663,10 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
           const int n1 = lp[nb][s];
           if (n1 != -1 && lp[n1][0] < lp[nb][0]) I.iref[n1] = 1;
         }
-  const int side = c->NT - 2 * c->ng;
   for (int nl = 0; nl < I.maxlevs - 1; nl++)
-    for (int nb = 0; nb < I.lastActive; nb++) {
-      if (!(lp[nb][0] == nl && I.iref[nb])) continue;
-      int s[4];
-      if (lp[nb][2] == 0) {
-        s[0] = I.lastActive;
-        s[1] = s[0] + 1; s[2] = s[0] + 2; s[3] = s[0] + 3;
-        lp[nb][2] = s[0];
-        I.lastActive += 4;
-        if (I.lastActive - 1 >= I.maxblocks) { c->fail_nb = nb; return AMR_ERR_OVERFLOW; }
-        grow_tree(I, I.lastActive);
-      } else {
-        s[0] = lp[nb][2]; s[1] = lp[s[0]][5]; s[2] = lp[s[0]][7]; s[3] = lp[s[2]][5];
-      }
-      RegridOp op{};
-      op.parent_gs = slot_of(c, I, nb);
-      for (int q = 0; q < 4; q++) {
-        op.son_gs[q] = slot_of(c, I, s[q]);
-        if (op.son_gs[q] < 0) { c->fail_nb = s[q]; return AMR_ERR_CAPACITY; }
-      }
-      op.inst = ii;
-      op.nb = nb;
-      op.son_nb = s[0];
-      op.seq = (int32_t)(refs.size() - first_ref);
-      refs.push_back(op);
-      // refine.cpp:61-84
-      const int x0 = I.icoord[nb][0] * 2, y0 = I.icoord[nb][1] * 2;
-      for (int q = 0; q < 4; q++) {
-        I.icoord[s[q]][0] = x0 + ((q & 1) ? side : 0);
-        I.icoord[s[q]][1] = y0 + ((q & 2) ? side : 0);
-      }
-      link_sons(I, nb, s);
-    }
+    for (int nb = 0; nb < I.lastActive; nb++)
+      if (lp[nb][0] == nl && I.iref[nb])
+        if (int e = refine_block(c, ii, nb, refs, first_ref)) return e;
   for (int nb = 0; nb < I.lastActive; nb++) {
     if (!I.icoarse[nb]) continue;
     for (int s = 4; s < 8; s++) {
@@ -1629,20 +1677,38 @@ int regrid_decide(amr_ctx* c, int ii, const std::vector<double>& grad_of_nb, std
   for (int nl = I.maxlevs - 1; nl >= 1; nl--)
     for (int nb = 0; nb < I.lastActive; nb++) {
       if (!(I.icoarse[nb] && lp[nb][3] == 0 && lp[nb][0] == nl)) continue;
-      const int dad = lp[nb][1];
-      int s[4] = {nb, lp[nb][5], lp[nb][7], 0};
-      s[3] = lp[s[1]][7];
-      if (!(I.icoarse[s[1]] && I.icoarse[s[2]] && I.icoarse[s[3]])) continue;
-      RegridOp op{};
-      op.parent_gs = slot_of(c, I, dad);
-      for (int q = 0; q < 4; q++) op.son_gs[q] = slot_of(c, I, s[q]);
-      op.inst = ii;
-      op.nb = dad;
-      op.son_nb = s[0];
-      op.seq = (int32_t)(refs.size() - first_ref + coars.size() - first_coars);
-      if (int e = unlink_sons(I, dad, s)) { c->fail_nb = dad; return e; }
-      coars.push_back(op);
+      const int s1 = lp[nb][5], s2 = lp[nb][7], s3 = lp[s1][7];
+      if (!(I.icoarse[s1] && I.icoarse[s2] && I.icoarse[s3])) continue;
+      if (int e = coarsen_group(c, ii, nb, refs.size() - first_ref, coars, first_coars)) return e;
     }
+  return 0;
+}
+
+// The regrid the device decided (regrid.hip) applied to the host copy of
+// instance ii's tree: its refinements, then its coarsenings, in the order the
+// device recorded them, with the same surgery as regrid_decide.  The ids come
+// from the device: each is checked against the host tree before it is used.
+int regrid_replay(amr_ctx* c, int ii, const RegridOut& o, const int32_t* log, std::vector<RegridOp>& refs,
+                  std::vector<RegridOp>& coars) {
+  Instance& I = c->inst[ii];
+  const size_t first_ref = refs.size(), first_coars = coars.size();
+  auto bad = [&](const char* what, int nb) {
+    std::fprintf(stderr, "amr2d_mhd: device regrid of instance %d: %s (block %d)\n", ii, what, nb);
+    return AMR_ERR_HIP;
+  };
+  for (int k = 0; k < o.nref; k++) {
+    const int nb = log[k];
+    if (nb < 0 || nb >= I.lastActive || !I.leafs[nb] || I.lp[nb][0] >= I.maxlevs - 1)
+      return bad("refinement of a block that cannot refine", nb);
+    if (int e = refine_block(c, ii, nb, refs, first_ref)) return e;
+  }
+  for (int k = 0; k < o.ncoars; k++) {
+    const int nb = log[o.nref + k];
+    if (nb < kNbRoots || nb + 3 >= I.lastActive || I.lp[nb][3] != 0 || I.lp[nb][5] != nb + 1 ||
+        I.lp[nb][7] != nb + 2 || I.lp[nb + 1][7] != nb + 3)
+      return bad("coarsening of a block that is not a first son", nb);
+    if (int e = coarsen_group(c, ii, nb, refs.size() - first_ref, coars, first_coars)) return e;
+  }
   return 0;
 }
 
@@ -1936,6 +2002,23 @@ void plan_stage_split(amr_ctx* c) {
   c->stage_split = true;
 }
 
+// The device copy of instance q's tree refreshed from the host tree: rows
+// and leaf flags of its blocks, zero beyond lastActive (blocks a regrid may
+// allocate), work flags cleared.
+int upload_tree(amr_ctx* c, int q) {
+  Instance& I = c->inst[q];
+  const size_t off = (size_t)c->tree_off[q], n = (size_t)I.lastActive, mb = (size_t)I.maxblocks;
+  HIPCHK(hipMemcpyAsync(c->dtree.lp + 8 * off, I.lp.data(), sizeof(int32_t) * 8 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->dtree.lp + 8 * (off + n), 0, sizeof(int32_t) * 8 * (mb - n), c->stream));
+  HIPCHK(hipMemcpyAsync(c->dtree.leafs + off, I.leafs.data(), n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->dtree.leafs + off + n, 0, mb - n, c->stream));
+  HIPCHK(hipMemsetAsync(c->dtree.iref + off, 0, mb, c->stream));
+  HIPCHK(hipMemsetAsync(c->dtree.icoarse + off, 0, mb, c->stream));
+  HIPCHK(hipMemsetAsync(c->dtree.mark + off, 0, mb, c->stream));
+  I.dev_tree_dirty = false;
+  return 0;
+}
+
 int rebuild_topology(amr_ctx* c) {
   HostTimed ht(c, PF_HOST_TOPO);
   HIPCHK(hipStreamSynchronize(c->stream));  // previous uploads consumed
@@ -2028,7 +2111,8 @@ int rebuild_topology(amr_ctx* c) {
   const size_t off_ops = 256;
   const size_t off_gen = off_ops + align(sizeof(FillOp*) * (ni + 1));
   const size_t off_pre = off_gen + align(sizeof(FillGen*) * (ni + 1));
-  const size_t off_leaf = off_pre + align(sizeof(int32_t) * (ni + 1));
+  const size_t off_rgi = off_pre + align(sizeof(int32_t) * (ni + 1));
+  const size_t off_leaf = off_rgi + align(sizeof(RegridInst) * ni);
   const size_t off_split = off_leaf + align(sizeof(LeafItem) * std::max(1, c->nleaf));
   const size_t topo_bytes = off_split + sizeof(LeafItem) * std::max(1, c->nleaf);
   if (topo_bytes > c->cap_topo) {
@@ -2056,6 +2140,30 @@ int rebuild_topology(amr_ctx* c) {
     h_pre[q + 1] = total;
   }
   c->nchunks = total;
+  // the device regrid's per-instance parameters; device trees that are stale
+  // are refreshed from the host trees first
+  RegridInst* h_rgi = (RegridInst*)(c->h_topo + off_rgi);
+  c->rg_on.assign(ni, 0);
+  bool tree_up = false;
+  for (int q = 0; q < ni; q++) {
+    Instance& I = c->inst[q];
+    const bool on = c->regrid_device && !I.done && I.maxlevs > 1;
+    // every block a regrid can reach: the current ones and the new son groups
+    // of leaves that can refine and have none yet
+    int fresh = 0;
+    if (on)
+      for (int nb = 0; nb < I.lastActive; nb++) fresh += I.leafs[nb] && I.lp[nb][0] < I.maxlevs - 1 && I.lp[nb][2] == 0;
+    const int64_t reach = std::min<int64_t>(I.maxblocks, (int64_t)I.lastActive + 4 * (int64_t)fresh);
+    h_rgi[q] = RegridInst{c->tree_off[q], I.maxblocks, I.maxlevs, I.lastActive, c->leaf_inst_begin[q],
+                          c->leaf_inst_begin[q + 1], on ? 1 : 0,
+                          (c->regrid_lds && reach <= kRgLdsRows) ? (int32_t)reach : 0};
+    c->rg_on[q] = on;
+    if (on && I.dev_tree_dirty) {
+      if (int e = upload_tree(c, q)) return e;
+      tree_up = true;
+    }
+  }
+  if (tree_up) HIPCHK(hipStreamSynchronize(c->stream));  // the host trees may change after this call
   ((int32_t*)c->h_topo)[kCountLeaves] = c->nleaf;
   ((int32_t*)c->h_topo)[kCountChunks] = c->nchunks;
   if (c->nleaf) std::memcpy(c->h_topo + off_leaf, c->h_leaves.data(), sizeof(LeafItem) * c->nleaf);
@@ -2070,6 +2178,7 @@ int rebuild_topology(amr_ctx* c) {
   c->d_inst_ops = (const FillOp**)(c->d_topo + off_ops);
   c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
   c->d_prefix = (int32_t*)(c->d_topo + off_pre);
+  c->d_rgi = (RegridInst*)(c->d_topo + off_rgi);
   c->d_leaves = (LeafItem*)(c->d_topo + off_leaf);
   c->topo_dirty = false;
   c->steps_same_topo = 0;
@@ -2299,6 +2408,36 @@ int apply_regrid_ops(amr_ctx* c, const std::vector<RegridOp>& ops, bool refine) 
   return 0;
 }
 
+// The end of a step's device work: per-leaf CFL bounds and indicators
+// (primitives from dW for the initial regrid, else from U), then, with the
+// device regrid, the regrid decision and the CFL minimum per instance.
+// `grid`: leaf-kernel grid (the leaf count, or a captured graph's capacity).
+void launch_cfl_regrid(amr_ctx* c, const double* dW, int grid) {
+  {
+    Timed tm(c, PF_CFL);
+    launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, grid, c->d_ip, c->d_red, c->d_err, c->stream,
+                        c->launch_counts);
+  }
+  if (c->regrid_device) {
+    Timed tm(c, PF_REGRID);
+    launch_regrid(c->d_rgi, (int)c->inst.size(), c->d_leaves, c->d_red, c->dtree, c->d_rgo, c->d_oplog, c->stream);
+  }
+}
+
+// The step's one download: the error records, then the regrid records and op
+// log (device regrid) or the per-leaf CFL bounds and indicators (host regrid).
+int download_results(amr_ctx* c, int nleaf, bool regrid_records) {
+  if (regrid_records) {
+    HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_off_log + sizeof(int32_t) * nleaf, hipMemcpyDeviceToHost,
+                          c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes, hipMemcpyDeviceToHost, c->stream));
+    if (nleaf > 0)
+      HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * 2 * nleaf, hipMemcpyDeviceToHost, c->stream));
+  }
+  return 0;
+}
+
 // per-leaf CFL bounds and indicators of the running instances -> host
 int cfl_and_indicators(amr_ctx* c, const double* dW) {
   {
@@ -2306,8 +2445,15 @@ int cfl_and_indicators(amr_ctx* c, const double* dW) {
     launch_cfl_criteria(c->NT, c->ng, c->F, dW, c->d_leaves, c->nleaf, c->d_ip, c->d_red, c->d_err,
                         c->stream);
   }
-  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
-                        hipMemcpyDeviceToHost, c->stream));
+  if (int e = download_results(c, c->nleaf, false)) return e;
+  return wait_errors(c);
+}
+
+// the same with the device regrid: CFL bounds, indicators, the regrid
+// decision and the per-instance CFL minimum, downloaded and waited for
+int cfl_and_regrid(amr_ctx* c, const double* dW) {
+  launch_cfl_regrid(c, dW, c->nleaf);
+  if (int e = download_results(c, c->nleaf, true)) return e;
   return wait_errors(c);
 }
 
@@ -2329,6 +2475,65 @@ int regrid_all(amr_ctx* c) {
     const size_t nr = refs.size(), nc = coars.size();
     if (int e = regrid_decide(c, (int)q, grad, refs, coars)) {
       if (e == AMR_ERR_HIP) return e;
+      fail_instance(c, (int)q, e, c->fail_nb, 0, 0);
+      continue;
+    }
+    if (refs.size() != nr || coars.size() != nc || I.lastActive != before) {
+      c->topo_dirty = true;
+      I.prog_dirty = true;
+    }
+  }
+  c->regrid_launched = !refs.empty() || !coars.empty();
+  if (int e = apply_regrid_ops(c, refs, true)) return e;
+  if (int e = apply_regrid_ops(c, coars, false)) return e;
+  return 0;
+}
+
+// admesh for every running instance as the device decided it (downloaded
+// records and op log): the host trees follow, the data-transfer ops run.
+// An instance that failed in this step, or whose replay stops at an error,
+// keeps a host tree the device copy no longer matches: its copy is marked
+// stale.
+int regrid_replay_all(amr_ctx* c) {
+  std::vector<RegridOp>& refs = c->h_refs;
+  std::vector<RegridOp>& coars = c->h_coars;
+  HIPCHK(hipStreamSynchronize(c->stream));  // previous ops consumed
+  refs.clear();
+  coars.clear();
+  HostTimed ht(c, PF_HOST_REGRID);
+  // a device result the host trees do not bear out: an internal error; every
+  // device copy is refreshed from the host trees before it is used again
+  auto internal = [&]() {
+    for (auto& J : c->inst) J.dev_tree_dirty = true;
+    c->topo_dirty = true;
+    return AMR_ERR_HIP;
+  };
+  for (size_t q = 0; q < c->inst.size(); q++) {
+    Instance& I = c->inst[q];
+    if (q >= c->rg_on.size() || !c->rg_on[q]) continue;
+    if (I.done) {
+      I.dev_tree_dirty = true;
+      continue;
+    }
+    const RegridOut& o = c->h_rgo[q];
+    const int lb = c->leaf_inst_begin[q], nl = c->leaf_inst_begin[q + 1] - lb;
+    if (o.nref < 0 || o.ncoars < 0 || o.nref + o.ncoars > nl || (o.status != 0 && o.status != kErrOverflow)) {
+      std::fprintf(stderr, "amr2d_mhd: device regrid of instance %d: %d + %d ops for %d leaves, status %d\n", (int)q,
+                   o.nref, o.ncoars, nl, o.status);
+      return internal();
+    }
+    const int before = I.lastActive;
+    const size_t nr = refs.size(), nc = coars.size();
+    const int e = regrid_replay(c, (int)q, o, c->h_oplog + lb, refs, coars);
+    if (e == AMR_ERR_HIP) return internal();
+    if ((!e || e == AMR_ERR_OVERFLOW) &&
+        (I.lastActive != o.last_active || (e == AMR_ERR_OVERFLOW) != (o.status == kErrOverflow))) {
+      std::fprintf(stderr, "amr2d_mhd: device regrid of instance %d: lastActive %d (host %d), status %d (host %d)\n",
+                   (int)q, o.last_active, I.lastActive, o.status, e);
+      return internal();
+    }
+    if (e) {
+      I.dev_tree_dirty = true;
       fail_instance(c, (int)q, e, c->fail_nb, 0, 0);
       continue;
     }
@@ -2426,8 +2631,11 @@ int enqueue_step_persistent(amr_ctx* c) {
       return AMR_ERR_HIP;
     }
   }
-  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->nleaf,
-                        hipMemcpyDeviceToHost, c->stream));
+  if (c->regrid_device) {
+    Timed tm(c, PF_REGRID);
+    launch_regrid(c->d_rgi, (int)c->inst.size(), c->d_leaves, c->d_red, c->dtree, c->d_rgo, c->d_oplog, c->stream);
+  }
+  if (int e = download_results(c, c->nleaf, c->regrid_device)) return e;
   if (stamps) {
     HIPCHK(hipMemcpyAsync(c->h_stamps, c->d_stamps, sizeof(unsigned long long) * (A.nphases + 1),
                           hipMemcpyDeviceToHost, c->stream));
@@ -2471,14 +2679,8 @@ int enqueue_step(amr_ctx* c) {
     run_ct(c, 3);
   }
   run_fill(c, make_set(c, {{kU, kFillCell}}));
-  {
-    Timed tm(c, PF_CFL);
-    launch_cfl_criteria(c->NT, c->ng, c->F, nullptr, c->d_leaves, c->grid_leaves, c->d_ip, c->d_red, c->d_err,
-                        c->stream);
-  }
-  // error records and per-leaf values in one download
-  HIPCHK(hipMemcpyAsync(c->h_rec, c->d_rec, c->rec_err_bytes + sizeof(double) * 2 * c->grid_leaves,
-                        hipMemcpyDeviceToHost, c->stream));
+  launch_cfl_regrid(c, nullptr, c->grid_leaves);
+  if (int e = download_results(c, c->grid_leaves, c->regrid_device)) return e;
   if (trial)
     HIPCHK(hipMemcpyAsync(c->h_span, c->d_span, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   return 0;
@@ -2586,14 +2788,17 @@ int step_end(amr_ctx* c, int* n_active) {
     Instance& I = c->inst[q];
     if (I.done) continue;
     double lambda = 1.E6;
-    for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) lambda = std_min(lambda, c->h_red[2 * l]);
+    if (c->regrid_device)
+      lambda = c->h_rgo[q].lambda;
+    else
+      for (int l = c->leaf_inst_begin[q]; l < c->leaf_inst_begin[q + 1]; l++) lambda = std_min(lambda, c->h_red[2 * l]);
     I.t = I.t + I.dt_main;
     I.dt_dom = I.dt_main;
     I.count = I.count + 1;
     I.dt_main = I.cfl * lambda;
     if (I.dt_main < 0 || std::isnan(I.dt_main)) fail_instance(c, (int)q, AMR_ERR_DT, -1, 0, 0);
   }
-  if (int e = regrid_all(c)) return e;
+  if (int e = c->regrid_device ? regrid_replay_all(c) : regrid_all(c)) return e;
   if (c->regrid_launched)
     if (int e = sync_errors(c)) return e;  // refine/coarsen errors
   for (auto& I : c->inst) {
@@ -2712,11 +2917,22 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
   }
   // error records, then the step kernel's barrier-failure flag
   c->rec_err_bytes = (sizeof(DevError) * n_inst + sizeof(int32_t) + 255) & ~size_t(255);
-  const size_t rec_bytes = c->rec_err_bytes + sizeof(double) * 2 * (size_t)std::max<int64_t>(1, c->cap_slots);
+  // [error records | regrid records | op log | per-leaf CFL bounds and indicators]
+  const size_t cap_leaves = (size_t)std::max<int64_t>(1, c->cap_slots);
+  c->rec_off_log = c->rec_err_bytes + ((sizeof(RegridOut) * n_inst + 255) & ~size_t(255));
+  c->rec_off_red = c->rec_off_log + ((sizeof(int32_t) * cap_leaves + 255) & ~size_t(255));
+  const size_t rec_bytes = c->rec_off_red + sizeof(double) * 2 * cap_leaves;
+  c->tree_off.assign(n_inst + 1, 0);
+  for (int q = 0; q < n_inst; q++) c->tree_off[q + 1] = c->tree_off[q] + c->inst[q].maxblocks;
+  const size_t tree_blocks = (size_t)c->tree_off[n_inst];
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_ip, sizeof(InstParams) * n_inst) != hipSuccess ||
       hipMalloc((void**)&c->d_rec, rec_bytes) != hipSuccess ||
       hipMemset(c->d_rec, 0, rec_bytes) != hipSuccess ||
+      hipMalloc((void**)&c->dtree.lp, sizeof(int32_t) * 8 * tree_blocks) != hipSuccess ||
+      hipMemset(c->dtree.lp, 0, sizeof(int32_t) * 8 * tree_blocks) != hipSuccess ||
+      hipMalloc((void**)&c->dtree.leafs, 4 * tree_blocks) != hipSuccess ||
+      hipMemset(c->dtree.leafs, 0, 4 * tree_blocks) != hipSuccess ||
       hipHostMalloc((void**)&c->h_rec, rec_bytes) != hipSuccess ||
       hipMalloc((void**)&c->d_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_span, 4 * sizeof(unsigned long long)) != hipSuccess ||
@@ -2732,10 +2948,17 @@ int amr_create(int n_inst, const amr_sim_config* cfgs, int64_t slot_capacity, am
   std::memset(c->h_rec, 0, rec_bytes);
   c->d_err = (DevError*)c->d_rec;
   c->h_err = (DevError*)c->h_rec;
-  c->d_red = (double*)(c->d_rec + c->rec_err_bytes);
+  c->d_red = (double*)(c->d_rec + c->rec_off_red);
   c->d_fail = (int32_t*)(c->d_rec + sizeof(DevError) * n_inst);
   c->h_fail = (int32_t*)(c->h_rec + sizeof(DevError) * n_inst);
-  c->h_red = (double*)(c->h_rec + c->rec_err_bytes);
+  c->h_red = (double*)(c->h_rec + c->rec_off_red);
+  c->d_rgo = (RegridOut*)(c->d_rec + c->rec_err_bytes);
+  c->h_rgo = (RegridOut*)(c->h_rec + c->rec_err_bytes);
+  c->d_oplog = (int32_t*)(c->d_rec + c->rec_off_log);
+  c->h_oplog = (int32_t*)(c->h_rec + c->rec_off_log);
+  c->dtree.iref = c->dtree.leafs + tree_blocks;
+  c->dtree.icoarse = c->dtree.iref + tree_blocks;
+  c->dtree.mark = c->dtree.icoarse + tree_blocks;
   {
     int dev = 0, khz = 0;
     if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
@@ -2774,6 +2997,7 @@ int amr_destroy(amr_ctx* c) {
     }
   }
   hipFree(c->d_ip); hipFree(c->d_rec); hipFree(c->d_rops);
+  hipFree(c->dtree.lp); hipFree(c->dtree.leafs);
   hipFree(c->d_cmapU); hipFree(c->d_cmapB); hipFree(c->d_W);
   if (c->h_rec) hipHostFree(c->h_rec);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -2817,6 +3041,7 @@ int amr_init(amr_ctx* c) {
       if (slot_of(c, I, r) < 0) return set_error(c, AMR_ERR_CAPACITY, (int)q, r, 0, 0);
     }
     I.lastActive = kNbRoots;
+    I.dev_tree_dirty = true;
     I.t = 0; I.dt_main = 0; I.dt_dom = 0; I.count = 0; I.done = false;
     for (int r = 0; r < kNbRoots; r++) {
       std::vector<double> W, B, U;
@@ -2835,10 +3060,10 @@ int amr_init(amr_ctx* c) {
   HIPCHK(hipMalloc((void**)&c->d_W, sizeof(double) * Wall.size()));
   HIPCHK(hipMemcpy(c->d_W, Wall.data(), sizeof(double) * Wall.size(), hipMemcpyHostToDevice));
   c->dma_since_direct = true;
-  if (int e = cfl_and_indicators(c, c->d_W)) return e;
+  if (int e = c->regrid_device ? cfl_and_regrid(c, c->d_W) : cfl_and_indicators(c, c->d_W)) return e;
   hipFree(c->d_W);
   c->d_W = nullptr;
-  if (int e = regrid_all(c)) return e;
+  if (int e = c->regrid_device ? regrid_replay_all(c) : regrid_all(c)) return e;
   c->topo_dirty = true;
   if (int e = sync_errors(c)) return e;
   return running(c) ? AMR_OK : c->call_fail;
@@ -3032,9 +3257,17 @@ int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* 
     I.icoord[nb] = {ic[2 * nb], ic[2 * nb + 1]};
     I.leafs[nb] = (uint8_t)lf[nb];
   }
+  // blocks past the new tree are unused: a regrid allocates its new son
+  // groups there and expects them blank (a longer tree may have left rows)
+  for (size_t nb = n; nb < I.lp.size(); nb++) {
+    I.lp[nb] = std::array<int32_t, 8>{0, 0, 0, 0, 0, 0, 0, 0};
+    I.icoord[nb] = std::array<int32_t, 2>{0, 0};
+    I.leafs[nb] = I.iref[nb] = I.icoarse[nb] = 0;
+  }
   I.lastActive = n;
   I.done = I.status != 0;
   I.prog_dirty = true;
+  I.dev_tree_dirty = true;
   c->topo_dirty = true;
   return AMR_OK;
   AMR_CATCH
@@ -3298,8 +3531,8 @@ int amr_regrid(amr_ctx* c) {
   if (c->topo_dirty)
     if (int e = rebuild_topology(c)) return e;
   if (int e = upload_params(c)) return e;
-  if (int e = cfl_and_indicators(c, nullptr)) return e;
-  if (int e = regrid_all(c)) return e;
+  if (int e = c->regrid_device ? cfl_and_regrid(c, nullptr) : cfl_and_indicators(c, nullptr)) return e;
+  if (int e = c->regrid_device ? regrid_replay_all(c) : regrid_all(c)) return e;
   return finish_call(c);
   AMR_CATCH
 }
@@ -3376,6 +3609,47 @@ int amr_set_step_kernel(amr_ctx* c, int mode) {
   AMR_TRY
   if (!c || mode < -1 || mode > 2 || c->in_flight) return AMR_ERR_CONFIG;
   c->step_mode = mode < 0 ? 0 : mode;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_set_regrid_device(amr_ctx* c, int on) {
+  AMR_TRY
+  if (!c || c->in_flight || (on != 0 && on != 1)) return AMR_ERR_CONFIG;
+  if (c->regrid_device != (on == 1)) {
+    c->regrid_device = on == 1;
+    for (auto& I : c->inst) I.dev_tree_dirty = true;
+    c->topo_dirty = true;
+  }
+  return AMR_OK;
+  AMR_CATCH
+}
+
+// device regrid: 0 works on every tree in place (global memory), 1 (the
+// default) stages the trees that fit in LDS
+int amr_debug_regrid_lds(amr_ctx* c, int on) {
+  AMR_TRY
+  if (!c || c->in_flight) return AMR_ERR_CONFIG;
+  c->regrid_lds = on != 0;
+  c->topo_dirty = true;
+  return AMR_OK;
+  AMR_CATCH
+}
+
+int amr_debug_device_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* leafs, int* current) {
+  AMR_TRY
+  if (check_inst(c, inst) || !lp || !leafs || !current || c->in_flight) return AMR_ERR_CONFIG;
+  Instance& I = c->inst[inst];
+  if (n < 0 || n > I.maxblocks) return AMR_ERR_CONFIG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t off = (size_t)c->tree_off[inst];
+  std::vector<uint8_t> lf(n);
+  if (n) {
+    HIPCHK(hipMemcpy(lp, c->dtree.lp + 8 * off, sizeof(int32_t) * 8 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lf.data(), c->dtree.leafs + off, n, hipMemcpyDeviceToHost));
+  }
+  for (int nb = 0; nb < n; nb++) leafs[nb] = lf[nb];
+  *current = (c->regrid_device && !I.dev_tree_dirty) ? 1 : 0;
   return AMR_OK;
   AMR_CATCH
 }

@@ -87,6 +87,20 @@ void launch_cfl_criteria(int NT, int ng, const FieldPtrs& F, const double* W, co
                          const int32_t* counts = nullptr);
 void launch_refine(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int nops, const InstParams* ip,
                    DevError* err, hipStream_t s);
+
+// device copy of the quadtrees of a context: lp rows [block][8] and per-block
+// flags; instance blocks from RegridInst::off on
+struct DeviceTree {
+  int32_t* lp;
+  uint8_t* leafs;
+  uint8_t* iref;
+  uint8_t* icoarse;
+  uint8_t* mark;
+};
+// regrid decision and tree surgery of every instance (regrid.hip), one
+// workgroup per instance, from the per-leaf CFL bounds and indicators
+void launch_regrid(const RegridInst* ri, int ninst, const LeafItem* leaves, const double* lam_grad,
+                   const DeviceTree& tree, RegridOut* out, int32_t* oplog, hipStream_t s);
 void launch_coarsen(int NT, int ng, const FieldPtrs& F, const RegridOp* ops, int nops, const int32_t* cmapU,
                     const int32_t* cmapB, const InstParams* ip, DevError* err, hipStream_t s);
 

@@ -170,6 +170,27 @@ class Simulation:
         as one launch, 0 launches per phase, -1 restores the default."""
         _check(self.h, lib().amr_set_step_kernel(self.h, int(mode)))
 
+    def set_regrid_device(self, on=True):
+        """Decide each regrid on the device (the default) or on the host from
+        downloaded per-leaf indicators; the trees and results are identical."""
+        _check(self.h, lib().amr_set_regrid_device(self.h, 1 if on else 0))
+
+    def set_regrid_lds(self, on=True):
+        """Device regrid: stage trees that fit in LDS (the default), or work on
+        every tree in place in device memory (same results)."""
+        _check(self.h, lib().amr_debug_regrid_lds(self.h, 1 if on else 0))
+
+    def device_tree(self, inst=0, nslots=None):
+        """The device copy of the instance's tree that the device regrid works
+        on: (lp[n, 8], leafs[n], current), current False when the copy is
+        stale (host regrid, or a tree set since the last step)."""
+        n = self.info(inst)["last_active"] if nslots is None else nslots
+        lp = np.zeros((n, 8), np.int32)
+        lf = np.zeros(n, np.int32)
+        cur = C.c_int(0)
+        _check(self.h, lib().amr_debug_device_tree(self.h, inst, n, lp.ctypes.data, lf.ctypes.data, C.byref(cur)))
+        return lp, lf, bool(cur.value)
+
     def use_stream(self, stream):
         """Run the context on `stream` (a torch.cuda.Stream or a raw hipStream_t
         handle as an int), or on its own stream again when `stream` is None."""

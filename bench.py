@@ -392,6 +392,7 @@ def rotor_tuning():
 
 
 STEP_KERNEL = 0  # engine step mode of the rotor runs (--step-kernel)
+REGRID_DEVICE = True  # regrid decided on the device (--regrid)
 # HIP events on one stage launch in EVENT_STRIDE (odd: stage 1 and stage 2
 # launches alternate in the sample); events on every launch cost ~6% of the run
 EVENT_STRIDE = 15
@@ -408,6 +409,7 @@ def rotor_run(warmup, steps, tuning):
     sim = A.Simulation(ROTOR)
     sim.set_stage_tuning(tuning)
     sim.set_step_kernel(STEP_KERNEL)
+    sim.set_regrid_device(REGRID_DEVICE)
     sim.init()
     if warmup:
         sim.run(warmup)
@@ -463,11 +465,14 @@ def main():
                     help="rotor runs with per-phase launches (0) or the persistent step kernel, 256 (1) or 512 (2) "
                          "lanes; with 1 or 2 the stage figures time whole stage phases, barrier included")
     ap.add_argument("--adj-flux", default="both", choices=["HLLE", "HLLD", "both"])
+    ap.add_argument("--regrid", default="device", choices=["device", "host"],
+                    help="where the rotor runs decide each regrid (identical results)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
-    global STEP_KERNEL
+    global STEP_KERNEL, REGRID_DEVICE
     STEP_KERNEL = args.step_kernel
+    REGRID_DEVICE = args.regrid == "device"
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -519,6 +524,8 @@ def main():
                    "zone_updates": win["zone_updates"], "parallelism": "single GPU",
                    "step_execution": ["per-phase launches", "persistent step kernel, 256 lanes",
                                       "persistent step kernel, 512 lanes"][args.step_kernel],
+                   "regrid": ("on the device (markref, critneighup/down, updatelpup/down per instance; the host "
+                              "reads back the op list)") if REGRID_DEVICE else "on the host (per-leaf indicators downloaded)",
                    "stage_tuner": "decisions imported from an untimed run; leaf-count buckets that ran "
                                   f"trial launches inside the timed window: {win['trial_buckets']}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

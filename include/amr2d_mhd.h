@@ -210,6 +210,19 @@ int amr_set_graph(amr_ctx *ctx, int on);
  * phase, and so do phase-level calls (amr_fill, amr_stage, ...). */
 int amr_set_step_kernel(amr_ctx *ctx, int mode);
 
+/* Where the regrid of amr_init / amr_step / amr_run / amr_regrid is decided
+ * (admesh.cpp:17-102 with markref.cpp, critneighup.cpp, critneighdown.cpp,
+ * updatelpup.cpp and updatelpdown.cpp; the reference runs it on the host).
+ * on = 1 (the default): on the device, one workgroup per instance over a
+ * device copy of its tree, right after the CFL/criteria kernel of the same
+ * step; the host reads back each instance's CFL minimum and the ordered list
+ * of its refinements and coarsenings -- no per-leaf values -- and applies
+ * that list to its own copy of the tree (which the ghost-fill compiler
+ * reads).  on = 0: the host downloads the per-leaf CFL bounds and indicators
+ * and decides.  Identical trees and results either way.  Not between
+ * amr_step_begin and amr_step_end (AMR_ERR_CONFIG). */
+int amr_set_regrid_device(amr_ctx *ctx, int on);
+
 /* Stage-kernel variant (no counterpart in the reference; all variants give
  * identical results): lanes per leaf block and where the face fluxes live
  * for the RK update -- 0: 256 lanes, fluxes in LDS; 1: 512 lanes, LDS;

@@ -26,8 +26,8 @@ int main(void) {
       ENTRY(amr_set_graph),       ENTRY(amr_set_step_kernel),  ENTRY(amr_set_stage_variant),
       ENTRY(amr_stage_tuning),    ENTRY(amr_set_stage_tuning), ENTRY(amr_set_fill_cache),
       ENTRY(amr_profile_enable),  ENTRY(amr_profile_read),     ENTRY(amr_profile_sample),
-      ENTRY(amr_adj_create),
-      ENTRY(amr_adj_destroy),     ENTRY(amr_adj_tape_doubles), ENTRY(amr_adj_step),
+      ENTRY(amr_set_regrid_device), ENTRY(amr_adj_create),
+      ENTRY(amr_adj_destroy),    ENTRY(amr_adj_tape_doubles), ENTRY(amr_adj_step),
       ENTRY(amr_adj_step_tape),   ENTRY(amr_adj_vjp_tape),     ENTRY(amr_adj_step_vjp),
       ENTRY(amr_adj_step_noalloc),
   };

@@ -121,3 +121,26 @@ def test_set_tree_rejects_corrupt_tree_unchanged():
     extra = sim.download("U", nslots=lp.shape[0] + 3)
     assert not extra[lp.shape[0]:].any()
     sim.run(3)
+
+
+def test_checkpoint_reloaded_into_a_context_that_ran_on(tmp_path):
+    """A checkpoint loaded back into the context that wrote it, after that
+    context ran on and allocated blocks past the checkpoint's lastActive,
+    continues as the uninterrupted run: the blocks past the loaded tree start
+    blank, so later refinements allocate the same son groups."""
+    import amr2d_mhd_amd as A
+
+    ref = A.Simulation(ROTOR)
+    ref.init()
+    ref.run(150)
+    sim = A.Simulation(ROTOR)
+    sim.init()
+    sim.run(40)
+    path = tmp_path / "run.ckpt"
+    sim.save_checkpoint(path)
+    n_saved = sim.info()["last_active"]
+    sim.run(260)
+    assert sim.info()["last_active"] > n_saved
+    sim.load_checkpoint(path)
+    sim.run(110)
+    _same_run(ref, sim, "reloaded into the context that ran on")
