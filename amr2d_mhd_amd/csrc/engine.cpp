@@ -309,6 +309,7 @@ struct FillProg {
   int cap_ops = 0, nops = 0;
   FillGen* d_gen = nullptr;
   int cap_gen = 0, ngen = 0;
+  std::array<int64_t, 4> kinds{};  // ops per kind (copy, restrict, general, padding)
   uint64_t used = 0;  // LRU stamp
 };
 
@@ -345,6 +346,7 @@ struct Instance {
   int cap_ops = 0, nops = 0;
   FillGen* d_gen = nullptr;
   int cap_gen = 0, ngen = 0;
+  std::array<int64_t, 4> op_kinds{};  // ops of the current program per kind (copy, restrict, general, padding)
   // the tree the current program was compiled for, and earlier programs
   std::vector<int32_t> prog_key;
   uint64_t prog_hash = 0;
@@ -1892,6 +1894,7 @@ void swap_program(Instance& I, FillProg& P) {
   std::swap(I.d_gen, P.d_gen);
   std::swap(I.cap_gen, P.cap_gen);
   std::swap(I.ngen, P.ngen);
+  std::swap(I.op_kinds, P.kinds);
   std::swap(I.prog_key, P.key);
   std::swap(I.prog_hash, P.hash);
 }
@@ -2091,6 +2094,11 @@ int rebuild_topology(amr_ctx* c) {
     Instance& I = c->inst[qi];
     const int n = (int)I.ops.size(), ng = (int)I.gen.size();
     I.nops = n;
+    I.op_kinds = {0, 0, 0, 0};
+    for (int k = 0; k < n; k++) {
+      const int op = I.ops.data()[k].op;
+      if (op >= 0 && op <= kFillNop) I.op_kinds[op] += 1;
+    }
     I.ngen = ng;
     if (int e = ensure(I.d_ops, I.cap_ops, std::max(1, n))) return e;
     if (int e = ensure(I.d_gen, I.cap_gen, std::max(1, ng))) return e;
@@ -3621,6 +3629,19 @@ int amr_set_regrid_device(amr_ctx* c, int on) {
     for (auto& I : c->inst) I.dev_tree_dirty = true;
     c->topo_dirty = true;
   }
+  return AMR_OK;
+  AMR_CATCH
+}
+
+// The instance's current ghost-fill program by op kind: out[0..4] = copies,
+// 2x2 restrictions, general restrictions, padding ops, explicit-operand
+// entries (what one fill launch reads per op and per component).
+int amr_debug_fill_stats(amr_ctx* c, int inst, int64_t* out) {
+  AMR_TRY
+  if (check_inst(c, inst) || !out) return AMR_ERR_CONFIG;
+  const Instance& I = c->inst[inst];
+  for (int k = 0; k < 4; k++) out[k] = I.op_kinds[k];
+  out[4] = I.ngen;
   return AMR_OK;
   AMR_CATCH
 }

@@ -175,6 +175,13 @@ class Simulation:
         downloaded per-leaf indicators; the trees and results are identical."""
         _check(self.h, lib().amr_set_regrid_device(self.h, 1 if on else 0))
 
+    def fill_stats(self, inst=0):
+        """Op mix of the instance's current ghost-fill program: copies, 2x2
+        restrictions, general restrictions, padding ops, explicit-operand entries."""
+        out = np.zeros(5, np.int64)
+        _check(self.h, lib().amr_debug_fill_stats(self.h, inst, out.ctypes.data))
+        return dict(zip(("copy", "restrict", "general", "nop", "gen_entries"), out.tolist()))
+
     def set_regrid_lds(self, on=True):
         """Device regrid: stage trees that fit in LDS (the default), or work on
         every tree in place in device memory (same results)."""

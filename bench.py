@@ -393,6 +393,7 @@ def rotor_tuning():
 
 STEP_KERNEL = 0  # engine step mode of the rotor runs (--step-kernel)
 REGRID_DEVICE = True  # regrid decided on the device (--regrid)
+FILL_STATS = False  # report the fill program's op mix at both ends of the window (--fill-stats)
 # HIP events on one stage launch in EVENT_STRIDE (odd: stage 1 and stage 2
 # launches alternate in the sample); events on every launch cost ~6% of the run
 EVENT_STRIDE = 15
@@ -415,6 +416,7 @@ def rotor_run(warmup, steps, tuning):
         sim.run(warmup)
     torch.cuda.synchronize()
     nleaf0 = sim.info()["nleaf"]
+    fill0 = sim.fill_stats() if FILL_STATS else None
     tuner0 = sim.stage_tuning()
     sim.profile(("stage",), stride=EVENT_STRIDE)
     t0 = time.perf_counter()
@@ -423,13 +425,15 @@ def rotor_run(warmup, steps, tuning):
     secs = time.perf_counter() - t0
     prof = sim.profile_read()
     info = sim.info()
+    fill1 = sim.fill_stats() if FILL_STATS else None
     # a trial launch changes its bucket's per-variant timings
     trial_buckets = sorted(b for b, v in sim.stage_tuning().items() if tuner0.get(b) != v)
     sim.close()
     stage_ms, stage_n = prof["stage"]
     return dict(steps=n, zone_updates=zu, seconds=secs, value=zu / secs, ms_per_step=secs / n * 1e3,
                 t_reached=info["t"], leaves_at_start=nleaf0, leaves_at_end=info["nleaf"],
-                stage_ms=stage_ms, stage_launches=stage_n, trial_buckets=trial_buckets)
+                stage_ms=stage_ms, stage_launches=stage_n, trial_buckets=trial_buckets,
+                fill_program=dict(window_start=fill0, window_end=fill1) if FILL_STATS else None)
 
 
 def main():
@@ -465,13 +469,16 @@ def main():
                     help="rotor runs with per-phase launches (0) or the persistent step kernel, 256 (1) or 512 (2) "
                          "lanes; with 1 or 2 the stage figures time whole stage phases, barrier included")
     ap.add_argument("--adj-flux", default="both", choices=["HLLE", "HLLD", "both"])
+    ap.add_argument("--fill-stats", action="store_true",
+                    help="report the ghost-fill program's op mix (copies, restrictions, padding) at both ends of the window")
     ap.add_argument("--regrid", default="device", choices=["device", "host"],
                     help="where the rotor runs decide each regrid (identical results)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
-    global STEP_KERNEL, REGRID_DEVICE
+    global STEP_KERNEL, REGRID_DEVICE, FILL_STATS
     STEP_KERNEL = args.step_kernel
+    FILL_STATS = args.fill_stats
     REGRID_DEVICE = args.regrid == "device"
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -526,6 +533,7 @@ def main():
                                       "persistent step kernel, 512 lanes"][args.step_kernel],
                    "regrid": ("on the device (markref, critneighup/down, updatelpup/down per instance; the host "
                               "reads back the op list)") if REGRID_DEVICE else "on the host (per-leaf indicators downloaded)",
+                   **({"fill_program": win["fill_program"]} if win["fill_program"] else {}),
                    "stage_tuner": "decisions imported from an untimed run; leaf-count buckets that ran "
                                   f"trial launches inside the timed window: {win['trial_buckets']}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
