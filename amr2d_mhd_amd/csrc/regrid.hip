@@ -11,14 +11,22 @@
 //    (two refined neighbours link their sons to each other whichever goes
 //    first); new son groups are numbered in block order by a prefix count, so
 //    every block gets the ids the serial loop gives it.
-//  * critneighdown reads leaf flags that its loop does not change.
+//  * critneighdown reads leaf flags that its loop does not change, and the
+//    choice of the groups that coarsen reads only fields coarsening does not
+//    change, so the groups of every level are chosen in one pass.
 //  * The coarsenings of one level differ with order only in the stale sons of
 //    two neighbouring dads: the serial loop updates a neighbour's sons when the
 //    neighbour is still refined when the loop reaches this dad, i.e. unless its
 //    own sons were coarsened earlier in the same level (a smaller first-son id).
-// The CFL minimum of main.cpp:208-230 (min over the instance's leaves, from
-// 1e6) is taken by the same workgroup; min over non-NaN values does not depend
-// on the order.
+// Passes that must follow block order give each lane a contiguous block range
+// and order a whole level with one prefix sum over the lanes; levels with
+// nothing to refine or coarsen are skipped.  The CFL minimum of
+// main.cpp:208-230 (min over the instance's leaves, from 1e6) is taken by the
+// same workgroup; min over non-NaN values does not depend on the order.
+//
+// The tree is staged in LDS (16-bit rows and byte flags) when the host's bound
+// on the rows a regrid can reach fits, and written back only if it changed;
+// otherwise the workgroup works on the device copy in place.
 //
 // Results per instance: the CFL minimum, the refinements (parent ids) and
 // coarsenings (first-son ids) in the reference's order, the new lastActive,
@@ -45,7 +53,8 @@ static_assert(kRgLdsRows * 20 + 512 <= 160 * 1024 && kRgLdsRows % 4 == 0, "stage
 
 // one instance's tree: rows [block][8] (level, father, first son, sibling
 // id, left, right, bottom, top) of type R, and per-block flags; `mark` holds
-// the first sons of the groups coarsened at the current level
+// the first sons of the groups that coarsen in this regrid (cleared level by
+// level as they are applied)
 template <typename R>
 struct TreeT {
   R* lp;
@@ -170,7 +179,7 @@ struct Scratch {
   int* fail_seq;
   int* fail_nb;
   int* bad;
-  volatile unsigned* levels;  // levels holding blocks marked for refinement (bit per level)
+  volatile unsigned* levels;  // bit per level: levels with blocks to refine, later levels that coarsen
 };
 
 struct AdmeshResult {
