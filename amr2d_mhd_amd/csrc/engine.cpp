@@ -217,14 +217,17 @@ class RangePool {
       v = state_.load(std::memory_order_acquire);
     }
   }
-  // true once ready() holds, false after ~200 us without
+  // true once ready() holds, false after ~3 ms without.  A regridding run
+  // compiles every few steps (every 2-3 ms on the rotor), so helpers that
+  // keep spinning across that gap start the next compile without a blocking
+  // wake-up; they sleep once compiles stop.
   template <class P>
   static bool spin(const P& ready) {
     const auto t0 = std::chrono::steady_clock::now();
     for (int i = 1;; i++) {
       if (ready()) return true;
       __builtin_ia32_pause();
-      if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) return false;
+      if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(3000)) return false;
     }
   }
   void loop() {
@@ -1059,11 +1062,12 @@ void for_ranges(FillScratch& S, int R, int nthreads, const F& f) {
 }
 
 // ring elements (blocks x ring cells) from which a compile is split over
-// threads (about 230 blocks of a 20x20 tile): below, the helpers' wake-ups
-// and the extra merge pass cost more than they save (whole rotor run on the
-// 16 host threads of a GPU: 1.385 s unsplit, 1.32-1.35 s split from 2^15
-// elements, 1.35-1.39 s from 2^14)
-constexpr size_t kSplitSlots = size_t(1) << 15;
+// threads (about 115 blocks of a 20x20 tile): below, the extra merge pass
+// costs more than it saves.  Whole rotor run on the 16 host threads of a GPU,
+// same box, interleaved: 1.407-1.412 s splitting from 2^15 elements with
+// helpers that spin 200 us, 1.377-1.387 s with a 3 ms spin, 1.378-1.379 s
+// with the 3 ms spin from 2^14.
+constexpr size_t kSplitSlots = size_t(1) << 14;
 
 // Program of one instance: 16-byte destination ops plus the explicit-operand
 // table of restrictions that read an element already written in the same
