@@ -276,6 +276,7 @@ struct FillWorker {
   std::vector<int32_t> gen_ops;  // ops that index gen
   std::vector<RingSrc> ringsrc;  // ops with an operand in a ghost ring
   bool nested = false;           // a restriction of a restriction (rejected)
+  int64_t nkind[3] = {0, 0, 0};  // ops emitted per kind (copy, restrict, general)
   std::vector<std::array<int32_t, 2>> edges;  // (reader op, writer op) of stale reads
   std::vector<int32_t> linked, gid;           // linked ops of this range's op segment, their groups
 };
@@ -1220,6 +1221,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
     W.gen_ops.clear();
     W.ringsrc.clear();
     W.nested = false;
+    int64_t nk[3] = {0, 0, 0};
     // locals, so that the stores below cannot be taken to alias them
     const uint8_t* __restrict rg = ring;
     const int32_t* __restrict ri = ridx;
@@ -1283,13 +1285,19 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
         W.ringsrc.push_back(rs);
       }
       cur[a] = o;  // from here on: the op writing the element
+      nk[op.op]++;
       ops[o++] = op;
     }
+    for (int k = 0; k < 3; k++) W.nkind[k] = nk[k];
     if (R == 1) opoff[1] = o;
   });
   const int n = opoff[R];
   I.ops.resize_uninit(n);
   int status = interior_write ? AMR_ERR_CONFIG : 0;
+  // ops per kind (the padding added below is counted by the caller)
+  I.op_kinds = {0, 0, 0, 0};
+  for (int r = 0; r < R; r++)
+    for (int k = 0; k < 3; k++) I.op_kinds[k] += S.w[r].nkind[k];
   for (int r = 0; r < R; r++) {
     FillWorker& W = S.w[r];
     if (W.nested) status = AMR_ERR_CONFIG;
@@ -1874,21 +1882,40 @@ int upload_params(amr_ctx* c) {
 
 // the tree a fill program depends on (FillProg::key)
 void program_key(const Instance& I, std::vector<int32_t>& key, uint64_t& hash) {
-  key.clear();
-  key.reserve(1 + (size_t)I.lastActive * 10);
-  key.push_back(I.lastActive);
-  for (int nb = 0; nb < I.lastActive; nb++) {
-    for (int s = 0; s < 8; s++) key.push_back(I.lp[nb][s]);
-    key.push_back(I.leafs[nb]);
-    key.push_back(nb < (int)I.gs.size() ? I.gs[nb] : -1);
+  const int n = I.lastActive;
+  key.resize(1 + (size_t)n * 10);
+  int32_t* k = key.data();
+  *k++ = n;
+  for (int nb = 0; nb < n; nb++) {
+    std::memcpy(k, I.lp[nb].data(), sizeof(int32_t) * 8);
+    k[8] = I.leafs[nb];
+    k[9] = nb < (int)I.gs.size() ? I.gs[nb] : -1;
+    k += 10;
   }
-  uint64_t h = 0x9e3779b97f4a7c15ull ^ key.size();
-  for (int32_t v : key) {
-    h ^= (uint32_t)v;
-    h *= 0xff51afd7ed558ccdull;
-    h ^= h >> 32;
+  // four independent multiply-xorshift chains (a single chain is bound by the
+  // multiply latency), then combined
+  const size_t m = key.size();
+  uint64_t h[4] = {0x9e3779b97f4a7c15ull ^ m, 0xc2b2ae3d27d4eb4full ^ m, 0x165667b19e3779f9ull ^ m,
+                   0x27d4eb2f165667c5ull ^ m};
+  size_t i = 0;
+  for (; i + 4 <= m; i += 4)
+    for (int l = 0; l < 4; l++) {
+      h[l] ^= (uint32_t)key[i + l];
+      h[l] *= 0xff51afd7ed558ccdull;
+      h[l] ^= h[l] >> 32;
+    }
+  for (; i < m; i++) {
+    h[0] ^= (uint32_t)key[i];
+    h[0] *= 0xff51afd7ed558ccdull;
+    h[0] ^= h[0] >> 32;
   }
-  hash = h;
+  uint64_t r = h[0];
+  for (int l = 1; l < 4; l++) {
+    r ^= h[l] + 0x9e3779b97f4a7c15ull + (r << 6) + (r >> 2);
+    r *= 0xff51afd7ed558ccdull;
+    r ^= r >> 32;
+  }
+  hash = r;
 }
 
 void swap_program(Instance& I, FillProg& P) {
@@ -2098,11 +2125,7 @@ int rebuild_topology(amr_ctx* c) {
     Instance& I = c->inst[qi];
     const int n = (int)I.ops.size(), ng = (int)I.gen.size();
     I.nops = n;
-    I.op_kinds = {0, 0, 0, 0};
-    for (int k = 0; k < n; k++) {
-      const int op = I.ops.data()[k].op;
-      if (op >= 0 && op <= kFillNop) I.op_kinds[op] += 1;
-    }
+    I.op_kinds[3] = n - I.op_kinds[0] - I.op_kinds[1] - I.op_kinds[2];  // padding
     I.ngen = ng;
     if (int e = ensure(I.d_ops, I.cap_ops, std::max(1, n))) return e;
     if (int e = ensure(I.d_gen, I.cap_gen, std::max(1, ng))) return e;
