@@ -125,6 +125,8 @@ _SIGS = {
                                     C.c_void_p, C.c_void_p]),
     "amr_debug_schedule": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "amr_debug_stage_flops": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "amr_debug_regrid_replay": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_fill_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "amr_debug_regrid_lds": (C.c_int, [C.c_void_p, C.c_int]),
     "amr_debug_device_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -4029,6 +4029,77 @@ int amr_debug_stage_flops(amr_ctx* c, int inst, double* out) {
   AMR_CATCH
 }
 
+// Host-only check of the host half of the device regrid: the regrid of the
+// given tree and per-block indicators (grad[nb], read for leaves) decided on
+// the host (regrid_decide), then its ordered op list -- refinement parents,
+// coarsening first sons, as the device records them -- replayed on a fresh
+// copy of the tree (regrid_replay).  Returns 0 when both give the same tree
+// (every row, coordinates, leaf flags, lastActive), slots and data-transfer
+// ops, 1 when they differ, negative on an input error; the decided tree goes
+// to lp_out/icoord_out/leafs_out (up to `cap` blocks) and counts[0..2] =
+// refinements, coarsenings, lastActive.
+int amr_debug_regrid_replay(int NT, int ng, int maxlevs, int n, const int32_t* lp, const int32_t* icoord,
+                            const int32_t* leafs, const double* grad, int cap, int32_t* lp_out, int32_t* icoord_out,
+                            int32_t* leafs_out, int32_t* counts) {
+  AMR_TRY
+  if (maxlevs < 1 || maxlevs > 7 || !lp || !icoord || !leafs || !grad || !counts) return AMR_ERR_CONFIG;
+  if (validate_tree(n, maxblocks_of(maxlevs), maxlevs, NT - 2 * ng, lp, icoord, leafs)) return AMR_ERR_CONFIG;
+  auto make = [&](amr_ctx& c) {
+    c.NT = NT;
+    c.ng = ng;
+    c.inst.resize(1);
+    Instance& I = c.inst[0];
+    I.maxlevs = maxlevs;
+    I.maxblocks = maxblocks_of(maxlevs);
+    c.cap_slots = I.maxblocks;
+    grow_tree(I, n);
+    for (int nb = 0; nb < n; nb++) {
+      for (int k = 0; k < 8; k++) I.lp[nb][k] = lp[8 * nb + k];
+      I.icoord[nb] = {icoord[2 * nb], icoord[2 * nb + 1]};
+      I.leafs[nb] = (uint8_t)leafs[nb];
+      slot_of(&c, I, nb);
+    }
+    I.lastActive = n;
+  };
+  amr_ctx a, b;
+  make(a);
+  make(b);
+  std::vector<double> g(grad, grad + n);
+  std::vector<RegridOp> ra, ca, rb, cb;
+  const int ea = regrid_decide(&a, 0, g, ra, ca);
+  // the op list a device regrid would record: parents, then first sons
+  std::vector<int32_t> log;
+  for (const auto& o : ra) log.push_back(o.nb);
+  for (const auto& o : ca) log.push_back(o.son_nb);
+  RegridOut o{};
+  o.nref = (int32_t)ra.size();
+  o.ncoars = (int32_t)ca.size();
+  const int eb = regrid_replay(&b, 0, o, log.data(), rb, cb);
+  const Instance &A = a.inst[0], &B = b.inst[0];
+  auto same_ops = [](const std::vector<RegridOp>& x, const std::vector<RegridOp>& y) {
+    if (x.size() != y.size()) return false;
+    return x.empty() || std::memcmp(x.data(), y.data(), sizeof(RegridOp) * x.size()) == 0;
+  };
+  bool same = ea == eb && A.lastActive == B.lastActive && A.gs == B.gs && same_ops(ra, rb) && same_ops(ca, cb);
+  for (int nb = 0; same && nb < A.lastActive; nb++)
+    same = A.lp[nb] == B.lp[nb] && A.icoord[nb] == B.icoord[nb] && A.leafs[nb] == B.leafs[nb];
+  counts[0] = o.nref;
+  counts[1] = o.ncoars;
+  counts[2] = A.lastActive;
+  for (int nb = 0; nb < std::min(cap, A.lastActive); nb++) {
+    if (lp_out)
+      for (int k = 0; k < 8; k++) lp_out[8 * nb + k] = A.lp[nb][k];
+    if (icoord_out) {
+      icoord_out[2 * nb] = A.icoord[nb][0];
+      icoord_out[2 * nb + 1] = A.icoord[nb][1];
+    }
+    if (leafs_out) leafs_out[nb] = A.leafs[nb];
+  }
+  if (ea) return -ea;
+  return same ? 0 : 1;
+  AMR_CATCH
+}
+
 int amr_debug_schedule(int n, const int32_t* lp, const int32_t* leafs, int cap, int32_t* ib3) {
   AMR_TRY
   Instance I;

@@ -367,3 +367,35 @@ def test_operation_counting_build_is_the_real_arithmetic():
     sub = [t for s, t in totals["HLLD"] if not s]
     # (some unshifted random states are supersonic too)
     assert len(sup) == 1 and min(sub) >= max(sup) and max(sub) > max(sup)
+
+
+@pytest.mark.parametrize("maxlevs,seed,p_ref", [(4, 1, 0.35), (5, 2, 0.3), (6, 3, 0.25), (5, 4, 0.6)])
+def test_regrid_replay_equals_host_decision(maxlevs, seed, p_ref):
+    """The host half of the device regrid: an ordered op list (refinement
+    parents, then coarsening first sons, as the device records them) replayed
+    on the host tree gives exactly the tree, slots and data-transfer ops of the
+    host's own admesh -- over 40 successive regrids with random indicators, so
+    that son groups are reused and stale blocks keep pointers."""
+    from amr2d_mhd_amd._lib import lib
+
+    d = O.Domain(ic="BWx", limiter="MC", flux="RUSA", cfl=0.4, nx=16, nghosts=2, maxlevs=maxlevs)
+    assert d.init() == 0
+    lp, ic, lf = [np.ascontiguousarray(a, np.int32) for a in d.get_tree()]
+    rng = np.random.default_rng(seed)
+    nref = ncoars = 0
+    for r in range(40):
+        n = len(lf)
+        grad = np.where(rng.random(n) < p_ref, 1.0, 0.0)
+        cap = 2 * 4 ** maxlevs
+        lp2 = np.zeros((cap, 8), np.int32)
+        ic2 = np.zeros((cap, 2), np.int32)
+        lf2 = np.zeros(cap, np.int32)
+        counts = np.zeros(3, np.int32)
+        rc = lib().amr_debug_regrid_replay(20, 2, maxlevs, n, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data,
+                                           grad.ctypes.data, cap, lp2.ctypes.data, ic2.ctypes.data, lf2.ctypes.data,
+                                           counts.ctypes.data)
+        assert rc == 0, (r, rc)
+        nref, ncoars = nref + counts[0], ncoars + counts[1]
+        m = counts[2]
+        lp, ic, lf = lp2[:m].copy(), ic2[:m].copy(), lf2[:m].copy()
+    assert nref > 20 and ncoars > 5, (nref, ncoars)
