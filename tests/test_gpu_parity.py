@@ -129,8 +129,10 @@ def test_weno5_ng4_bitwise():
     lockstep(sim, dom, 40, full_every=10, label="WENO5 ng4")
 
 
-def test_phases_on_oracle_state():
-    """Each phase applied to the same mid-run rotor state on both sides."""
+@pytest.mark.parametrize("regrid_device", [True, False], ids=["regrid on the device", "regrid on the host"])
+def test_phases_on_oracle_state(regrid_device):
+    """Each phase applied to the same mid-run rotor state on both sides, the
+    regrid (admesh, with its data transfer) last."""
     import amr2d_mhd_amd as A
 
     dom = O.Domain(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5)
@@ -139,6 +141,7 @@ def test_phases_on_oracle_state():
         assert dom.step() == 0
     st = oracle_state(dom)
     sim = A.Simulation(dict(ic="rotor", limiter="MC", flux="HLLD", cfl=0.5, nx=16, nghosts=2, maxlevs=5))
+    sim.set_regrid_device(regrid_device)
     sim.init()
     sim.load_state(st)
     dom.locate_bounds()
@@ -160,6 +163,13 @@ def test_phases_on_oracle_state():
     lam_g = sim.cfl()
     st_o, lam_o = dom.cfl()
     assert st_o == 0 and lam_g == lam_o
+    tree_before = dom.get_tree()[0]
+    sim.regrid()
+    assert dom.regrid() == 0
+    assert not np.array_equal(dom.get_tree()[0][: len(tree_before)], tree_before) or \
+        len(dom.get_tree()[0]) != len(tree_before)  # the regrid changed the tree
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "after regrid: " + "; ".join(d[:4])
 
 
 def test_pressure_error_reported():
