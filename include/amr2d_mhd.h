@@ -261,7 +261,7 @@ int amr_set_fill_cache(amr_ctx *ctx, int entries);
 /* Kernel timing (no counterpart in the reference, which only keeps OpenMP
  * wall-clock accumulators, main.cpp:124-127): launches of the kernel families
  * selected by `mask` (bit 0 stage, 1 ghost fill, 2 CT, 3 CFL/criteria,
- * 4 regrid) are bracketed by HIP events on the context's stream without extra
+ * 4 regrid: the device regrid decision and the refine/coarsen data transfer) are bracketed by HIP events on the context's stream without extra
  * synchronisation (the stage kernel's events are attached to its dispatch);
  * bits 5-7 accumulate host wall time of the topology rebuild (work lists and
  * fill program), of the regrid decisions and of the per-step wait for the
