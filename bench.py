@@ -164,6 +164,7 @@ def ensemble_bench(steps, full, n_inst, plain=True, warmup=0, groups=2):
     def make_batch():
         sims = [A.Simulation(cfgs[bounds[k]:bounds[k + 1]]) for k in range(groups)]
         for b in sims:
+            b.set_regrid_device(REGRID_DEVICE)
             b.init()
         return sims
 
@@ -184,6 +185,7 @@ def ensemble_bench(steps, full, n_inst, plain=True, warmup=0, groups=2):
     tp, zp = 0.0, 0
     for q, cfg in enumerate(cfgs if plain and not warmup else []):
         sim = A.Simulation(cfg)
+        sim.set_regrid_device(REGRID_DEVICE)
         try:
             sim.init()
         except A.AmrError:
@@ -472,7 +474,7 @@ def main():
     ap.add_argument("--fill-stats", action="store_true",
                     help="report the ghost-fill program's op mix (copies, restrictions, padding) at both ends of the window")
     ap.add_argument("--regrid", default="device", choices=["device", "host"],
-                    help="where the rotor runs decide each regrid (identical results)")
+                    help="where the rotor and ensemble runs decide each regrid (identical results)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()

@@ -136,3 +136,32 @@ def test_batch_of_uneven_depths_device_vs_host_regrid():
         assert not d, f"instance {q}: " + "; ".join(d[:4])
     dev.close()
     host.close()
+
+
+def test_ensemble_of_128_rotors_device_vs_host_regrid():
+    """128 seeded rotors (depths 3 to 6, HLLC and HLLD) in one context for 400
+    steps, regrid decided on the device and on the host: every instance's
+    info, tree and slot fields identical."""
+    import amr2d_mhd_amd as A
+    from amr2d_mhd_amd.ensemble import ensemble_configs
+
+    cfgs = ensemble_configs(128)
+    runs = []
+    for dev in (True, False):
+        print("case: 128 rotors, regrid on the", "device" if dev else "host", flush=True)
+        sim = A.Simulation(cfgs)
+        sim.set_regrid_device(dev)
+        sim.init()
+        sim.run(400)
+        runs.append(sim)
+    dev, host = runs
+    changed = 0
+    for q in range(len(cfgs)):
+        a, b = dev.info(q), host.info(q)
+        assert a == b, f"instance {q}: {a} vs {b}"
+        changed += a["last_active"] > 4
+        d = compare_states(gpu_state(dev, q), gpu_state(host, q))
+        assert not d, f"instance {q}: " + "; ".join(d[:4])
+    assert changed > 100
+    dev.close()
+    host.close()
