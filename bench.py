@@ -328,7 +328,7 @@ def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):
     step_vjp_ms = per_call(lambda: mesh.step_vjp(U, ctrl[3], ctrl[4], dts[0], gU, gBx, gBy))
     out = {"workload": f"frozen uniform periodic {n}x{n}, Orszag-Tang controls, minmod-MUSCL + {flux}, CT, RK2, "
                        f"{steps} steps, checkpoints every {every}",
-           "J": float(J), "forward_ms_per_step": tf / steps * 1e3,
+           "J": float(J.detach()), "forward_ms_per_step": tf / steps * 1e3,
            "backward_ms_per_step": tb / steps * 1e3,
            "backward_over_forward": tb / tf,
            "note": "backward recomputes every checkpoint segment, writing each step's tape, then runs the "
