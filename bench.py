@@ -333,7 +333,7 @@ def adjoint_bench(n, steps, every, fd_dirs, flux="HLLE"):
            "backward_over_forward": tb / tf,
            "note": "backward recomputes every checkpoint segment, writing each step's tape, then runs the "
                    "taped reverse sweep",
-           "taped_every_step": {"J": float(J0), "forward_ms_per_step": tf0 / steps * 1e3,
+           "taped_every_step": {"J": float(J0.detach()), "forward_ms_per_step": tf0 / steps * 1e3,
                                 "backward_ms_per_step": tb0 / steps * 1e3, "backward_over_forward": tb0 / tf0,
                                 "tape_bytes": steps * mesh.tape_doubles() * 8},
            "step_ms": step_ms, "step_tape_ms": step_tape_ms, "vjp_tape_ms": vjp_ms, "step_vjp_ms": step_vjp_ms,
