@@ -115,6 +115,8 @@ _SIGS = {
     "amr_debug_prim_vjp": (None, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "amr_debug_host_fill": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
+    "amr_debug_host_fill_tail": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "amr_debug_coarsen_maps": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "amr_debug_compile_ns": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_int]),

@@ -278,6 +278,7 @@ struct FillWorker {
   bool nested = false;           // a restriction of a restriction (rejected)
   int64_t nkind[3] = {0, 0, 0};  // ops emitted per kind (copy, restrict, general)
   std::vector<std::array<int32_t, 2>> edges;  // (reader op, writer op) of stale reads
+  bool self_read = false;        // an op reads (pre-fill) the element it writes
   std::vector<int32_t> linked, gid;           // linked ops of this range's op segment, their groups
 };
 
@@ -314,6 +315,7 @@ struct FillProg {
   FillGen* d_gen = nullptr;
   int cap_gen = 0, ngen = 0;
   std::array<int64_t, 4> kinds{};  // ops per kind (copy, restrict, general, padding)
+  int tail_chunk0 = 0;             // Instance::tail_chunk0 of this program
   uint64_t used = 0;  // LRU stamp
 };
 
@@ -351,6 +353,11 @@ struct Instance {
   FillGen* d_gen = nullptr;
   int cap_gen = 0, ngen = 0;
   std::array<int64_t, 4> op_kinds{};  // ops of the current program per kind (copy, restrict, general, padding)
+  // first chunk of the current program's linked (stale-read) groups: applied
+  // again to its own output, every op before it writes the value already
+  // there, so a repeated fill needs only the chunks from here on (0: every
+  // chunk, when an op reads its own destination)
+  int tail_chunk0 = 0;
   // the tree the current program was compiled for, and earlier programs
   std::vector<int32_t> prog_key;
   uint64_t prog_hash = 0;
@@ -405,6 +412,20 @@ struct amr_ctx {
   const FillGen** d_inst_gen = nullptr;
   int32_t* d_prefix = nullptr;
   int nchunks = 0;
+  // the same tables over each program's linked tail (Instance::tail_chunk0 on)
+  const FillOp** d_tail_ops = nullptr;
+  int32_t* d_tail_prefix = nullptr;
+  int ntail_chunks = 0;
+  // Set by a step that leaves every tree as it was (no regrid op, no instance
+  // retired) and that ended, like every step, with the fills of Bi (before
+  // the last cell sync) and of U (after it); cleared by every other entry
+  // point that takes the context.  While set, U and Bi hold those fills'
+  // results and nothing has written either since, so the next step's first
+  // fill of U and Bi -- the same programs over the same values -- changes
+  // only what its linked tails write, and runs only those.
+  bool fill_fresh = false;
+  bool tail_first_fill = false;  // the step being enqueued runs its first fill on the tails
+  bool step_direct = false;      // the step in flight was enqueued by enqueue_step, not captured
   InstParams* d_ip = nullptr;
   PinnedVec<InstParams> h_ip;  // pinned: the step graph copies from it
   // the device work of one step replayed as a HIP graph; re-captured when
@@ -1312,6 +1333,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
   for_ranges(S, R, nthreads, [&](int r) {
     FillWorker& W = S.w[r];
     W.edges.clear();
+    W.self_read = false;
     for (const auto& rs : W.ringsrc)
       for (int t = 0; t < rs.nref; t++) {
         const int64_t ref = rs.ref[t];
@@ -1325,12 +1347,17 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
         } else if (const uint32_t m = S.wmask[a]) {
           w = S.w[__builtin_ctz(m)].cur[a];
         }
+        if (w == rs.op) W.self_read = true;
         if (w < 0 || w == rs.op) continue;
         W.edges.push_back({rs.op, w});
       }
   });
   size_t nedges = 0;
-  for (int r = 0; r < R; r++) nedges += S.w[r].edges.size();
+  bool self_read = false;
+  for (int r = 0; r < R; r++) {
+    nedges += S.w[r].edges.size();
+    self_read |= S.w[r].self_read;
+  }
   // groups = connected components (lock-free union-find, -1: not linked; a
   // root is linked under a smaller one, so every component's root is its
   // smallest op); the tables go back to "unwritten" where this compile
@@ -1397,6 +1424,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
   };
   if (!nedges) {
     pad_to_chunk();
+    I.tail_chunk0 = self_read ? 0 : (int)(I.ops.size() / kFillChunk);
     return status;
   }
   // linked ops of every range's op segment with their roots (all unions are
@@ -1437,6 +1465,7 @@ int compile_instance(Instance& I, int NT, int ng, FillScratch& S, int nthreads =
     for (int32_t g : S.w[r].gid) S.group_count[g] += 1;
   S.group_pos.resize(ngroups);
   const size_t tail = ((size_t)n + kFillChunk - 1) / kFillChunk * kFillChunk;
+  I.tail_chunk0 = self_read ? 0 : (int)(tail / kFillChunk);
   size_t pos = tail;
   for (int32_t g = 0; g < ngroups; g++) {
     const size_t m = (size_t)S.group_count[g];
@@ -1926,6 +1955,7 @@ void swap_program(Instance& I, FillProg& P) {
   std::swap(I.cap_gen, P.cap_gen);
   std::swap(I.ngen, P.ngen);
   std::swap(I.op_kinds, P.kinds);
+  std::swap(I.tail_chunk0, P.tail_chunk0);
   std::swap(I.prog_key, P.key);
   std::swap(I.prog_hash, P.hash);
 }
@@ -2146,7 +2176,9 @@ int rebuild_topology(amr_ctx* c) {
   const size_t off_ops = 256;
   const size_t off_gen = off_ops + align(sizeof(FillOp*) * (ni + 1));
   const size_t off_pre = off_gen + align(sizeof(FillGen*) * (ni + 1));
-  const size_t off_rgi = off_pre + align(sizeof(int32_t) * (ni + 1));
+  const size_t off_tops = off_pre + align(sizeof(int32_t) * (ni + 1));
+  const size_t off_tpre = off_tops + align(sizeof(FillOp*) * (ni + 1));
+  const size_t off_rgi = off_tpre + align(sizeof(int32_t) * (ni + 1));
   const size_t off_leaf = off_rgi + align(sizeof(RegridInst) * ni);
   const size_t off_split = off_leaf + align(sizeof(LeafItem) * std::max(1, c->nleaf));
   const size_t topo_bytes = off_split + sizeof(LeafItem) * std::max(1, c->nleaf);
@@ -2165,16 +2197,24 @@ int rebuild_topology(amr_ctx* c) {
   const FillOp** h_ops = (const FillOp**)(c->h_topo + off_ops);
   const FillGen** h_gen = (const FillGen**)(c->h_topo + off_gen);
   int32_t* h_pre = (int32_t*)(c->h_topo + off_pre);
-  int total = 0;
+  const FillOp** h_tops = (const FillOp**)(c->h_topo + off_tops);
+  int32_t* h_tpre = (int32_t*)(c->h_topo + off_tpre);
+  int total = 0, ttotal = 0;
   h_pre[0] = 0;
+  h_tpre[0] = 0;
   for (int q = 0; q < ni; q++) {
     const Instance& I = c->inst[q];
     h_ops[q] = I.d_ops;
     h_gen[q] = I.d_gen;
     total += I.done ? 0 : I.nops / kFillChunk;
     h_pre[q + 1] = total;
+    const int t0 = std::min(std::max(I.tail_chunk0, 0), I.nops / kFillChunk);
+    h_tops[q] = I.d_ops + (size_t)t0 * kFillChunk;
+    ttotal += I.done ? 0 : I.nops / kFillChunk - t0;
+    h_tpre[q + 1] = ttotal;
   }
   c->nchunks = total;
+  c->ntail_chunks = ttotal;
   // the device regrid's per-instance parameters; device trees that are stale
   // are refreshed from the host trees first
   RegridInst* h_rgi = (RegridInst*)(c->h_topo + off_rgi);
@@ -2213,6 +2253,8 @@ int rebuild_topology(amr_ctx* c) {
   c->d_inst_ops = (const FillOp**)(c->d_topo + off_ops);
   c->d_inst_gen = (const FillGen**)(c->d_topo + off_gen);
   c->d_prefix = (int32_t*)(c->d_topo + off_pre);
+  c->d_tail_ops = (const FillOp**)(c->d_topo + off_tops);
+  c->d_tail_prefix = (int32_t*)(c->d_topo + off_tpre);
   c->d_rgi = (RegridInst*)(c->d_topo + off_rgi);
   c->d_leaves = (LeafItem*)(c->d_topo + off_leaf);
   c->topo_dirty = false;
@@ -2690,7 +2732,11 @@ int enqueue_step(amr_ctx* c) {
     // the step's parameters (dt) ride on the first fill launch
     FillSet s0 = make_set(c, {{kU, kFillCell}, {kBi, kFillFace}});
     static_assert(sizeof(InstParams) % sizeof(double) == 0, "parameters copied as 8-byte words");
-    if (c->grid_chunks > 0) {
+    // U and Bi as the previous step's last fills of the same programs left
+    // them (fill_fresh): only the programs' linked tails change anything
+    const bool tail = c->tail_first_fill;
+    const int chunks = tail ? c->ntail_chunks : c->grid_chunks;
+    if (chunks > 0) {
       s0.copy_words = (int32_t)(sizeof(InstParams) * c->inst.size() / sizeof(double));
       s0.copy_src = (const double*)c->h_ip.data();
       s0.copy_dst = (double*)c->d_ip;
@@ -2698,7 +2744,13 @@ int enqueue_step(amr_ctx* c) {
       HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstParams) * c->inst.size(), hipMemcpyHostToDevice,
                             c->stream));
     }
-    run_fill(c, s0);
+    if (tail) {
+      Timed tm(c, PF_FILL);
+      launch_fill(c->NT, c->d_tail_ops, c->d_inst_gen, c->d_tail_prefix, (int)c->inst.size(), chunks, s0, c->stream,
+                  nullptr);
+    } else {
+      run_fill(c, s0);
+    }
   }
   run_stage(c, 1);
   run_fill(c, make_set(c, {{kBi, kFillFace}, {kFF, kFillFace}, {kGG, kFillFace}}));
@@ -2760,11 +2812,16 @@ int capture_step(amr_ctx* c) {
 // trees changed, then launch the step's device work (graph or direct) on the
 // context's stream and return without waiting.
 int step_begin(amr_ctx* c) {
+  bool fresh = c->fill_fresh;
+  c->fill_fresh = false;
+  c->step_direct = false;
   c->call_fail = 0;
   c->in_flight = false;
   if (!running(c)) return 0;
-  if (c->topo_dirty)
+  if (c->topo_dirty) {
+    fresh = false;
     if (int e = rebuild_topology(c)) return e;
+  }
   fill_params(c);
   const bool device_timing = (c->prof_mask & 0x1fu) != 0;
   if (persistent_step(c)) {
@@ -2800,7 +2857,11 @@ int step_begin(amr_ctx* c) {
     HIPCHK(le);
     if (c->prof_mask & (1u << PF_GRAPH)) c->prof[PF_GRAPH].launches += 1;
   } else {
-    if (int e = enqueue_step(c)) return e;
+    c->tail_first_fill = fresh;
+    const int e = enqueue_step(c);
+    c->tail_first_fill = false;
+    if (e) return e;
+    c->step_direct = true;
     c->dma_since_direct = false;
   }
   c->steps_same_topo++;
@@ -2811,6 +2872,7 @@ int step_begin(amr_ctx* c) {
 // Second half: wait for the step, advance time and dt (main.cpp:231-238),
 // regrid (admesh) and retire finished instances.
 int step_end(amr_ctx* c, int* n_active) {
+  c->fill_fresh = false;
   if (!c->in_flight) {
     if (n_active) *n_active = running(c);
     return running(c) ? 0 : c->diag.code;
@@ -2843,8 +2905,15 @@ int step_end(amr_ctx* c, int* n_active) {
     }
     active += I.done ? 0 : 1;
   }
+  c->fill_fresh = c->step_direct && !c->topo_dirty && !c->regrid_launched && !c->call_fail;
   if (n_active) *n_active = active;
   return (active == 0 && c->call_fail) ? c->call_fail : 0;
+}
+
+// every entry point other than the step calls and the read-only queries:
+// the next step runs its first fill whole
+inline void fill_touched(amr_ctx* c) {
+  if (c) c->fill_fresh = false;
 }
 
 int step_all(amr_ctx* c, int* n_active) {
@@ -3056,6 +3125,7 @@ int amr_destroy(amr_ctx* c) {
 // basegrid.cpp:4-89 + IC2Dtype.cpp + main.cpp:97-110
 int amr_init(amr_ctx* c) {
   AMR_TRY
+  fill_touched(c);
   if (!c) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   static const int32_t root_lp[4][8] = {{0, 0, 0, 0, -1, 1, -1, 2}, {0, 0, 0, 0, 0, -1, -1, 3},
@@ -3250,6 +3320,7 @@ int amr_download(amr_ctx* c, int inst, int field, int nslots, double* host) {
 
 int amr_upload(amr_ctx* c, int inst, int field, int nslots, const double* host) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || field < 0 || field >= kNumFields || !host) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   // only blocks the instance uses (amr_init, amr_set_tree, regrids) have slots
@@ -3280,6 +3351,7 @@ int amr_get_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* ic, int32_t*
 
 int amr_set_tree(amr_ctx* c, int inst, int n, const int32_t* lp, const int32_t* ic, const int32_t* lf) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || c->in_flight) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   if (int e = validate_tree(n, I.maxblocks, I.maxlevs, c->nxi, lp, ic, lf)) return e;
@@ -3317,6 +3389,7 @@ int amr_validate_tree(int nx, int maxlevs, int n, const int32_t* lp, const int32
 
 int amr_set_time(amr_ctx* c, int inst, double t, double dt, double dt_next, int32_t count) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst)) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   I.t = t;
@@ -3463,6 +3536,7 @@ int amr_checkpoint_save(amr_ctx* c, int inst, const char* path) {
 
 int amr_checkpoint_load(amr_ctx* c, int inst, const char* path) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || !path || c->in_flight) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   std::unique_ptr<std::FILE, FileCloser> fp(std::fopen(path, "rb"));
@@ -3510,6 +3584,7 @@ int amr_checkpoint_load(amr_ctx* c, int inst, const char* path) {
 
 int amr_fill(amr_ctx* c, int field, int kind) {
   AMR_TRY
+  fill_touched(c);
   if (!c || field < 0 || field >= kNumFields || kind < 0 || kind > 2) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -3521,6 +3596,7 @@ int amr_fill(amr_ctx* c, int field, int kind) {
 
 int amr_stage(amr_ctx* c, int step) {
   AMR_TRY
+  fill_touched(c);
   if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -3533,6 +3609,7 @@ int amr_stage(amr_ctx* c, int step) {
 
 int amr_ct(amr_ctx* c, int step) {
   AMR_TRY
+  fill_touched(c);
   if (!c || step < 1 || step > c->rk) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -3545,6 +3622,7 @@ int amr_ct(amr_ctx* c, int step) {
 
 int amr_cfl(amr_ctx* c, int inst, double* lambda) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || !lambda) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -3561,6 +3639,7 @@ int amr_cfl(amr_ctx* c, int inst, double* lambda) {
 
 int amr_regrid(amr_ctx* c) {
   AMR_TRY
+  fill_touched(c);
   if (!c) return AMR_ERR_CONFIG;
   c->call_fail = 0;
   if (c->topo_dirty)
@@ -3574,6 +3653,7 @@ int amr_regrid(amr_ctx* c) {
 
 int amr_schedule(amr_ctx* c, int inst, int cap, int32_t* ib3, int* n) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || !n) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   build_schedule(I);
@@ -3607,6 +3687,7 @@ int amr_recon_batch(int NT, int nghosts, const char* limiter, double dx, int nt,
 
 int amr_set_stage_variant(amr_ctx* c, int variant) {
   AMR_TRY
+  fill_touched(c);
   if (!c || variant < -1 || variant >= kStageVariants) return AMR_ERR_CONFIG;
   c->stage_variant = (variant < 0 && c->wall_khz <= 0) ? 0 : variant;
   c->graph_dirty = true;
@@ -3629,6 +3710,7 @@ int amr_stage_tuning(amr_ctx* c, int nbuckets, int32_t* chosen, double* us_per_l
 
 int amr_set_stage_tuning(amr_ctx* c, int nbuckets, const int32_t* chosen) {
   AMR_TRY
+  fill_touched(c);
   if (!c || nbuckets < 0 || nbuckets > 24 || (nbuckets && !chosen)) return AMR_ERR_CONFIG;
   for (int b = 0; b < nbuckets; b++)
     if (chosen[b] < -1 || chosen[b] >= kStageVariants) return AMR_ERR_CONFIG;
@@ -3642,6 +3724,7 @@ int amr_set_stage_tuning(amr_ctx* c, int nbuckets, const int32_t* chosen) {
 
 int amr_set_step_kernel(amr_ctx* c, int mode) {
   AMR_TRY
+  fill_touched(c);
   if (!c || mode < -1 || mode > 2 || c->in_flight) return AMR_ERR_CONFIG;
   c->step_mode = mode < 0 ? 0 : mode;
   return AMR_OK;
@@ -3650,6 +3733,7 @@ int amr_set_step_kernel(amr_ctx* c, int mode) {
 
 int amr_set_regrid_device(amr_ctx* c, int on) {
   AMR_TRY
+  fill_touched(c);
   if (!c || c->in_flight || (on != 0 && on != 1)) return AMR_ERR_CONFIG;
   if (c->regrid_device != (on == 1)) {
     c->regrid_device = on == 1;
@@ -3677,6 +3761,7 @@ int amr_debug_fill_stats(amr_ctx* c, int inst, int64_t* out) {
 // default) stages the trees that fit in LDS
 int amr_debug_regrid_lds(amr_ctx* c, int on) {
   AMR_TRY
+  fill_touched(c);
   if (!c || c->in_flight) return AMR_ERR_CONFIG;
   c->regrid_lds = on != 0;
   c->topo_dirty = true;
@@ -3686,6 +3771,7 @@ int amr_debug_regrid_lds(amr_ctx* c, int on) {
 
 int amr_debug_device_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* leafs, int* current) {
   AMR_TRY
+  fill_touched(c);
   if (check_inst(c, inst) || !lp || !leafs || !current || c->in_flight) return AMR_ERR_CONFIG;
   Instance& I = c->inst[inst];
   if (n < 0 || n > I.maxblocks) return AMR_ERR_CONFIG;
@@ -3704,6 +3790,7 @@ int amr_debug_device_tree(amr_ctx* c, int inst, int n, int32_t* lp, int32_t* lea
 
 int amr_use_stream(amr_ctx* c, void* stream, int caller) {
   AMR_TRY
+  fill_touched(c);
   if (!c || c->in_flight || c->in_group) return AMR_ERR_CONFIG;
   hipStream_t next = caller ? (hipStream_t)stream : c->own_stream;
   if (next == c->stream) return AMR_OK;
@@ -3718,6 +3805,7 @@ int amr_use_stream(amr_ctx* c, void* stream, int caller) {
 
 int amr_set_graph(amr_ctx* c, int on) {
   AMR_TRY
+  fill_touched(c);
   if (!c) return AMR_ERR_CONFIG;
   if (on) {
     // replaying one captured step over changing trees (device-side counts,
@@ -3734,6 +3822,7 @@ int amr_set_graph(amr_ctx* c, int on) {
 
 int amr_set_fill_cache(amr_ctx* c, int entries) {
   AMR_TRY
+  fill_touched(c);
   if (!c || entries < 0 || entries > 64) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->prog_cache_max = entries;
@@ -3750,6 +3839,7 @@ int amr_set_fill_cache(amr_ctx* c, int entries) {
 
 int amr_profile_enable(amr_ctx* c, int mask) {
   AMR_TRY
+  fill_touched(c);
   if (!c) return AMR_ERR_CONFIG;
   HIPCHK(hipStreamSynchronize(c->stream));
   flush_profile(c);
@@ -3823,9 +3913,9 @@ extern "C" {
 // element one chunk writes is read by another chunk.  *nnodes receives the
 // number of nested restriction nodes.  force_general != 0 emits every
 // restriction through the explicit-operand path.
-int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
-                        double* Q, int nv, int kind, int* nnodes, int force_general) {
-  AMR_TRY
+namespace {
+int host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs, double* Q, int nv,
+              int kind, int* nnodes, int force_general, int* tail_chunk0) {
   amr_ctx c;
   c.NT = NT;
   c.ng = ng;
@@ -3869,7 +3959,9 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
   }
   // emulate the device: per chunk, read every value, then write
   std::vector<double> buf((size_t)kFillChunk * 8, 0.0);
-  for (size_t c0 = 0; c0 < m; c0 += kFillChunk) {
+  const size_t first = tail_chunk0 ? (size_t)I.tail_chunk0 * kFillChunk : 0;
+  if (tail_chunk0) *tail_chunk0 = I.tail_chunk0;
+  for (size_t c0 = first; c0 < m; c0 += kFillChunk) {
     for (size_t e = c0; e < c0 + kFillChunk; e++)
       if (ops[e].op != kFillNop)
         for (int k = 0; k < nv; k++) buf[(e - c0) * 8 + k] = hf.value(ops[e], k);
@@ -3881,6 +3973,25 @@ int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t*
   int live = 0;
   for (size_t e = 0; e < m; e++) live += ops[e].op != kFillNop;
   return live;
+}
+}  // namespace
+
+int amr_debug_host_fill(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
+                        double* Q, int nv, int kind, int* nnodes, int force_general) {
+  AMR_TRY
+  return host_fill(NT, ng, n, lp, icoord, leafs, Q, nv, kind, nnodes, force_general, nullptr);
+  AMR_CATCH
+}
+
+// The same program applied from its linked tail on only (the chunks a
+// repeated fill changes); *tail_chunk0 receives the tail's first chunk.
+// Applied to the output of a whole fill, it must give what a second whole
+// fill gives.
+int amr_debug_host_fill_tail(int NT, int ng, int n, const int32_t* lp, const int32_t* icoord, const int32_t* leafs,
+                             double* Q, int nv, int kind, int* tail_chunk0) {
+  AMR_TRY
+  if (!tail_chunk0) return -1;
+  return host_fill(NT, ng, n, lp, icoord, leafs, Q, nv, kind, nullptr, 0, tail_chunk0);
   AMR_CATCH
 }
 

@@ -171,6 +171,46 @@ def test_fill_program_compiler_matches_reference(tmp_path, ng, lv, seed, pr, pc,
         assert np.array_equal(Q, post[fld]), fld
 
 
+@need_ref
+@pytest.mark.parametrize("ng,lv,seed,pr,pc,rounds", [(2, 4, 1, 0.5, 0.2, 5), (2, 6, 7, 0.3, 0.3, 8),
+                                                     (4, 5, 3, 0.4, 0.25, 6), (2, 5, 99, 0.25, 0.5, 10)])
+def test_repeated_fill_needs_only_the_linked_tail(tmp_path, ng, lv, seed, pr, pc, rounds):
+    """A step whose tree did not change starts with a fill of U and Bi that
+    repeats the previous step's last fills of both; the engine runs only the
+    chunks from the program's linked (stale-read) tail on.  On the host
+    emulation of the device program, over the reference's random trees and
+    random values: whole fill, then the tail alone, equals whole fill twice
+    bit for bit; the tail is a small part of the program, and the second
+    whole fill does change values (the stale reads), so skipping it
+    altogether would not do."""
+    from amr2d_mhd_amd._lib import lib
+
+    rc, err = refrun.run(str(tmp_path), ng, lv, ic="BWx", lim="MC", flux="RUSA", randtree=(seed, pr, pc, rounds))
+    assert rc == 0, err
+    pre = refrun.read_state(str(tmp_path / "state_rt_pre.bin"))
+    n, NT = pre["lastActive"], pre["NT"]
+    lp, ic, lf = (np.ascontiguousarray(pre[k], dtype=np.int32) for k in ("lp", "icoord", "leafs"))
+    rng = np.random.default_rng(seed)
+    changed = False
+    for nv, kind in [(8, 0), (2, 1), (1, 2)]:
+        once = rng.standard_normal((n, nv, NT, NT))
+        nops = lib().amr_debug_host_fill(NT, ng, n, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data,
+                                         once.ctypes.data, nv, kind, C.byref(C.c_int(0)), 0)
+        assert nops > 0
+        twice, tail = once.copy(), once.copy()
+        assert lib().amr_debug_host_fill(NT, ng, n, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data,
+                                         twice.ctypes.data, nv, kind, C.byref(C.c_int(0)), 0) == nops
+        t0 = C.c_int(-1)
+        assert lib().amr_debug_host_fill_tail(NT, ng, n, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data,
+                                              tail.ctypes.data, nv, kind, C.byref(t0)) == nops
+        assert np.array_equal(tail, twice)
+        nchunks = lib().amr_debug_compile(NT, ng, n, lp.ctypes.data, ic.ctypes.data, lf.ctypes.data, 0, None,
+                                          C.byref(C.c_int(0))) // 256
+        assert 0 < t0.value and nchunks - t0.value <= nchunks // 4  # 6-13% on these trees
+        changed |= not np.array_equal(twice, once)
+    assert changed
+
+
 def _compile(lp, ic, lf, NT, ng, ranges, threads, reuse):
     from amr2d_mhd_amd._lib import lib
 
