@@ -97,6 +97,33 @@ def _pair(ic, lim, flux, cfl, maxlevs, nghosts=2):
     return sim, dom
 
 
+def test_upload_between_steps_makes_the_next_first_fill_whole():
+    """After a step that leaves the tree unchanged, the next step's first fill
+    of U and Bi runs only the fill programs' linked tails -- valid because U
+    and Bi still hold the last fills' output.  An amr_upload in between
+    breaks that: here it raises the total energy of every interior cell, the
+    same on the oracle, so the next step needs every ghost cell refilled; the
+    run must stay bitwise on the oracle."""
+    sim, dom = _pair("rotor", "MC", "HLLD", 0.5, 4)
+    lockstep(sim, dom, 20, full_every=10, label="rotor lv4 before")
+    for _ in range(200):  # reach a step after which the tree is unchanged
+        before = [a.copy() for a in sim.get_tree()]
+        lockstep(sim, dom, 1, full_every=1, label="rotor lv4 search")
+        after = sim.get_tree()
+        if all(x.shape == y.shape and np.array_equal(x, y) for x, y in zip(before, after)):
+            break
+    else:
+        pytest.fail("no step without a regrid op in 200 steps")
+    ng = 2
+    U = dom.get_field(O.U)
+    U[:, 4, ng:-ng, ng:-ng] *= 1.001
+    assert dom.set_field(O.U, U) == 0
+    sim.upload("U", U)
+    d = compare_states(gpu_state(sim), oracle_state(dom))
+    assert not d, "uploaded state differs: " + "; ".join(d[:4])
+    lockstep(sim, dom, 10, full_every=1, label="rotor lv4 after upload")
+
+
 def test_brio_wu_lv4_full_run_bitwise():
     """Config 1: BWx, 4 levels, MC + Rusanov, CT, CFL 0.4, to t=0.2 (453 steps)."""
     sim, dom = _pair("BWx", "MC", "RUSA", 0.4, 4)
