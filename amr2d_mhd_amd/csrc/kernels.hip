@@ -628,10 +628,54 @@ __device__ inline void fill_chunk(const FillProgram& P, const FillSet& set, int 
   }
 }
 
+// restrict_combine for a kind fixed at compile time (the same IEEE
+// operations, no branch on the kind between an op's loads)
+template <int KIND>
+__device__ inline double restrict_kind(int k, double v00, double v01, double v10, double v11) {
+  double v = 0.;
+  if constexpr (KIND == kFillCell) {
+    v += v00;
+    v += v01;
+    v += v10;
+    v += v11;
+    return 0.25 * v;
+  } else if constexpr (KIND == kFillFace) {
+    v += v00;
+    v += k == 0 ? v01 : v10;
+    return 0.5 * v;
+  } else {
+    return v00;
+  }
+}
+
+// the NV components of one field of combine kind KIND for an op whose
+// operands are the elements e[t] (block slot << 16 | cell): one (copy) or
+// four (restriction, 2x2 order)
+template <int NV, int KIND, bool COPY>
+__device__ inline void fill_gather(const double* __restrict__ base, int N2, const int64_t* e, double* v) {
+  if constexpr (COPY) {
+    const double* p = base + (size_t)(e[0] >> 16) * NV * N2 + (e[0] & 0xffff);
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = p[(size_t)k * N2];
+  } else {
+    const double* p0 = base + (size_t)(e[0] >> 16) * NV * N2 + (e[0] & 0xffff);
+    const double* p1 = base + (size_t)(e[1] >> 16) * NV * N2 + (e[1] & 0xffff);
+    const double* p2 = base + (size_t)(e[2] >> 16) * NV * N2 + (e[2] & 0xffff);
+    const double* p3 = base + (size_t)(e[3] >> 16) * NV * N2 + (e[3] & 0xffff);
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      v[k] = restrict_kind<KIND>(k, p0[(size_t)k * N2], p1[(size_t)k * N2], p2[(size_t)k * N2], p3[(size_t)k * N2]);
+  }
+}
+
 // The same for a set of up to three fields with NV0, NV1, NV2 components
-// (0: absent), fixed at compile time: every value of every field of the chunk
-// is read before the one barrier, then all are written.
-template <int NV0, int NV1, int NV2>
+// (0: absent) and combine kinds K0, K1, K2, fixed at compile time: every value
+// of every field of the chunk is read before the one barrier, then all are
+// written.  The op's kind and operand elements are decoded once, outside the
+// loops over fields and components, so that all of an op's loads are in
+// flight together (branches per component made each component's loads wait
+// for the previous ones).
+template <int NV0, int NV1, int NV2, int K0, int K1, int K2>
 __device__ inline void fill_chunk_nv(const FillProgram& P, const FillSet& set, int NT, int w, int lane, bool valid) {
   FillOp op{};
   op.op = kFillNop;
@@ -641,19 +685,31 @@ __device__ inline void fill_chunk_nv(const FillProgram& P, const FillSet& set, i
     op = P.ops[q][(w - P.cprefix[q]) * kFillChunk + lane];
   }
   const bool live = op.op != kFillNop;
-  const FillGen* gen = valid ? P.gen[q] : nullptr;
   const int N2 = NT * NT;
-  const FillCtx f0{set.base[0], NV0, set.kind[0], N2, NT, gen};
-  const FillCtx f1{set.base[1], NV1, set.kind[1], N2, NT, gen};
-  const FillCtx f2{set.base[2], NV2, set.kind[2], N2, NT, gen};
   double v0[NV0 > 0 ? NV0 : 1], v1[NV1 > 0 ? NV1 : 1], v2[NV2 > 0 ? NV2 : 1];
   if (live) {
+    int64_t e[4];
+    if (op.op == kFillCopy) {
+      e[0] = src_ref(op.src, op.src_cell);
+      if constexpr (NV0 > 0) fill_gather<NV0, K0, true>(set.base[0], N2, e, v0);
+      if constexpr (NV1 > 0) fill_gather<NV1, K1, true>(set.base[1], N2, e, v1);
+      if constexpr (NV2 > 0) fill_gather<NV2, K2, true>(set.base[2], N2, e, v2);
+    } else {
+      if (op.op == kFillRestrict) {
+        const int a = op.src_cell;
+        e[0] = src_ref(op.src, a);
+        e[1] = src_ref(op.src, a + NT);
+        e[2] = src_ref(op.src, a + 1);
+        e[3] = src_ref(op.src, a + 1 + NT);
+      } else {
+        const FillGen& g = P.gen[q][op.src];
 #pragma unroll
-    for (int k = 0; k < NV0; k++) v0[k] = fill_value(f0, op, k);
-#pragma unroll
-    for (int k = 0; k < NV1; k++) v1[k] = fill_value(f1, op, k);
-#pragma unroll
-    for (int k = 0; k < NV2; k++) v2[k] = fill_value(f2, op, k);
+        for (int t = 0; t < 4; t++) e[t] = g.ref[t];
+      }
+      if constexpr (NV0 > 0) fill_gather<NV0, K0, false>(set.base[0], N2, e, v0);
+      if constexpr (NV1 > 0) fill_gather<NV1, K1, false>(set.base[1], N2, e, v1);
+      if constexpr (NV2 > 0) fill_gather<NV2, K2, false>(set.base[2], N2, e, v2);
+    }
   }
   __syncthreads();  // every read of the chunk precedes every write
   if (live) {
@@ -666,15 +722,16 @@ __device__ inline void fill_chunk_nv(const FillProgram& P, const FillSet& set, i
   }
 }
 
-// field sets of the step (engine.cpp): {U, Bi}, {Bi, ff, gg}, {ff, gg}, one
-// cell field, one face field, EMF; any other set takes the per-field path
-template <int NV0, int NV1, int NV2>
+// field sets of the step (engine.cpp) with their combine kinds: {U, Bi},
+// {Bi, ff, gg}, {ff, gg}, one cell field, one face field, EMF; any other set
+// takes the per-field path
+template <int NV0, int NV1, int NV2, int K0, int K1, int K2>
 __global__ __launch_bounds__(kThreads) void fill_kernel_nv(FillProgram P, FillSet set, int NT,
                                                            const int32_t* __restrict__ counts) {
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < set.copy_words; i += gridDim.x * kThreads)
     set.copy_dst[i] = set.copy_src[i];
   if (counts && (int)blockIdx.x >= counts[kCountChunks]) return;
-  fill_chunk_nv<NV0, NV1, NV2>(P, set, NT, blockIdx.x, threadIdx.x, true);
+  fill_chunk_nv<NV0, NV1, NV2, K0, K1, K2>(P, set, NT, blockIdx.x, threadIdx.x, true);
 }
 
 __global__ __launch_bounds__(kThreads) void fill_kernel(FillProgram P, FillSet set, int NT,
@@ -1197,12 +1254,19 @@ void launch_fill(int NT, const FillOp* const* inst_ops, const FillGen* const* in
   if (nchunks <= 0 || set.n <= 0) return;
   FillProgram P{inst_ops, inst_gen, cprefix, ninst};
   const int a = set.nv[0], b = set.n > 1 ? set.nv[1] : 0, c = set.n > 2 ? set.nv[2] : 0;
-  if (set.n == 1 && a == 8) fill_kernel_nv<8, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
-  else if (set.n == 1 && a == 2) fill_kernel_nv<2, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
-  else if (set.n == 1 && a == 1) fill_kernel_nv<1, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
-  else if (set.n == 2 && a == 8 && b == 2) fill_kernel_nv<8, 2, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
-  else if (set.n == 2 && a == 8 && b == 8) fill_kernel_nv<8, 8, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
-  else if (set.n == 3 && a == 2 && b == 8 && c == 8) fill_kernel_nv<2, 8, 8><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  const int ka = set.kind[0], kb = set.n > 1 ? set.kind[1] : -1, kc = set.n > 2 ? set.kind[2] : -1;
+  constexpr int C = kFillCell, F = kFillFace, E = kFillEdge;
+  if (set.n == 1 && a == 8 && ka == C) fill_kernel_nv<8, 0, 0, C, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 1 && a == 2 && ka == F)
+    fill_kernel_nv<2, 0, 0, F, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 1 && a == 1 && ka == E)
+    fill_kernel_nv<1, 0, 0, E, 0, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 2 && a == 8 && b == 2 && ka == C && kb == F)
+    fill_kernel_nv<8, 2, 0, C, F, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 2 && a == 8 && b == 8 && ka == F && kb == F)
+    fill_kernel_nv<8, 8, 0, F, F, 0><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
+  else if (set.n == 3 && a == 2 && b == 8 && c == 8 && ka == F && kb == F && kc == F)
+    fill_kernel_nv<2, 8, 8, F, F, F><<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
   else fill_kernel<<<nchunks, kThreads, 0, s>>>(P, set, NT, counts);
 }
 
