@@ -137,14 +137,29 @@ __device__ inline void stage_face(const double* sWb, int VS, const double* __res
   if (nan) report(err, L.inst, error_key(step, L.nb, 2, i, j, ydir ? 4 : 1, L.nb, kErrNanFlux));
 }
 
-// RK update of component k of cell e of the tile from its residual
-// (main.cpp:147-157 / 179-189; SSP-RK3 build-defined)
-__device__ inline void stage_rk(int step, int rk, double dt, double res, const double* U, double* Us, double* Uw,
-                                int e) {
-  if (step == 1) Us[e] = U[e] - dt * res;
-  else if (rk == 2) Uw[e] = 0.5 * (U[e] + Us[e] - dt * res);
-  else if (step == 2) Us[e] = 0.75 * U[e] + 0.25 * (Us[e] - dt * res);
-  else Uw[e] = (1. / 3.) * U[e] + (2. / 3.) * (Us[e] - dt * res);
+// RK update of the eight components of cell c of the tile from their
+// residuals (main.cpp:147-157 / 179-189; SSP-RK3 build-defined).  Every input
+// is loaded before the first store: U, Us and Uw may alias (Uw is U), so a
+// store between two components' loads would hold each component's loads
+// back until the previous component's store; the cell's elements are read
+// and written by this lane only, so the order changes no value.
+__device__ inline void stage_rk_cell(int step, int rk, double dt, const double* res, const double* U, double* Us,
+                                     double* Uw, int c, int N2) {
+  double u[8], us[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) u[k] = U[k * N2 + c];
+  if (step != 1) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) us[k] = Us[k * N2 + c];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int e = k * N2 + c;
+    if (step == 1) Us[e] = u[k] - dt * res[k];
+    else if (rk == 2) Uw[e] = 0.5 * (u[k] + us[k] - dt * res[k]);
+    else if (step == 2) Us[e] = 0.75 * u[k] + 0.25 * (us[k] - dt * res[k]);
+    else Uw[e] = (1. / 3.) * u[k] + (2. / 3.) * (us[k] - dt * res[k]);
+  }
 }
 
 // One leaf, all TPB lanes of the workgroup; sW/sF/sG/sNan: the workgroup's
@@ -212,24 +227,25 @@ __device__ inline void stage_block(const FieldPtrs& F, const LeafItem L, const I
   for (int c = tid; c < N2; c += TPB) {
     const int i = c % NT, j = c / NT;
     const bool in = i >= LO && i <= HI && j >= LO && j <= HI;
+    double res[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      double res = 0.;
+      res[k] = 0.;
       if (in) {
         if (FLDS) {
           const double* Fk = sF + k * NF * (NF + 1);
           const double* Gk = sG + k * NF * (NF + 1);
           const int fi = (j - LO) * (NF + 1) + (i - LO);
           const int gi = (j - LO) * NF + (i - LO);
-          res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+          res[k] = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
         } else {
           const double* Fk = ff + k * N2;
           const double* Gk = gg + k * N2;
-          res = (Fk[c + 1] - Fk[c]) / dxl + (Gk[c + NT] - Gk[c]) / dxl;
+          res[k] = (Fk[c + 1] - Fk[c]) / dxl + (Gk[c + NT] - Gk[c]) / dxl;
         }
       }
-      stage_rk(step, P.rk, dt, res, U, Us, Uw, k * N2 + c);
     }
+    stage_rk_cell(step, P.rk, dt, res, U, Us, Uw, c, N2);
   }
 }
 
@@ -367,18 +383,19 @@ __global__ __launch_bounds__((StripGeom<NT, NG, K>::kLanes)) void stage_strip_ke
   for (int c = r0 * NT + tid; c <= r1 * NT + NT - 1; c += TPB) {
     const int i = c % NT, j = c / NT;
     const bool in = i >= LO && i <= HI && j >= LO && j <= HI;
+    double res[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      double res = 0.;
+      res[k] = 0.;
       if (in) {
         const double* Fk = sF + k * XR * (NF + 1);
         const double* Gk = sG + k * (XR + 1) * NF;
         const int fi = (j - jr0) * (NF + 1) + (i - LO);
         const int gi = (j - jr0) * NF + (i - LO);
-        res = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
+        res[k] = (Fk[fi + 1] - Fk[fi]) / dxl + (Gk[gi + NF] - Gk[gi]) / dxl;
       }
-      stage_rk(step, P.rk, dt, res, U, Us, Uw, k * N2 + c);
     }
+    stage_rk_cell(step, P.rk, dt, res, U, Us, Uw, c, N2);
   }
   if (span) {
     __syncthreads();
